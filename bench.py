@@ -1,0 +1,159 @@
+"""Benchmark of the three-body FS2+FS3 energy+forces call on one MI355X.
+
+Workload: 512 rigid waters, L = 24.84 A (0.997 g/cm^3), 8x8x8 jittered lattice
+with random orientations (seeded), FS2+FS3 variant with the shipped fs2fs3
+parameters.  One step = one full call (list build, FS2, FS3, reductions) with
+inputs resident in HBM.  The metric is FS3 triplet evaluations per second of
+the mixed-precision path; the FP64 path is timed beside it, the force error of
+mixed vs FP64 is reported, and the CPU oracle is timed on the host cores.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "3-body triplet evals/s (E+F), 512 H2O: mixed vs FP64 at force RMS err<1e-5"
+
+# Algorithmic work per unit (fixed figures): one FS3 region evaluation of one
+# triplet is 217,139 flops + 4,340 transcendentals = 221,479 ops; a triplet
+# with 4.5 < R_avg < 5.5 A is two region evaluations.
+W3_OPS = 221_479
+PEAK_FP32_VECTOR_TFLOPS = 157.3   # MI355X datasheet, packed FP32 vector
+PEAK_FP64_VECTOR_TFLOPS = 78.6    # MI355X datasheet, FP64 vector
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # 256 CU x 64 lanes x 2 x 2.4 GHz, one FMA per lane per clock
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=2025)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def timed_steps(tb, com, site, box, steps, warmup):
+    import torch
+    fs = torch.zeros_like(site)
+    c2 = torch.zeros_like(com)
+    c3 = torch.zeros_like(com)
+    en = torch.zeros(3, dtype=torch.float64, device=com.device)
+    for _ in range(warmup):
+        tb.eval_device(com, site, box, fs, c2, c3, en)
+    torch.cuda.synchronize()
+    ms_fs3 = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tb.eval_device(com, site, box, fs, c2, c3, en)
+        ms_fs3.append(tb.stats()["ms_fs3"])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1e3 / steps, float(np.mean(ms_fs3)), tb.stats()
+
+
+def cpu_baseline(p, box, threads):
+    from oracle import oracle as O
+    log(f"cpu baseline: oracle (faithful per-triplet table rebuild), {threads} threads, full 512-water call")
+    t0 = time.perf_counter()
+    r = O.threebody_term(p, box, nthreads=threads, rebuild=True)
+    wall = time.perf_counter() - t0
+    rate = r["n_fs3"] / wall
+    return {"value": rate, "unit": "triplet evals/s", "cores": threads, "kind": "port",
+            "sample": (f"one full FS2+FS3 E+F call at 512 waters ({r['n_fs3']} FS3 triplets, {r['n_fs2']} FS2 "
+                       f"items) of the C FP64 restatement in the reference's loop order with the per-triplet "
+                       f"orbit-table rebuild, {wall:.2f} s wall"),
+            "ms_per_call": wall * 1e3}, r
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the three-body path is "
+              "measured on a single MI355X and shards by triplet-list chunks (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from ccpol23_plus_2025_mbmd_amd.geometry import lattice_box
+    from ccpol23_plus_2025_mbmd_amd.params import load_variant
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+
+    assert torch.cuda.is_available(), "needs a GPU"
+    dev = torch.device("cuda:0")
+    p = load_variant("fs2fs3")
+    box = lattice_box(args.n, seed=args.seed)
+    com = torch.tensor(box.com.T.copy(), dtype=torch.float64, device=dev).contiguous()
+    site = torch.tensor(np.stack([box.xs.reshape(box.nm, box.ld), box.ys.reshape(box.nm, box.ld),
+                                  box.zs.reshape(box.nm, box.ld)]), dtype=torch.float64, device=dev).contiguous()
+
+    tb_mix = ThreeBody(p, "mixed")
+    tb_64 = ThreeBody(p, "fp64")
+    log(f"box: {box.nm} waters, L={box.box:.4f} A")
+    ms_mix, ms_fs3_mix, st_mix = timed_steps(tb_mix, com, site, box.box, args.steps, args.warmup)
+    log(f"mixed: {ms_mix:.3f} ms/step, FS3 kernel {ms_fs3_mix:.3f} ms, stats {st_mix}")
+    ms_64, ms_fs3_64, st_64 = timed_steps(tb_64, com, site, box.box, args.steps, args.warmup)
+    log(f"fp64:  {ms_64:.3f} ms/step, FS3 kernel {ms_fs3_64:.3f} ms")
+
+    # force error of mixed vs FP64 on this snapshot
+    r64 = tb_64.evaluate(box)
+    rmx = tb_mix.evaluate(box)
+    f64 = np.stack([r64["fxs"][:, :17], r64["fys"][:, :17], r64["fzs"][:, :17]])
+    fmx = np.stack([rmx["fxs"][:, :17], rmx["fys"][:, :17], rmx["fzs"][:, :17]])
+    ferr = float(np.sqrt(((fmx - f64) ** 2).sum() / (f64 ** 2).sum()))
+    eerr = float(abs((rmx["qfs2"] + rmx["qfs3"]) - (r64["qfs2"] + r64["qfs3"])) / abs(r64["qfs2"] + r64["qfs3"]))
+
+    n3 = st_mix["n_fs3"]
+    nreg = st_mix["n_fs3_sr"] + st_mix["n_fs3_lr"] + 2 * st_mix["n_fs3_switch"]
+    value = n3 / (ms_mix * 1e-3)
+    value_64 = n3 / (ms_64 * 1e-3)
+    ops = nreg * W3_OPS
+    achieved = ops / (ms_fs3_mix * 1e-3) / 1e12
+    achieved_64 = ops / (ms_fs3_64 * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "triplet evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_mix, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"{box.nm} rigid H2O, L={box.box:.2f} A, jittered 8x8x8 lattice, random "
+                               f"quaternions (seed {args.seed}), FS2+FS3 fs2fs3 parameters, mixed precision "
+                               f"(FP32 exp/polynomial, FP64 displacements, switch and reductions)",
+                   "n_molecules": box.nm, "n_fs3_triplets": n3, "n_fs2_items": st_mix["n_fs2_items"],
+                   "region_hist_sr_switch_lr": [st_mix["n_fs3_sr"], st_mix["n_fs3_switch"], st_mix["n_fs3_lr"]]},
+        "roofline": {"bound": "valu", "kernel": "k_fs3_eval<float>", "achieved": achieved,
+                     "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VECTOR_TFLOPS,
+                     "traffic": None, "ops_per_launch": ops, "ms_per_launch": ms_fs3_mix,
+                     "peak_fp32_nonpacked": PEAK_FP32_NONPACKED_TFLOPS},
+        "fp64": {"value": value_64, "ms_per_step": ms_64, "fs3_kernel_ms": ms_fs3_64,
+                 "achieved_tflops": achieved_64, "frac_of_fp64_vector_peak": achieved_64 / PEAK_FP64_VECTOR_TFLOPS},
+        "mixed_over_fp64": value / value_64,
+        "mixed_vs_fp64_error": {"site_force_rel_rms": ferr, "energy_rel": eerr},
+        "stats_mixed_ms": {k: st_mix[k] for k in ("ms_lists", "ms_fs2", "ms_fs3", "ms_reduce", "ms_total")},
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cb, _ = cpu_baseline(p, box, threads)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,678 @@
+// Host side of libccpol3b.so: the C ABI declared in include/ccpol3b.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ccpol3b.h"
+#include "ccpol3b_internal.h"
+
+namespace ccpol3b {
+// kernels (lists.hip, eval.hip)
+__global__ void k_prep(int, int, const double*, const double*, const double*, const double*,
+                       const double*, const double*, Dev);
+__global__ void k_neighbours(Dev, Consts);
+__global__ void k_fs3_list(Dev, Consts, int);
+__global__ void k_fs2_pairs(Dev, int);
+__global__ void k_fs2_items(Dev, Consts, int);
+__global__ void k_scan(Dev, int);
+template <typename T>
+__global__ void k_fs3_eval(Dev, Consts, const double4*, const T*);
+__global__ void k_fs3_gather(Dev, int, double*, double*, double*, double*);
+__global__ void k_fs2_pair(Dev, Consts);
+__global__ void k_fs2_spec(Dev, Consts);
+__global__ void k_fs2_gather(Dev, int, double*, double*, double*, double*);
+__global__ void k_energy(Dev, int, int, double*);
+}  // namespace ccpol3b
+
+using namespace ccpol3b;
+
+namespace {
+
+// Orbit numbering of the FS3 linear coefficients: the label of a site is
+// (type, power of the distance opposite that site); a coefficient belongs to
+// the multiset of the three labels.  Multisets are numbered 1..364 in order of
+// first appearance in the loop site_i -> site_j -> site_k -> p_i -> p_j -> p_k,
+// which is the numbering threebody_potential.f90:909-946 produces with fill.
+struct OrbitMap {
+  std::array<int, 17 * 17 * 17 * 8> idx{};  // [(is*17+js)*17+ks][ip*4+jp*2+kp]
+  int count = 0;
+  OrbitMap() {
+    std::map<std::array<int, 3>, int> seen;
+    for (int is = 0; is < 17; ++is)
+      for (int js = 0; js < 17; ++js)
+        for (int ks = 0; ks < 17; ++ks)
+          for (int ip = 0; ip < 2; ++ip)
+            for (int jp = 0; jp < 2; ++jp)
+              for (int kp = 0; kp < 2; ++kp) {
+                std::array<int, 3> key = {site_type(is) * 2 + ip, site_type(js) * 2 + jp,
+                                          site_type(ks) * 2 + kp};
+                std::sort(key.begin(), key.end());
+                auto it = seen.find(key);
+                int n;
+                if (it == seen.end()) {
+                  n = (int)seen.size() + 1;
+                  seen.emplace(key, n);
+                } else {
+                  n = it->second;
+                }
+                idx[((is * 17 + js) * 17 + ks) * 8 + ip * 4 + jp * 2 + kp] = n;
+              }
+    count = (int)seen.size();
+  }
+  int operator()(int is, int js, int ks, int ip, int jp, int kp) const {
+    return idx[((is * 17 + js) * 17 + ks) * 8 + ip * 4 + jp * 2 + kp];
+  }
+};
+
+const OrbitMap& orbit_map() {
+  static OrbitMap m;
+  return m;
+}
+
+int first_site_of_type(int t) {
+  for (int s = 0; s < 17; ++s)
+    if (site_type(s) == t) return s;
+  return -1;
+}
+
+template <typename P>
+void dev_free(P*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct ccpol3b_ctx {
+  int device = 0;
+  int precision = CCPOL3B_FP64;
+  int s2 = 1, s3 = 1;
+  Consts K{};
+  std::string err;
+  ccpol3b_stats stats{};
+  // parameters on the device
+  double4* d_pairpar = nullptr;  // [2][289]
+  double* d_coef64 = nullptr;    // [2][NCOEF]
+  float* d_coef32 = nullptr;
+  int cu_count = 256;
+  // evaluation buffers
+  Dev D{};
+  int cap_nm = 0, cap_box = 0;
+  int* h_totals = nullptr;  // pinned
+  // staging for the host entry point
+  double* d_in = nullptr;   // x,y,z,xs,ys,zs
+  double* d_out = nullptr;  // fxs,fys,fzs (ld=17), fcom2, fcom3, energies
+  size_t cap_in = 0, cap_out = 0;
+  std::vector<int> h_molbox, h_boxstart;
+  std::vector<double> h_boxL;
+  hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+int fail(ccpol3b_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(ctx, CCPOL3B_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define LAUNCHCHK(ctx, what)                                                                  \
+  do {                                                                                        \
+    hipError_t e_ = hipGetLastError();                                                        \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(ctx, CCPOL3B_EHIP, std::string("launch ") + what + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename P>
+int dev_alloc(ccpol3b_ctx* c, P*& p, size_t n) {
+  dev_free(p);
+  if (n == 0) n = 1;
+  if (hipMalloc((void**)&p, n * sizeof(P)) != hipSuccess) {
+    p = nullptr;
+    return fail(c, CCPOL3B_ENOMEM, "device allocation failed");
+  }
+  return CCPOL3B_OK;
+}
+
+void free_dev(ccpol3b_ctx* c) {
+  Dev& D = c->D;
+  dev_free(D.cx); dev_free(D.cy); dev_free(D.cz); dev_free(D.site);
+  dev_free(D.mol_box); dev_free(D.boxL); dev_free(D.box_start);
+  dev_free(D.nb3_cnt); dev_free(D.nb3); dev_free(D.nb2_cnt); dev_free(D.nb2);
+  dev_free(D.cnt3); dev_free(D.off3); dev_free(D.tri);
+  dev_free(D.pcnt2); dev_free(D.poff2); dev_free(D.pairs2);
+  dev_free(D.cnt2); dev_free(D.off2); dev_free(D.items2);
+  dev_free(D.slot3); dev_free(D.ew3); dev_free(D.pslot2); dev_free(D.spec2);
+  dev_free(D.totals);
+}
+
+// Per-molecule arrays for nm molecules and nbox boxes.
+int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
+  Dev& D = c->D;
+  int rc;
+  if (nm > c->cap_nm) {
+    const int n = std::max(nm, 16);
+    if ((rc = dev_alloc(c, D.cx, n)) || (rc = dev_alloc(c, D.cy, n)) || (rc = dev_alloc(c, D.cz, n)) ||
+        (rc = dev_alloc(c, D.site, (size_t)n * NS3 * 3)) || (rc = dev_alloc(c, D.mol_box, n)) ||
+        (rc = dev_alloc(c, D.nb3_cnt, n)) || (rc = dev_alloc(c, D.nb2_cnt, n)) ||
+        (rc = dev_alloc(c, D.nb3, (size_t)n * D.cap_nb)) || (rc = dev_alloc(c, D.nb2, (size_t)n * D.cap_nb)) ||
+        (rc = dev_alloc(c, D.cnt3, n)) || (rc = dev_alloc(c, D.off3, n + 1)) ||
+        (rc = dev_alloc(c, D.pcnt2, n)) || (rc = dev_alloc(c, D.poff2, n + 1)) ||
+        (rc = dev_alloc(c, D.cnt2, n)) || (rc = dev_alloc(c, D.off2, n + 1)) ||
+        (rc = dev_alloc(c, D.spec2, (size_t)n * SPEC2)))
+      return rc;
+    c->cap_nm = n;
+  }
+  if (nbox > c->cap_box) {
+    if ((rc = dev_alloc(c, D.boxL, nbox)) || (rc = dev_alloc(c, D.box_start, nbox + 1))) return rc;
+    c->cap_box = nbox;
+  }
+  if (!D.totals && (rc = dev_alloc(c, D.totals, 16))) return rc;
+  return CCPOL3B_OK;
+}
+
+int ensure_lists(ccpol3b_ctx* c, long n3, long np2, long ni2) {
+  Dev& D = c->D;
+  int rc;
+  if (n3 > D.cap3) {
+    const long n = std::max<long>(n3 + n3 / 8, 1024);
+    if ((rc = dev_alloc(c, D.tri, n)) || (rc = dev_alloc(c, D.slot3, (size_t)n * 3 * SLOT3)) ||
+        (rc = dev_alloc(c, D.ew3, (size_t)n * 2)))
+      return rc;
+    D.cap3 = n;
+  }
+  if (np2 > D.cap_pairs2) {
+    const long n = std::max<long>(np2 + np2 / 8, 1024);
+    if ((rc = dev_alloc(c, D.pairs2, n)) || (rc = dev_alloc(c, D.pslot2, (size_t)n * PSLOT2))) return rc;
+    D.cap_pairs2 = n;
+  }
+  if (ni2 > D.cap_items2) {
+    const long n = std::max<long>(ni2 + ni2 / 8, 1024);
+    if ((rc = dev_alloc(c, D.items2, n))) return rc;
+    D.cap_items2 = n;
+  }
+  return CCPOL3B_OK;
+}
+
+int waves_grid(long nwaves) { return (int)((nwaves * 64 + 255) / 256); }
+
+float event_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
+// The evaluation proper, on device arrays already in place.
+int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double* y, const double* z,
+        const double* xs, const double* ys, const double* zs, double* fxs, double* fys, double* fzs,
+        double* fcom2, double* fcom3, double* energies, hipStream_t st) {
+  Dev& D = c->D;
+  int rc;
+  if ((rc = ensure_molecules(c, nm, nbox))) return rc;
+  D.nm = nm;
+  D.nbox = nbox;
+  HIPCHK(c, hipMemcpyAsync(D.mol_box, c->h_molbox.data(), sizeof(int) * nm, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(D.boxL, c->h_boxL.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(D.box_start, c->h_boxstart.data(), sizeof(int) * (nbox + 1),
+                           hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemsetAsync(D.totals, 0, 16 * sizeof(int), st));
+  HIPCHK(c, hipEventRecord(c->ev[0], st));
+  hipLaunchKernelGGL(k_prep, dim3((nm * NS3 + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
+  LAUNCHCHK(c, "k_prep");
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    hipLaunchKernelGGL(k_neighbours, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K);
+    LAUNCHCHK(c, "k_neighbours");
+    if (c->s3) hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K, 0);
+    if (c->s2) {
+      hipLaunchKernelGGL(k_fs2_pairs, dim3((nm + 255) / 256), dim3(256), 0, st, D, 0);
+      hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K, 0);
+    }
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, (c->s3 ? 1 : 0) | (c->s2 ? 6 : 0));
+    LAUNCHCHK(c, "list counts");
+    HIPCHK(c, hipMemcpyAsync(c->h_totals, D.totals, 16 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (c->h_totals[3] > 0) {
+      // neighbour capacity exceeded: grow and rebuild once
+      if (attempt == 1 || c->h_totals[3] > 65536)
+        return fail(c, CCPOL3B_ECAPACITY, "neighbour list capacity exceeded");
+      D.cap_nb = ((c->h_totals[3] + 63) / 64) * 64 + 64;
+      dev_free(D.nb3); dev_free(D.nb2);
+      if ((rc = dev_alloc(c, D.nb3, (size_t)c->cap_nm * D.cap_nb)) ||
+          (rc = dev_alloc(c, D.nb2, (size_t)c->cap_nm * D.cap_nb)))
+        return rc;
+      HIPCHK(c, hipMemsetAsync(D.totals, 0, 16 * sizeof(int), st));
+      continue;
+    }
+    break;
+  }
+  const long n3 = c->s3 ? c->h_totals[0] : 0, np2 = c->s2 ? c->h_totals[1] : 0;
+  const long ni2 = c->s2 ? c->h_totals[2] : 0;
+  if ((rc = ensure_lists(c, n3, np2, ni2))) return rc;
+  if (c->s3 && n3 > 0) hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K, 1);
+  if (c->s2) {
+    hipLaunchKernelGGL(k_fs2_pairs, dim3((nm + 255) / 256), dim3(256), 0, st, D, 1);
+    if (ni2 > 0) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K, 1);
+  }
+  LAUNCHCHK(c, "list fill");
+  HIPCHK(c, hipEventRecord(c->ev[1], st));
+  if (c->s2) {
+    if (np2 > 0) hipLaunchKernelGGL(k_fs2_pair, dim3(waves_grid(np2)), dim3(256), 0, st, D, c->K);
+    hipLaunchKernelGGL(k_fs2_spec, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K);
+    hipLaunchKernelGGL(k_fs2_gather, dim3(waves_grid(nm)), dim3(256), 0, st, D, ld, fxs, fys, fzs, fcom2);
+    LAUNCHCHK(c, "FS2");
+  }
+  HIPCHK(c, hipEventRecord(c->ev[2], st));
+  if (c->s3) {
+    if (n3 > 0) {
+      const int grid = (int)std::min<long>(n3, (long)c->cu_count * 8);
+      if (c->precision == CCPOL3B_MIXED)
+        hipLaunchKernelGGL(k_fs3_eval<float>, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+                           (const float*)c->d_coef32);
+      else
+        hipLaunchKernelGGL(k_fs3_eval<double>, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+                           (const double*)c->d_coef64);
+      LAUNCHCHK(c, "k_fs3_eval");
+    }
+    HIPCHK(c, hipEventRecord(c->ev[3], st));
+    hipLaunchKernelGGL(k_fs3_gather, dim3(waves_grid(nm)), dim3(256), 0, st, D, ld, fxs, fys, fzs, fcom3);
+    LAUNCHCHK(c, "k_fs3_gather");
+  } else {
+    HIPCHK(c, hipEventRecord(c->ev[3], st));
+  }
+  hipLaunchKernelGGL(k_energy, dim3(nbox), dim3(256), 0, st, D, c->s2, c->s3, energies);
+  LAUNCHCHK(c, "k_energy");
+  HIPCHK(c, hipEventRecord(c->ev[4], st));
+  HIPCHK(c, hipMemcpyAsync(c->h_totals + 4, D.totals + 4, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+  c->stats.n_fs3 = n3;
+  c->stats.n_fs2_pairs = np2;
+  c->stats.n_fs2_items = ni2;
+  return CCPOL3B_OK;
+}
+
+void finish_stats(ccpol3b_ctx* c) {
+  c->stats.n_fs3_sr = c->h_totals[4];
+  c->stats.n_fs3_switch = c->h_totals[5];
+  c->stats.n_fs3_lr = c->h_totals[6];
+  c->stats.ms_lists = event_ms(c->ev[0], c->ev[1]);
+  c->stats.ms_fs2 = event_ms(c->ev[1], c->ev[2]);
+  c->stats.ms_fs3 = event_ms(c->ev[2], c->ev[3]);
+  c->stats.ms_reduce = event_ms(c->ev[3], c->ev[4]);
+  c->stats.ms_total = event_ms(c->ev[0], c->ev[4]);
+}
+
+void set_single_box(ccpol3b_ctx* c, int nm, double box) {
+  c->h_molbox.assign(nm, 0);
+  c->h_boxstart = {0, nm};
+  c->h_boxL = {box > 0.0 ? box / BOHR2A : 0.0};
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ccpol3b_status_string(int s) {
+  switch (s) {
+    case CCPOL3B_OK: return "ok";
+    case CCPOL3B_EINVAL: return "invalid argument";
+    case CCPOL3B_EHIP: return "HIP runtime error";
+    case CCPOL3B_ENOMEM: return "device out of memory";
+    case CCPOL3B_ECAPACITY: return "list capacity exceeded";
+    case CCPOL3B_ENODEV: return "no gfx950 device";
+    default: return "unknown status";
+  }
+}
+
+const char* ccpol3b_last_error(const ccpol3b_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ccpol3b_orbit_index(int is, int js, int ks, int ip, int jp, int kp) {
+  if (is < 1 || is > 17 || js < 1 || js > 17 || ks < 1 || ks > 17 || ip < 0 || ip > 1 || jp < 0 ||
+      jp > 1 || kp < 0 || kp > 1)
+    return CCPOL3B_EINVAL;
+  return orbit_map()(is - 1, js - 1, ks - 1, ip, jp, kp);
+}
+
+int ccpol3b_read_data3b(const char* path, double* beta, double* gamma, double* r0, int ld_pair,
+                        double* cs, int* nlin) {
+  if (!path || !beta || !gamma || !r0 || !cs || !nlin || ld_pair < 17) return CCPOL3B_EINVAL;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return CCPOL3B_EINVAL;
+  char line[512];
+  auto next = [&](char* buf) -> bool {
+    while (std::fgets(buf, sizeof(line), f)) {
+      bool blank = true;
+      for (char* q = buf; *q; ++q)
+        if (*q != ' ' && *q != '\t' && *q != '\n' && *q != '\r') blank = false;
+      if (!blank) return true;
+    }
+    return false;
+  };
+  auto fixd = [](char* s) {
+    for (char* q = s; *q; ++q)
+      if (*q == 'd' || *q == 'D') *q = 'e';
+  };
+  int rc = CCPOL3B_OK;
+  if (!next(line)) rc = CCPOL3B_EINVAL;  // header record
+  for (int is = 0; is < 17 && rc == 0; ++is)
+    for (int js = 0; js < 17 && rc == 0; ++js) {
+      double b, g, r;
+      if (!next(line)) { rc = CCPOL3B_EINVAL; break; }
+      fixd(line);
+      if (std::sscanf(line, "%lf %lf %lf", &b, &g, &r) != 3) { rc = CCPOL3B_EINVAL; break; }
+      beta[is + (size_t)ld_pair * js] = b;
+      gamma[is + (size_t)ld_pair * js] = g;
+      r0[is + (size_t)ld_pair * js] = r;
+    }
+  int n0 = 0;
+  if (rc == 0 && !next(line)) rc = CCPOL3B_EINVAL;  // label record, content ignored
+  if (rc == 0 && (!next(line) || std::sscanf(line, "%d", &n0) != 1)) rc = CCPOL3B_EINVAL;
+  for (int q = 0; q < n0 && rc == 0; ++q) {
+    long idx;
+    double v;
+    if (!next(line)) { rc = CCPOL3B_EINVAL; break; }
+    fixd(line);
+    if (std::sscanf(line, "%ld %lf", &idx, &v) != 2) { rc = CCPOL3B_EINVAL; break; }
+    if (q < *nlin) cs[q] = v;
+  }
+  std::fclose(f);
+  if (rc == 0) *nlin = n0;
+  return rc;
+}
+
+int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
+  if (!p || !out) return CCPOL3B_EINVAL;
+  *out = nullptr;
+  if (p->nlin != CCPOL3B_NLIN || p->ld_pair < 17 || !p->beta_sr || !p->gamma_sr || !p->r0_sr ||
+      !p->beta_lr || !p->gamma_lr || !p->r0_lr || !p->cs_sr || !p->cs_lr ||
+      (p->precision != CCPOL3B_FP64 && p->precision != CCPOL3B_MIXED))
+    return CCPOL3B_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CCPOL3B_ENODEV;
+  int dev = p->device;
+  if (dev < 0 && hipGetDevice(&dev) != hipSuccess) return CCPOL3B_ENODEV;
+  if (dev >= ndev) return CCPOL3B_EINVAL;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return CCPOL3B_ENODEV;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return CCPOL3B_ENODEV;
+  if (hipSetDevice(dev) != hipSuccess) return CCPOL3B_ENODEV;
+
+  ccpol3b_ctx* c = new ccpol3b_ctx();
+  c->device = dev;
+  c->precision = p->precision;
+  c->s2 = p->s2_term ? 1 : 0;
+  c->s3 = p->s3_term ? 1 : 0;
+  c->cu_count = prop.multiProcessorCount;
+  c->D.cap_nb = 256;
+  // constants exactly as the reference forms them
+  Consts& K = c->K;
+  K.alp3 = p->alpha3 * BOHR2A;
+  K.delta33 = p->delta33;
+  K.r0dmp3 = 6.20446 / BOHR2A;
+  K.r0dmp2 = 7.40446 / BOHR2A;
+  K.rc2_3 = std::pow(6.3, 2.0) * std::pow(BOHR2A, -2.0);
+  K.rc2_2 = (p->rcut3 * p->rcut3) * std::pow(BOHR2A, -2.0);
+  K.rmin = 4.5 / BOHR2A;
+  K.rmax = 5.5 / BOHR2A;
+  // parameter tables
+  std::vector<double4> pairpar(2 * NPAIR);
+  for (int X = 0; X < 2; ++X) {
+    const double* b = X ? p->beta_lr : p->beta_sr;
+    const double* g = X ? p->gamma_lr : p->gamma_sr;
+    const double* r = X ? p->r0_lr : p->r0_sr;
+    for (int is = 0; is < 17; ++is)
+      for (int js = 0; js < 17; ++js) {
+        const size_t q = is + (size_t)p->ld_pair * js;
+        pairpar[X * NPAIR + is * 17 + js] = make_double4(b[q], g[q], r[q], 0.0);
+      }
+  }
+  // dense coefficient tables C[X][tx][ty][tz][p][q][s]
+  std::vector<double> c64(2 * NCOEF);
+  std::vector<float> c32(2 * NCOEF);
+  const OrbitMap& om = orbit_map();
+  for (int X = 0; X < 2; ++X) {
+    const double* cs = X ? p->cs_lr : p->cs_sr;
+    for (int tx = 0; tx < NTYPE; ++tx)
+      for (int ty = 0; ty < NTYPE; ++ty)
+        for (int tz = 0; tz < NTYPE; ++tz)
+          for (int pp = 0; pp < 2; ++pp)
+            for (int qq = 0; qq < 2; ++qq)
+              for (int ss = 0; ss < 2; ++ss) {
+                const int n = om(first_site_of_type(tx), first_site_of_type(ty), first_site_of_type(tz),
+                                 pp, qq, ss);
+                const size_t q = (size_t)X * NCOEF + (((tx * NTYPE + ty) * NTYPE + tz) * 2 + pp) * 4 + qq * 2 + ss;
+                c64[q] = cs[n - 1];
+                c32[q] = (float)cs[n - 1];
+              }
+  }
+  int rc = CCPOL3B_OK;
+  if ((rc = dev_alloc(c, c->d_pairpar, 2 * NPAIR)) || (rc = dev_alloc(c, c->d_coef64, 2 * NCOEF)) ||
+      (rc = dev_alloc(c, c->d_coef32, 2 * NCOEF))) {
+    ccpol3b_destroy(c);
+    return rc;
+  }
+  if (hipMemcpy(c->d_pairpar, pairpar.data(), sizeof(double4) * 2 * NPAIR, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_coef64, c64.data(), sizeof(double) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_coef32, c32.data(), sizeof(float) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_totals, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    ccpol3b_destroy(c);
+    return CCPOL3B_EHIP;
+  }
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      ccpol3b_destroy(c);
+      return CCPOL3B_EHIP;
+    }
+  *out = c;
+  return CCPOL3B_OK;
+}
+
+void ccpol3b_destroy(ccpol3b_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
+  free_dev(c);
+  dev_free(c->d_pairpar);
+  dev_free(c->d_coef64);
+  dev_free(c->d_coef32);
+  dev_free(c->d_in);
+  dev_free(c->d_out);
+  if (c->h_totals) (void)hipHostFree(c->h_totals);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete c;
+}
+
+int ccpol3b_set_precision(ccpol3b_ctx* c, int precision) {
+  if (!c || (precision != CCPOL3B_FP64 && precision != CCPOL3B_MIXED)) return CCPOL3B_EINVAL;
+  c->precision = precision;
+  return CCPOL3B_OK;
+}
+
+int ccpol3b_eval_device(ccpol3b_ctx* c, int nm, int ld, const double* x, const double* y, const double* z,
+                        const double* xs, const double* ys, const double* zs, double box, double* fxs,
+                        double* fys, double* fzs, double* fcom2, double* fcom3, double* energies,
+                        void* stream) {
+  if (!c || nm < 0 || ld < 17 || (nm > 0 && (!x || !y || !z || !xs || !ys || !zs || !fxs || !fys || !fzs ||
+                                             !fcom2 || !fcom3 || !energies)))
+    return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  set_single_box(c, nm, box);
+  if (nm == 0) return CCPOL3B_OK;
+  int rc = run(c, nm, 1, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(st));
+  finish_stats(c);
+  return CCPOL3B_OK;
+}
+
+int ccpol3b_eval_batch_device(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld,
+                              const double* x, const double* y, const double* z, const double* xs,
+                              const double* ys, const double* zs, double* fxs, double* fys, double* fzs,
+                              double* fcom2, double* fcom3, double* energies, void* stream) {
+  if (!c || nbox <= 0 || !mol_start || !box_len || ld < 17) return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  const int nm = mol_start[nbox];
+  if (mol_start[0] != 0 || nm < 0) return fail(c, CCPOL3B_EINVAL, "mol_start must start at 0");
+  for (int b = 0; b < nbox; ++b)
+    if (mol_start[b + 1] < mol_start[b]) return fail(c, CCPOL3B_EINVAL, "mol_start not ascending");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  c->h_molbox.resize(nm);
+  c->h_boxstart.assign(mol_start, mol_start + nbox + 1);
+  c->h_boxL.resize(nbox);
+  for (int b = 0; b < nbox; ++b) {
+    c->h_boxL[b] = box_len[b] > 0.0 ? box_len[b] / BOHR2A : 0.0;
+    for (int m = mol_start[b]; m < mol_start[b + 1]; ++m) c->h_molbox[m] = b;
+  }
+  if (nm == 0) return CCPOL3B_OK;
+  int rc = run(c, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(st));
+  finish_stats(c);
+  return CCPOL3B_OK;
+}
+
+int ccpol3b_threebody_term(ccpol3b_ctx* c, int nm, int ld, const double* x, const double* y, const double* z,
+                           const double* xs, const double* ys, const double* zs, double box, double* qfs2,
+                           double* qfs3, double* fxs, double* fys, double* fzs, double* fxfs2, double* fyfs2,
+                           double* fzfs2, double* fxfs3, double* fyfs3, double* fzfs3, double* txfs2,
+                           double* tyfs2, double* tzfs2, double* txfs3, double* tyfs3, double* tzfs3,
+                           double* vir) {
+  if (!c || nm < 0 || ld < 17 || !qfs2 || !qfs3 || !vir ||
+      (nm > 0 && (!x || !y || !z || !xs || !ys || !zs || !fxs || !fys || !fzs || !fxfs2 || !fyfs2 ||
+                  !fzfs2 || !fxfs3 || !fyfs3 || !fzfs3)))
+    return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
+  if (nm == 0) return CCPOL3B_OK;
+  const int ldd = NS3;  // compact device layout of site arrays
+  const size_t nin = 3 * (size_t)nm + 3 * (size_t)ldd * nm;
+  const size_t nout = 3 * (size_t)ldd * nm + 6 * (size_t)nm + 3;
+  int rc;
+  if (nin > c->cap_in) {
+    if ((rc = dev_alloc(c, c->d_in, nin))) return rc;
+    c->cap_in = nin;
+  }
+  if (nout > c->cap_out) {
+    if ((rc = dev_alloc(c, c->d_out, nout))) return rc;
+    c->cap_out = nout;
+  }
+  // pack the inputs: COM and the first 17 rows of the site arrays
+  std::vector<double> hin(nin);
+  std::memcpy(hin.data(), x, sizeof(double) * nm);
+  std::memcpy(hin.data() + nm, y, sizeof(double) * nm);
+  std::memcpy(hin.data() + 2 * nm, z, sizeof(double) * nm);
+  const double* srcs[3] = {xs, ys, zs};
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < nm; ++m)
+      std::memcpy(hin.data() + 3 * nm + (size_t)a * ldd * nm + (size_t)ldd * m, srcs[a] + (size_t)ld * m,
+                  sizeof(double) * ldd);
+  hipStream_t st = nullptr;
+  HIPCHK(c, hipEventRecord(c->ev[5], st));
+  HIPCHK(c, hipMemcpyAsync(c->d_in, hin.data(), sizeof(double) * nin, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemsetAsync(c->d_out, 0, sizeof(double) * nout, st));
+  double* din = c->d_in;
+  double* dout = c->d_out;
+  set_single_box(c, nm, box);
+  rc = run(c, nm, 1, ldd, din, din + nm, din + 2 * nm, din + 3 * nm, din + 3 * nm + (size_t)ldd * nm,
+           din + 3 * nm + 2 * (size_t)ldd * nm, dout, dout + (size_t)ldd * nm, dout + 2 * (size_t)ldd * nm,
+           dout + 3 * (size_t)ldd * nm, dout + 3 * (size_t)ldd * nm + 3 * nm, dout + 3 * (size_t)ldd * nm + 6 * nm,
+           st);
+  if (rc) return rc;
+  std::vector<double> hout(nout);
+  HIPCHK(c, hipEventRecord(c->ev[6], st));
+  HIPCHK(c, hipMemcpyAsync(hout.data(), dout, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipEventRecord(c->ev[7], st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  finish_stats(c);
+  c->stats.ms_h2d = event_ms(c->ev[5], c->ev[0]);
+  c->stats.ms_d2h = event_ms(c->ev[6], c->ev[7]);
+  // scatter back with the reference's output semantics
+  const double* f[3] = {hout.data(), hout.data() + (size_t)ldd * nm, hout.data() + 2 * (size_t)ldd * nm};
+  double* dst[3] = {fxs, fys, fzs};
+  const int nsite_out = c->s3 ? NS3 : (c->s2 ? NS2 : 0);
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < nm; ++m)
+      for (int s = 0; s < nsite_out; ++s) dst[a][(size_t)ld * m + s] += f[a][(size_t)ldd * m + s];
+  const double* c2 = hout.data() + 3 * (size_t)ldd * nm;
+  const double* c3 = c2 + 3 * nm;
+  const double* en = c3 + 3 * nm;
+  if (c->s2) {
+    for (int m = 0; m < nm; ++m) {
+      fxfs2[m] = c2[m];
+      fyfs2[m] = c2[nm + m];
+      fzfs2[m] = c2[2 * nm + m];
+      if (txfs2) txfs2[m] = 0.0;
+      if (tyfs2) tyfs2[m] = 0.0;
+      if (tzfs2) tzfs2[m] = 0.0;
+    }
+    *qfs2 = en[0];
+  }
+  if (c->s3) {
+    for (int m = 0; m < nm; ++m) {
+      fxfs3[m] = c3[m];
+      fyfs3[m] = c3[nm + m];
+      fzfs3[m] = c3[2 * nm + m];
+      if (txfs3) txfs3[m] = 0.0;
+      if (tyfs3) tyfs3[m] = 0.0;
+      if (tzfs3) tzfs3[m] = 0.0;
+    }
+    *qfs3 = en[1];
+  }
+  *vir += en[2];
+  return CCPOL3B_OK;
+}
+
+int ccpol3b_last_stats(const ccpol3b_ctx* c, ccpol3b_stats* s) {
+  if (!c || !s) return CCPOL3B_EINVAL;
+  *s = c->stats;
+  return CCPOL3B_OK;
+}
+
+long ccpol3b_get_fs3_list(ccpol3b_ctx* c, int* ijk, long cap) {
+  if (!c || (cap > 0 && !ijk)) return CCPOL3B_EINVAL;
+  const long n = c->stats.n_fs3;
+  const long m = std::min(n, cap);
+  if (m > 0) {
+    std::vector<int4> tmp(m);
+    if (hipMemcpy(tmp.data(), c->D.tri, sizeof(int4) * m, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(c, CCPOL3B_EHIP, "copy of the FS3 list");
+    for (long t = 0; t < m; ++t) {
+      ijk[3 * t] = tmp[t].x;
+      ijk[3 * t + 1] = tmp[t].y;
+      ijk[3 * t + 2] = tmp[t].z;
+    }
+  }
+  return n;
+}
+
+long ccpol3b_get_fs2_list(ccpol3b_ctx* c, int* kij, long cap) {
+  if (!c || (cap > 0 && !kij)) return CCPOL3B_EINVAL;
+  const long n = c->stats.n_fs2_items;
+  const long m = std::min(n, cap);
+  if (m > 0) {
+    std::vector<int4> tmp(m);
+    if (hipMemcpy(tmp.data(), c->D.items2, sizeof(int4) * m, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(c, CCPOL3B_EHIP, "copy of the FS2 list");
+    for (long t = 0; t < m; ++t) {
+      kij[3 * t] = tmp[t].x;
+      kij[3 * t + 1] = tmp[t].y;
+      kij[3 * t + 2] = tmp[t].z;
+    }
+  }
+  return n;
+}
+
+}  // extern "C"

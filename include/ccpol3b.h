@@ -1,0 +1,150 @@
+/*
+ * ccpol3b.h -- C ABI of the MI355X three-body library (libccpol3b.so).
+ *
+ * Drop-in for the reference routine
+ *     subroutine threebody_term(xxx,yyy,zzz,xxxs,yyys,zzzs,boxii,qfs2,qfs3)
+ * of source/threebody_potential.f90:17-99 together with the module globals it
+ * reads (parameters, common_arrays.f90:201-218) and writes (site forces fxs..,
+ * COM forces fxfs2../fxfs3.., torques txfs2../txfs3.., virial `vir`).
+ *
+ * Conventions (identical to the reference):
+ *   - positions in Angstrom; site offsets are Fortran column-major arrays with
+ *     leading dimension ld_site: xs(is,i) == xs[(is-1) + ld_site*(i-1)]
+ *     (common_arrays.f90:22, nsite=50);
+ *   - site forces fxs,fys,fzs are ACCUMULATED (+=) at sites 1..17 (FS3) and
+ *     1..5 (FS2); fxfs2.. and fxfs3.. are OVERWRITTEN; torque arrays, when given,
+ *     are set to zero; `vir` is ACCUMULATED;
+ *   - energies in kcal/mol, forces in 10 J mol^-1 A^-1, virial in 10 J mol^-1
+ *     (threebody_potential.f90:574-585, 604, 669-671, 1964-1977, 1996, 2058-2059);
+ *   - box <= 0 selects an open boundary (isolated clusters).
+ * Every entry point returns 0 on success and a negative CCPOL3B_E* code on
+ * failure; on failure the caller's output arrays are left untouched.
+ * One context is used by one host thread at a time.
+ */
+#ifndef CCPOL3B_H
+#define CCPOL3B_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCPOL3B_NS3 17
+#define CCPOL3B_NLIN 364
+
+enum ccpol3b_precision {
+  CCPOL3B_FP64 = 0,  /* everything in FP64 */
+  CCPOL3B_MIXED = 1  /* FP32 exponential/polynomial work, FP64 displacements,
+                        switch weights and final reductions */
+};
+
+enum ccpol3b_status {
+  CCPOL3B_OK = 0,
+  CCPOL3B_EINVAL = -1,    /* bad argument */
+  CCPOL3B_EHIP = -2,      /* a HIP runtime call or kernel launch failed */
+  CCPOL3B_ENOMEM = -3,    /* device allocation failed */
+  CCPOL3B_ECAPACITY = -4, /* a molecule has more neighbours than the list capacity */
+  CCPOL3B_ENODEV = -5     /* no usable gfx950 device */
+};
+
+/* Everything threebody_term reads from module globals.
+ * Pair tables are Fortran-ordered (is,js): beta(is,js) == beta_sr[(is-1) + ld_pair*(js-1)],
+ * so the reference's bS3_s(nsite,nsite) arrays can be passed with ld_pair = 50.
+ * alpha3 (1/A), delta33 and rcut3 (A) are CONTROL record 15; s2_term/s3_term record 14. */
+typedef struct {
+  const double *beta_sr, *gamma_sr, *r0_sr;   /* bS3_s, gS3_s, r0S3_s */
+  const double *beta_lr, *gamma_lr, *r0_lr;   /* bS3_l, gS3_l, r0S3_l */
+  int ld_pair;                                /* >= 17 */
+  const double *cs_sr, *cs_lr;                /* cs3_s(1:364), cs3_l(1:364) */
+  int nlin;                                   /* must be 364 */
+  double alpha3, delta33, rcut3;
+  int s2_term, s3_term;
+  int precision;                              /* enum ccpol3b_precision */
+  int device;                                 /* HIP device ordinal, -1 = current */
+} ccpol3b_params;
+
+typedef struct ccpol3b_ctx ccpol3b_ctx;
+
+/* Counters and timings of the last evaluation. */
+typedef struct {
+  long n_fs3;                        /* unordered FS3 triplets i<j<k accepted */
+  long n_fs3_sr, n_fs3_switch, n_fs3_lr; /* region histogram (R_avg <=4.5, between, >=5.5 A) */
+  long n_fs2_items;                  /* ordered FS2 items (k; i<j) accepted */
+  long n_fs2_pairs;                  /* unordered FS2 pairs within rcut3 */
+  double ms_total;                   /* device time of the whole evaluation (HIP events) */
+  double ms_lists, ms_fs2, ms_fs3, ms_reduce;
+  double ms_h2d, ms_d2h;             /* host-pointer entry point only */
+} ccpol3b_stats;
+
+/* Create / destroy a context holding device copies of the parameters.
+ * Replaces the parameter reads of initialization.f90:640-701 (values passed in). */
+int ccpol3b_create(const ccpol3b_params *params, ccpol3b_ctx **ctx);
+void ccpol3b_destroy(ccpol3b_ctx *ctx);
+
+/* Switch the arithmetic mode of an existing context. */
+int ccpol3b_set_precision(ccpol3b_ctx *ctx, int precision);
+
+/* The drop-in: host arrays in, host arrays out, same semantics as
+ * threebody_term (threebody_potential.f90:17-99).  Torque pointers may be NULL. */
+int ccpol3b_threebody_term(ccpol3b_ctx *ctx, int nm, int ld_site,
+                           const double *x, const double *y, const double *z,
+                           const double *xs, const double *ys, const double *zs,
+                           double box, double *qfs2, double *qfs3,
+                           double *fxs, double *fys, double *fzs,
+                           double *fxfs2, double *fyfs2, double *fzfs2,
+                           double *fxfs3, double *fyfs3, double *fzfs3,
+                           double *txfs2, double *tyfs2, double *tzfs2,
+                           double *txfs3, double *tyfs3, double *tzfs3,
+                           double *vir);
+
+/* Same computation on device-resident arrays (same layouts and units), enqueued
+ * on `stream` (a hipStream_t; NULL = default stream).  `energies` receives
+ * {qfs2, qfs3, vir_increment} on the device.  Site forces are accumulated,
+ * COM forces overwritten, as above.  The call returns after the work is
+ * enqueued and the list sizes were checked (one small device-to-host read). */
+int ccpol3b_eval_device(ccpol3b_ctx *ctx, int nm, int ld_site,
+                        const double *d_x, const double *d_y, const double *d_z,
+                        const double *d_xs, const double *d_ys, const double *d_zs,
+                        double box,
+                        double *d_fxs, double *d_fys, double *d_fzs,
+                        double *d_fcom2, double *d_fcom3, /* [3*nm] x..,y..,z.. blocks */
+                        double *d_energies, void *stream);
+
+/* Batched evaluation of nbox independent boxes in one launch sequence.
+ * Molecules of box b are rows mol_start[b] .. mol_start[b+1]-1 of the packed
+ * device arrays; box_len[b] (A, host array) <= 0 means open boundary.
+ * d_energies receives 3 doubles per box {qfs2, qfs3, vir_increment}. */
+int ccpol3b_eval_batch_device(ccpol3b_ctx *ctx, int nbox, const int *mol_start,
+                              const double *box_len, int ld_site,
+                              const double *d_x, const double *d_y, const double *d_z,
+                              const double *d_xs, const double *d_ys, const double *d_zs,
+                              double *d_fxs, double *d_fys, double *d_fzs,
+                              double *d_fcom2, double *d_fcom3,
+                              double *d_energies, void *stream);
+
+int ccpol3b_last_stats(const ccpol3b_ctx *ctx, ccpol3b_stats *stats);
+
+/* Copy the last FS3 triplet list (0-based i<j<k, lexicographic order, the
+ * reference loop order of threebody_potential.f90:859-892) or the last FS2
+ * item list (rows k,i,j in the reference loop order :297-301) to host memory.
+ * Returns the number of entries, or a negative error; writes at most cap rows. */
+long ccpol3b_get_fs3_list(ccpol3b_ctx *ctx, int *ijk, long cap);
+long ccpol3b_get_fs2_list(ccpol3b_ctx *ctx, int *kij, long cap);
+
+/* Reader for one data3b_full* file (initialization.f90:660-680): the 17x17
+ * (beta, gamma, r0) table in Fortran (is,js) order with leading dimension
+ * ld_pair, and up to *nlin coefficients (on return *nlin = nlin0 of the file). */
+int ccpol3b_read_data3b(const char *path, double *beta, double *gamma, double *r0,
+                        int ld_pair, double *cs, int *nlin);
+
+/* The orbit number (1..364) of (site i, site j, site k, power of r_jk,
+ * power of r_ik, power of r_ij), 1-based sites 1..17: ind_lin_2 of
+ * threebody_potential.f90:909-946 built with fill (:2064-2131). */
+int ccpol3b_orbit_index(int is, int js, int ks, int ip, int jp, int kp);
+
+const char *ccpol3b_status_string(int status);
+const char *ccpol3b_last_error(const ccpol3b_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

