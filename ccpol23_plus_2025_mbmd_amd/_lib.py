@@ -57,6 +57,13 @@ def load():
     if not LIB_PATH.exists():
         raise CcpolError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                          "(make -C ccpol23_plus_2025_mbmd_amd/csrc)")
+    # PyTorch-ROCm ships its own HIP runtime under the same soname; loading it
+    # first makes the library bind to that runtime, so one process never holds
+    # two HIP runtimes (torch would otherwise see no devices).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     dp = C.POINTER(C.c_double)
     vp = C.c_void_p

@@ -20,15 +20,15 @@ __global__ void k_prep(int, int, const double*, const double*, const double*, co
                        const double*, const double*, Dev);
 __global__ void k_neighbours(Dev, Consts);
 __global__ void k_fs3_list(Dev, Consts, int);
-__global__ void k_fs2_pairs(Dev, int);
+__global__ void k_fs2_pairs(Dev);
 __global__ void k_fs2_items(Dev, Consts, int);
-__global__ void k_scan(Dev, int);
+__global__ void k_scan(Dev, int, int, int);
+__global__ void k_box_ranges(Dev, int, int);
 template <typename T>
 __global__ void k_fs3_eval(Dev, Consts, const double4*, const T*);
-__global__ void k_fs3_gather(Dev, int, double*, double*, double*, double*);
-__global__ void k_fs2_pair(Dev, Consts);
-__global__ void k_fs2_spec(Dev, Consts);
-__global__ void k_fs2_gather(Dev, int, double*, double*, double*, double*);
+__global__ void k_fs2_items_eval(Dev, Consts);
+__global__ void k_fs2_pair_sites(Dev, Consts);
+__global__ void k_finish(Dev, int, int, int, int, double*, double*, double*, double*, double*);
 __global__ void k_energy(Dev, int, int, double*);
 }  // namespace ccpol3b
 
@@ -152,57 +152,60 @@ void free_dev(ccpol3b_ctx* c) {
   Dev& D = c->D;
   dev_free(D.cx); dev_free(D.cy); dev_free(D.cz); dev_free(D.site);
   dev_free(D.mol_box); dev_free(D.boxL); dev_free(D.box_start);
-  dev_free(D.nb3_cnt); dev_free(D.nb3); dev_free(D.nb2_cnt); dev_free(D.nb2);
-  dev_free(D.cnt3); dev_free(D.off3); dev_free(D.tri);
-  dev_free(D.pcnt2); dev_free(D.poff2); dev_free(D.pairs2);
-  dev_free(D.cnt2); dev_free(D.off2); dev_free(D.items2);
-  dev_free(D.slot3); dev_free(D.ew3); dev_free(D.pslot2); dev_free(D.spec2);
-  dev_free(D.totals);
+  dev_free(D.nb3_cnt); dev_free(D.nb3); dev_free(D.up3);
+  dev_free(D.nb2_cnt); dev_free(D.nb2); dev_free(D.up2);
+  dev_free(D.sl3_cnt); dev_free(D.sl3_off); dev_free(D.sl2_cnt); dev_free(D.sl2_off);
+  dev_free(D.cnt3s); dev_free(D.off3s); dev_free(D.cnt2s); dev_free(D.off2s);
+  dev_free(D.tri); dev_free(D.pcnt2); dev_free(D.poff2); dev_free(D.pairs2); dev_free(D.items2);
+  dev_free(D.wt2); dev_free(D.acc); dev_free(D.ew3); dev_free(D.ew2);
+  dev_free(D.box_t3); dev_free(D.box_t2); dev_free(D.totals);
 }
 
-// Per-molecule arrays for nm molecules and nbox boxes.
+// Per-molecule arrays for nm molecules and nbox boxes.  Sub-list and pair
+// arrays are bounded by nm * cap_nb, so they need no mid-call size check.
 int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
   Dev& D = c->D;
   int rc;
   if (nm > c->cap_nm) {
     const int n = std::max(nm, 16);
+    const size_t nn = (size_t)n * D.cap_nb;
     if ((rc = dev_alloc(c, D.cx, n)) || (rc = dev_alloc(c, D.cy, n)) || (rc = dev_alloc(c, D.cz, n)) ||
         (rc = dev_alloc(c, D.site, (size_t)n * NS3 * 3)) || (rc = dev_alloc(c, D.mol_box, n)) ||
         (rc = dev_alloc(c, D.nb3_cnt, n)) || (rc = dev_alloc(c, D.nb2_cnt, n)) ||
-        (rc = dev_alloc(c, D.nb3, (size_t)n * D.cap_nb)) || (rc = dev_alloc(c, D.nb2, (size_t)n * D.cap_nb)) ||
-        (rc = dev_alloc(c, D.cnt3, n)) || (rc = dev_alloc(c, D.off3, n + 1)) ||
+        (rc = dev_alloc(c, D.up3, n)) || (rc = dev_alloc(c, D.up2, n)) ||
+        (rc = dev_alloc(c, D.nb3, nn)) || (rc = dev_alloc(c, D.nb2, nn)) ||
+        (rc = dev_alloc(c, D.sl3_cnt, n)) || (rc = dev_alloc(c, D.sl3_off, n + 1)) ||
+        (rc = dev_alloc(c, D.sl2_cnt, n)) || (rc = dev_alloc(c, D.sl2_off, n + 1)) ||
         (rc = dev_alloc(c, D.pcnt2, n)) || (rc = dev_alloc(c, D.poff2, n + 1)) ||
-        (rc = dev_alloc(c, D.cnt2, n)) || (rc = dev_alloc(c, D.off2, n + 1)) ||
-        (rc = dev_alloc(c, D.spec2, (size_t)n * SPEC2)))
+        (rc = dev_alloc(c, D.cnt3s, nn)) || (rc = dev_alloc(c, D.off3s, nn + 1)) ||
+        (rc = dev_alloc(c, D.cnt2s, nn)) || (rc = dev_alloc(c, D.off2s, nn + 1)) ||
+        (rc = dev_alloc(c, D.pairs2, nn)) || (rc = dev_alloc(c, D.wt2, nn)) ||
+        (rc = dev_alloc(c, D.acc, (size_t)n * ACC_W)))
       return rc;
+    D.cap_sl3 = D.cap_sl2 = D.cap_pairs2 = (long)nn;
     c->cap_nm = n;
   }
   if (nbox > c->cap_box) {
-    if ((rc = dev_alloc(c, D.boxL, nbox)) || (rc = dev_alloc(c, D.box_start, nbox + 1))) return rc;
+    if ((rc = dev_alloc(c, D.boxL, nbox)) || (rc = dev_alloc(c, D.box_start, nbox + 1)) ||
+        (rc = dev_alloc(c, D.box_t3, nbox + 1)) || (rc = dev_alloc(c, D.box_t2, nbox + 1)))
+      return rc;
     c->cap_box = nbox;
   }
-  if (!D.totals && (rc = dev_alloc(c, D.totals, 16))) return rc;
+  if (!D.totals && (rc = dev_alloc(c, D.totals, T_COUNT))) return rc;
   return CCPOL3B_OK;
 }
 
-int ensure_lists(ccpol3b_ctx* c, long n3, long np2, long ni2) {
+int ensure_lists(ccpol3b_ctx* c, long n3, long ni2) {
   Dev& D = c->D;
   int rc;
   if (n3 > D.cap3) {
     const long n = std::max<long>(n3 + n3 / 8, 1024);
-    if ((rc = dev_alloc(c, D.tri, n)) || (rc = dev_alloc(c, D.slot3, (size_t)n * 3 * SLOT3)) ||
-        (rc = dev_alloc(c, D.ew3, (size_t)n * 2)))
-      return rc;
+    if ((rc = dev_alloc(c, D.tri, n)) || (rc = dev_alloc(c, D.ew3, (size_t)n * 2))) return rc;
     D.cap3 = n;
-  }
-  if (np2 > D.cap_pairs2) {
-    const long n = std::max<long>(np2 + np2 / 8, 1024);
-    if ((rc = dev_alloc(c, D.pairs2, n)) || (rc = dev_alloc(c, D.pslot2, (size_t)n * PSLOT2))) return rc;
-    D.cap_pairs2 = n;
   }
   if (ni2 > D.cap_items2) {
     const long n = std::max<long>(ni2 + ni2 / 8, 1024);
-    if ((rc = dev_alloc(c, D.items2, n))) return rc;
+    if ((rc = dev_alloc(c, D.items2, n)) || (rc = dev_alloc(c, D.ew2, (size_t)n * 2))) return rc;
     D.cap_items2 = n;
   }
   return CCPOL3B_OK;
@@ -222,6 +225,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
         double* fcom2, double* fcom3, double* energies, hipStream_t st) {
   Dev& D = c->D;
   int rc;
+  const int do3 = c->s3, do2 = c->s2;
   if ((rc = ensure_molecules(c, nm, nbox))) return rc;
   D.nm = nm;
   D.nbox = nbox;
@@ -229,74 +233,77 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   HIPCHK(c, hipMemcpyAsync(D.boxL, c->h_boxL.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(D.box_start, c->h_boxstart.data(), sizeof(int) * (nbox + 1),
                            hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemsetAsync(D.totals, 0, 16 * sizeof(int), st));
   HIPCHK(c, hipEventRecord(c->ev[0], st));
-  hipLaunchKernelGGL(k_prep, dim3((nm * NS3 + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
-  LAUNCHCHK(c, "k_prep");
   for (int attempt = 0; attempt < 2; ++attempt) {
+    HIPCHK(c, hipMemsetAsync(D.totals, 0, T_COUNT * sizeof(int), st));
+    const long nprep = (long)nm * ACC_W;
+    hipLaunchKernelGGL(k_prep, dim3((nprep + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
     hipLaunchKernelGGL(k_neighbours, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K);
-    LAUNCHCHK(c, "k_neighbours");
-    if (c->s3) hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K, 0);
-    if (c->s2) {
-      hipLaunchKernelGGL(k_fs2_pairs, dim3((nm + 255) / 256), dim3(256), 0, st, D, 0);
-      hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K, 0);
-    }
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, (c->s3 ? 1 : 0) | (c->s2 ? 6 : 0));
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
+    LAUNCHCHK(c, "neighbour lists");
+    // sub-list sweeps, counting; their sizes are bounded by nm * cap_nb
+    const long max_sl = std::min<long>((long)nm * D.cap_nb, (long)c->cu_count * 32);
+    if (do3) hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 0);
+    if (do2) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 0);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 1, do3, do2);
     LAUNCHCHK(c, "list counts");
-    HIPCHK(c, hipMemcpyAsync(c->h_totals, D.totals, 16 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(c->h_totals, D.totals, T_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
-    if (c->h_totals[3] > 0) {
-      // neighbour capacity exceeded: grow and rebuild once
-      if (attempt == 1 || c->h_totals[3] > 65536)
+    if (c->h_totals[T_NBOVER] > 0) {
+      // a neighbour list overflowed: grow its capacity and rebuild once
+      if (attempt == 1 || c->h_totals[T_NBOVER] > 65536)
         return fail(c, CCPOL3B_ECAPACITY, "neighbour list capacity exceeded");
-      D.cap_nb = ((c->h_totals[3] + 63) / 64) * 64 + 64;
-      dev_free(D.nb3); dev_free(D.nb2);
-      if ((rc = dev_alloc(c, D.nb3, (size_t)c->cap_nm * D.cap_nb)) ||
-          (rc = dev_alloc(c, D.nb2, (size_t)c->cap_nm * D.cap_nb)))
-        return rc;
-      HIPCHK(c, hipMemsetAsync(D.totals, 0, 16 * sizeof(int), st));
+      D.cap_nb = ((c->h_totals[T_NBOVER] + 63) / 64) * 64 + 64;
+      c->cap_nm = 0;  // force reallocation with the new capacity
+      free_dev(c);
+      c->D.cap3 = c->D.cap_items2 = 0;
+      c->cap_box = 0;
+      if ((rc = ensure_molecules(c, nm, nbox))) return rc;
+      HIPCHK(c, hipMemcpyAsync(D.mol_box, c->h_molbox.data(), sizeof(int) * nm, hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(D.boxL, c->h_boxL.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(D.box_start, c->h_boxstart.data(), sizeof(int) * (nbox + 1),
+                               hipMemcpyHostToDevice, st));
       continue;
     }
     break;
   }
-  const long n3 = c->s3 ? c->h_totals[0] : 0, np2 = c->s2 ? c->h_totals[1] : 0;
-  const long ni2 = c->s2 ? c->h_totals[2] : 0;
-  if ((rc = ensure_lists(c, n3, np2, ni2))) return rc;
-  if (c->s3 && n3 > 0) hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K, 1);
-  if (c->s2) {
-    hipLaunchKernelGGL(k_fs2_pairs, dim3((nm + 255) / 256), dim3(256), 0, st, D, 1);
-    if (ni2 > 0) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K, 1);
+  const long n3 = do3 ? c->h_totals[T_N3] : 0, ni2 = do2 ? c->h_totals[T_NI2] : 0;
+  const long np2 = do2 ? c->h_totals[T_NP2] : 0;
+  const long nsl3 = c->h_totals[T_SL3], nsl2 = c->h_totals[T_SL2];
+  if ((rc = ensure_lists(c, n3, ni2))) return rc;
+  if (do3 && nsl3 > 0) hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(nsl3)), dim3(256), 0, st, D, c->K, 1);
+  if (do2) {
+    hipLaunchKernelGGL(k_fs2_pairs, dim3((nm + 255) / 256), dim3(256), 0, st, D);
+    if (nsl2 > 0) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nsl2)), dim3(256), 0, st, D, c->K, 1);
+    if (np2 > 0) HIPCHK(c, hipMemsetAsync(D.wt2, 0, sizeof(double) * np2, st));
   }
+  hipLaunchKernelGGL(k_box_ranges, dim3((nbox + 1 + 255) / 256), dim3(256), 0, st, D, do3, do2);
   LAUNCHCHK(c, "list fill");
   HIPCHK(c, hipEventRecord(c->ev[1], st));
-  if (c->s2) {
-    if (np2 > 0) hipLaunchKernelGGL(k_fs2_pair, dim3(waves_grid(np2)), dim3(256), 0, st, D, c->K);
-    hipLaunchKernelGGL(k_fs2_spec, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K);
-    hipLaunchKernelGGL(k_fs2_gather, dim3(waves_grid(nm)), dim3(256), 0, st, D, ld, fxs, fys, fzs, fcom2);
+  if (do2 && ni2 > 0) {
+    hipLaunchKernelGGL(k_fs2_items_eval, dim3((ni2 + 255) / 256), dim3(256), 0, st, D, c->K);
+    hipLaunchKernelGGL(k_fs2_pair_sites, dim3((np2 + 255) / 256), dim3(256), 0, st, D, c->K);
     LAUNCHCHK(c, "FS2");
   }
   HIPCHK(c, hipEventRecord(c->ev[2], st));
-  if (c->s3) {
-    if (n3 > 0) {
-      const int grid = (int)std::min<long>(n3, (long)c->cu_count * 8);
-      if (c->precision == CCPOL3B_MIXED)
-        hipLaunchKernelGGL(k_fs3_eval<float>, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
-                           (const float*)c->d_coef32);
-      else
-        hipLaunchKernelGGL(k_fs3_eval<double>, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
-                           (const double*)c->d_coef64);
-      LAUNCHCHK(c, "k_fs3_eval");
-    }
-    HIPCHK(c, hipEventRecord(c->ev[3], st));
-    hipLaunchKernelGGL(k_fs3_gather, dim3(waves_grid(nm)), dim3(256), 0, st, D, ld, fxs, fys, fzs, fcom3);
-    LAUNCHCHK(c, "k_fs3_gather");
-  } else {
-    HIPCHK(c, hipEventRecord(c->ev[3], st));
+  if (do3 && n3 > 0) {
+    const int grid = (int)std::min<long>(n3, (long)c->cu_count * 8);
+    if (c->precision == CCPOL3B_MIXED)
+      hipLaunchKernelGGL(k_fs3_eval<float>, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+                         (const float*)c->d_coef32);
+    else
+      hipLaunchKernelGGL(k_fs3_eval<double>, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+                         (const double*)c->d_coef64);
+    LAUNCHCHK(c, "k_fs3_eval");
   }
-  hipLaunchKernelGGL(k_energy, dim3(nbox), dim3(256), 0, st, D, c->s2, c->s3, energies);
-  LAUNCHCHK(c, "k_energy");
+  HIPCHK(c, hipEventRecord(c->ev[3], st));
+  const long nfin = (long)nm * ACC_W;
+  hipLaunchKernelGGL(k_finish, dim3((nfin + 255) / 256), dim3(256), 0, st, D, ld, do3 ? NS3 : (do2 ? NS2 : 0),
+                     do2, do3, fxs, fys, fzs, fcom2, fcom3);
+  hipLaunchKernelGGL(k_energy, dim3(nbox), dim3(1024), 0, st, D, do2, do3, energies);
+  LAUNCHCHK(c, "finish");
   HIPCHK(c, hipEventRecord(c->ev[4], st));
-  HIPCHK(c, hipMemcpyAsync(c->h_totals + 4, D.totals + 4, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(c->h_totals + T_HIST, D.totals + T_HIST, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
   c->stats.n_fs3 = n3;
   c->stats.n_fs2_pairs = np2;
   c->stats.n_fs2_items = ni2;
@@ -304,9 +311,9 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
 }
 
 void finish_stats(ccpol3b_ctx* c) {
-  c->stats.n_fs3_sr = c->h_totals[4];
-  c->stats.n_fs3_switch = c->h_totals[5];
-  c->stats.n_fs3_lr = c->h_totals[6];
+  c->stats.n_fs3_sr = c->h_totals[T_HIST];
+  c->stats.n_fs3_switch = c->h_totals[T_HIST + 1];
+  c->stats.n_fs3_lr = c->h_totals[T_HIST + 2];
   c->stats.ms_lists = event_ms(c->ev[0], c->ev[1]);
   c->stats.ms_fs2 = event_ms(c->ev[1], c->ev[2]);
   c->stats.ms_fs3 = event_ms(c->ev[2], c->ev[3]);
@@ -466,7 +473,7 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
   if (hipMemcpy(c->d_pairpar, pairpar.data(), sizeof(double4) * 2 * NPAIR, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_coef64, c64.data(), sizeof(double) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_coef32, c32.data(), sizeof(float) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_totals, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_totals, T_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     ccpol3b_destroy(c);
     return CCPOL3B_EHIP;
   }

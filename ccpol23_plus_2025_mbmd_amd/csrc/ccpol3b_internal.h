@@ -12,9 +12,19 @@ constexpr int NTYPE = 6;       // FS3 site types (O, H, D1, N1, N2, N3)
 constexpr int NPAIR = NS3 * NS3;      // 289 site pairs per molecule pair
 constexpr int NPAIR3 = 3 * NPAIR;     // 867 site pairs per triplet
 constexpr int NCOEF = NTYPE * NTYPE * NTYPE * 8;  // dense coefficients per region
-constexpr int SLOT3 = NS3 * 3 + 3;    // FS3 per-role output: 17 site forces + COM force
-constexpr int PSLOT2 = 2 + 2 * (3 * 3 + 3);  // FS2 per-pair slot: S, WT, i:(3 sites+COM), j:(...)
-constexpr int SPEC2 = NS2 * 3 + 3 + 2;       // FS2 per-spectator: 5 site forces, COM, E, W
+constexpr int ACC_W = NS3 * 3 + 6;     // per-molecule accumulator: 17 site forces, COM(FS2), COM(FS3)
+constexpr int ACC_C2 = NS3 * 3;        // offset of the FS2 COM force
+constexpr int ACC_C3 = NS3 * 3 + 3;    // offset of the FS3 COM force
+
+// indices into Dev::totals
+constexpr int T_N3 = 0;      // FS3 triplets
+constexpr int T_NP2 = 1;     // FS2 pairs
+constexpr int T_NI2 = 2;     // FS2 items
+constexpr int T_NBOVER = 3;  // largest neighbour count when a list overflowed
+constexpr int T_HIST = 4;    // [4..6] FS3 region histogram SR, switch, LR
+constexpr int T_SL3 = 7;     // FS3 sub-lists (i; j)
+constexpr int T_SL2 = 8;     // FS2 sub-lists (k; i)
+constexpr int T_COUNT = 16;
 
 constexpr double BOHR2A = 0.529177249;  // common_arrays.f90:34
 constexpr double H2KCAL = 627.510;      // common_arrays.f90:35
@@ -38,32 +48,34 @@ struct Consts {
 // Device arrays of one evaluation (sized by the context; see api.hip).
 struct Dev {
   int nm, nbox, cap_nb;
-  long cap3, cap_pairs2, cap_items2;
+  long cap3, cap_pairs2, cap_items2, cap_sl3, cap_sl2;
   // geometry in bohr
   double *cx, *cy, *cz;   // [nm]
   double *site;           // [nm][17][3] site offsets (bohr)
   int *mol_box;           // [nm]
   double *boxL;           // [nbox] (bohr, <= 0 open)
   int *box_start;         // [nbox+1]
-  // neighbour lists (ascending molecule index)
-  int *nb3_cnt, *nb3;     // FS3 cutoff: [nm], [nm][cap_nb]
-  int *nb2_cnt, *nb2;     // FS2 cutoff
+  // neighbour lists (ascending molecule index) and their first upper entries
+  int *nb3_cnt, *nb3, *up3;   // FS3 cutoff: [nm], [nm][cap_nb], [nm]
+  int *nb2_cnt, *nb2, *up2;   // FS2 cutoff
+  // sub-list counts/offsets: FS3 (i; j) and FS2 (k; i)
+  int *sl3_cnt, *sl3_off;     // [nm], [nm+1]
+  int *sl2_cnt, *sl2_off;
+  int *cnt3s, *off3s;         // [cap_sl3], [cap_sl3+1] triplets per FS3 sub-list
+  int *cnt2s, *off2s;         // [cap_sl2], [cap_sl2+1] items per FS2 sub-list
   // FS3 triplets (i<j<k, lexicographic)
-  int *cnt3, *off3;       // [nm], [nm+1]
-  int4 *tri;              // [cap3] (i,j,k,region)
+  int4 *tri;                  // [cap3] (i,j,k,-)
   // FS2 pairs (i<j within rcut3) and k-major items (k,i,j,pair)
-  int *pcnt2, *poff2;     // [nm], [nm+1]
-  int2 *pairs2;           // [cap_pairs2]
-  int *cnt2, *off2;       // [nm], [nm+1]
-  int4 *items2;           // [cap_items2]
-  // per-unit results (hartree, bohr)
-  double *slot3;          // [cap3][3][SLOT3]
-  double *ew3;            // [cap3][2] energy, virial
-  double *pslot2;         // [cap_pairs2][PSLOT2]
-  double *spec2;          // [nm][SPEC2]
-  // list totals and flags: [0] n3, [1] npairs2, [2] nitems2, [3] nb overflow,
-  // [4..6] region histogram
-  int *totals;
+  int *pcnt2, *poff2;         // [nm], [nm+1]
+  int2 *pairs2;               // [cap_pairs2]
+  int4 *items2;               // [cap_items2]
+  double *wt2;                // [cap_pairs2] sum over spectators of f f T
+  // results (hartree, bohr)
+  double *acc;                // [nm][ACC_W] force accumulators
+  double *ew3;                // [cap3][2] per-triplet energy, virial
+  double *ew2;                // [cap_items2/64+1][2] per-wavefront FS2 energy, virial
+  int *box_t3, *box_t2;       // [nbox+1] triplet / item ranges of each box
+  int *totals;                // [T_COUNT]
 };
 
 }  // namespace ccpol3b

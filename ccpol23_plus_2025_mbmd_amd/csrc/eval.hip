@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
   const int tid = threadIdx.x;
   for (int q = tid; q < 2 * NCOEF; q += 256) sC[q] = coef[q];
 
-  const long n3 = D.totals[0];
+  const long n3 = D.totals[T_N3];
   for (long t = blockIdx.x; t < n3; t += gridDim.x) {
     const int4 tr = D.tri[t];
     const int mi = tr.x, mj = tr.y, mk = tr.z;
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
       w0 = sm; wp0 = dsm;
       w1 = 1.0 - sm; wp1 = -dsm;
     }
-    if (tid == 0) atomicAdd(D.totals + 4 + (nreg == 2 ? 1 : 2 * reg0), 1);
+    if (tid == 0) atomicAdd(D.totals + T_HIST + (nreg == 2 ? 1 : 2 * reg0), 1);
 
     if (tid < 3 * NS3 * 3) {
       const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
@@ -261,7 +261,6 @@ __global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
       sR[p] = g / r;
     }
     const double Wsites = block_sum256(wsum, sRed);  // includes the barrier before reuse of sR
-    double* slot = D.slot3 + t * (3 * SLOT3);
     if (tid < 3 * NS3 * 3) {
       const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
       const int s = rem / 3, c = rem - s * 3;
@@ -285,7 +284,8 @@ __global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
         for (int y = 0; y < NS3; ++y)
           F += sR[2 * NPAIR + y * NS3 + s] * (ajk + (sP[1][y][c] - sP[2][s][c]));
       }
-      slot[role * SLOT3 + s * 3 + c] = F;
+      const int m = role == 0 ? mi : role == 1 ? mj : mk;
+      unsafeAtomicAdd(D.acc + (long)m * ACC_W + rem, F);
     } else if (tid < 3 * NS3 * 3 + 9) {
       // COM forces from the R_avg dependence of f_d and w (:1130-1166, 1613-1665)
       const int q = tid - 3 * NS3 * 3, role = q / 3, c = q - role * 3;
@@ -297,60 +297,14 @@ __global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
       if (role == 0) F = g3 * (aij / Rij + aik / Rik);
       else if (role == 1) F = g3 * (-aij / Rij + ajk / Rjk);
       else F = g3 * (-aik / Rik - ajk / Rjk);
-      slot[role * SLOT3 + NS3 * 3 + c] = F;
+      const int m = role == 0 ? mi : role == 1 ? mj : mk;
+      unsafeAtomicAdd(D.acc + (long)m * ACC_W + ACC_C3 + c, F);
     }
     if (tid == 0) {
       D.ew3[2 * t] = Et;
       D.ew3[2 * t + 1] = Wsites + Ravg * dEdR;
     }
     __syncthreads();
-  }
-}
-
-// FS3 gather: one wavefront per molecule m.  Its triplets are those of its own
-// segment (role i) and, for each neighbour i < m, the entries of segment i
-// with j == m or k == m; they are summed in ascending (i, t) order so the
-// result is bitwise reproducible.  Adds to the site forces, sets COM forces.
-__global__ __launch_bounds__(256) void k_fs3_gather(Dev D, int ld, double* fxs, double* fys,
-                                                     double* fzs, double* fcom) {
-  const int m = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (m >= D.nm) return;
-  const int* nb = D.nb3 + (long)m * D.cap_nb;
-  const int n = D.nb3_cnt[m];
-  int u0 = 0;
-  while (u0 < n && nb[u0] < m) ++u0;
-  double acc = 0.0;
-  for (int a = 0; a <= u0; ++a) {
-    const int i = a < u0 ? nb[a] : m;
-    const int s0 = D.off3[i], s1 = D.off3[i + 1];
-    for (int base = s0; base < s1; base += 64) {
-      const int t = base + lane;
-      int role = -1;
-      if (t < s1) {
-        if (i == m) role = 0;
-        else {
-          const int4 q = D.tri[t];
-          role = q.y == m ? 1 : q.z == m ? 2 : -1;
-        }
-      }
-      uint64_t mask = __ballot(role >= 0);
-      while (mask) {
-        const int b = __ffsll((unsigned long long)mask) - 1;
-        mask &= mask - 1;
-        const int rb = __shfl(role, b, 64);
-        if (lane < SLOT3) acc += D.slot3[(long)(base + b) * 3 * SLOT3 + rb * SLOT3 + lane];
-      }
-    }
-  }
-  const double conv = H2KCAL * 418.4 / BOHR2A;
-  if (lane < NS3 * 3) {
-    const int s = lane / 3, c = lane - s * 3;
-    double* f = c == 0 ? fxs : c == 1 ? fys : fzs;
-    f[(long)ld * m + s] += acc * conv;
-  } else if (lane < SLOT3) {
-    const int c = lane - NS3 * 3;
-    fcom[(long)c * D.nm + m] = acc * conv;
   }
 }
 
@@ -442,14 +396,109 @@ __device__ __forceinline__ void fs2_geometry(const Dev& D, int k, int i, int j, 
   G.Rkj = sqrt(G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2]);
 }
 
-// Pair pass: one wavefront per FS2 pair (i<j).  Lanes 0..8 build S_ij over the
-// 3x3 O/H site pairs; then lanes run over the spectator candidates k in nb2(i).
-// Slot layout: [S, S_lambda, F_i sites 0..2 (9), F_COM_i (3), F_j sites (9), F_COM_j (3)].
-__global__ __launch_bounds__(256) void k_fs2_pair(Dev D, Consts K) {
-  const long p = (long)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+// Segmented inclusive sum over runs of equal keys within a wavefront (keys
+// form contiguous runs); the last lane of each run ends up with the run total.
+__device__ __forceinline__ double seg_sum(double v, long long key, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double vu = __shfl_up(v, d, 64);
+    const long long ku = __shfl_up(key, d, 64);
+    if (lane >= d && ku == key) v += vu;
+  }
+  return v;
+}
+
+// One thread per FS2 item (k; i<j) of the k-major list.  Writes the spectator
+// site forces and all three COM forces through wavefront-segmented sums and
+// FP64 atomics, accumulates sum_k f f T per pair for the exchange-site forces,
+// and stores the item's energy and virial for the deterministic box sums.
+__global__ __launch_bounds__(256) void k_fs2_items_eval(Dev D, Consts K) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const long np = D.totals[1];
-  if (p >= np) return;
+  const long nit = D.totals[T_NI2];
+  const bool live = t < nit;
+  int k = -1, i = -1, pr = -1;
+  double Fk[NS2 * 3 + 3], Fi[3], Fj[3], WT = 0.0, E = 0.0, W = 0.0;
+#pragma unroll
+  for (int a = 0; a < NS2 * 3 + 3; ++a) Fk[a] = 0.0;
+  Fi[0] = Fi[1] = Fi[2] = Fj[0] = Fj[1] = Fj[2] = 0.0;
+  int j = -1;
+  if (live) {
+    const int4 it = D.items2[t];
+    k = it.x; i = it.y; j = it.z; pr = it.w;
+    Fs2Geo G;
+    fs2_geometry(D, k, i, j, G);
+    // exchange sum S_ij = eta sum exp(-beta r) over the O/H site pairs (:361-372)
+    double S = 0.0, Sl = 0.0;
+    for (int is = 0; is < 3; ++is) {
+      const double* si = D.site + ((long)i * NS3 + is) * 3;
+      for (int js = 0; js < 3; ++js) {
+        const double* sj = D.site + ((long)j * NS3 + js) * 3;
+        const double vx = G.xij[0] + (si[0] - sj[0]);
+        const double vy = G.xij[1] + (si[1] - sj[1]);
+        const double vz = G.xij[2] + (si[2] - sj[2]);
+        const double r = sqrt(vx * vx + vy * vy + vz * vz);
+        const double e = ETA3 * exp(-BETA3 * r);
+        S += e;
+        Sl += -BETA3 * r * e;
+      }
+    }
+    double sk[NS2][3];
+    for (int s2 = 0; s2 < NS2; ++s2)
+      for (int c = 0; c < 3; ++c) sk[s2][c] = D.site[((long)k * NS3 + s2) * 3 + c];
+    double dTdsk[NS2][3], dTdRi[3], dTdRj[3], Tl;
+    const double T = fs2_T<true, true>(G, sk, dTdsk, dTdRi, dTdRj, Tl);
+    double fki, fpki, fkj, fpkj;
+    fs2_damp(G.Rki, K, fki, fpki);
+    fs2_damp(G.Rkj, K, fkj, fpkj);
+    const double ff = fki * fkj;
+    for (int s2 = 0; s2 < NS2; ++s2)
+      for (int c = 0; c < 3; ++c) Fk[s2 * 3 + c] = -ff * S * dTdsk[s2][c];
+    for (int c = 0; c < 3; ++c) {
+      const double uki = G.xki[c] / G.Rki, ukj = G.xkj[c] / G.Rkj;
+      Fk[NS2 * 3 + c] = -S * T * (fpki * fkj * uki + fki * fpkj * ukj);
+      Fi[c] = -ff * S * dTdRi[c] + S * T * fpki * fkj * uki;
+      Fj[c] = -ff * S * dTdRj[c] + S * T * fki * fpkj * ukj;
+    }
+    WT = ff * T;
+    E = ff * S * T;
+    W = ff * (Sl * T + S * Tl) + S * T * (G.Rki * fpki * fkj + G.Rkj * fki * fpkj);
+  }
+  // spectator part: runs of equal k (the list is k-major)
+  const int knext = __shfl_down(k, 1, 64);
+  const bool ktail = live && (lane == 63 || knext != k);
+#pragma unroll
+  for (int a = 0; a < NS2 * 3 + 3; ++a) {
+    const double v = seg_sum(Fk[a], (long long)k, lane);
+    if (ktail) {
+      const int off = a < NS2 * 3 ? a : ACC_C2 + (a - NS2 * 3);
+      unsafeAtomicAdd(D.acc + (long)k * ACC_W + off, v);
+    }
+  }
+  // COM of i: runs of equal (k, i)
+  const long long ki = live ? (long long)k * D.nm + i : -1;
+  const long long kinext = __shfl_down(ki, 1, 64);
+  const bool itail = live && (lane == 63 || kinext != ki);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double v = seg_sum(Fi[c], ki, lane);
+    if (itail) unsafeAtomicAdd(D.acc + (long)i * ACC_W + ACC_C2 + c, v);
+  }
+  if (live) {
+    for (int c = 0; c < 3; ++c) unsafeAtomicAdd(D.acc + (long)j * ACC_W + ACC_C2 + c, Fj[c]);
+    unsafeAtomicAdd(D.wt2 + pr, WT);
+    D.ew2[2 * t] = E;
+    D.ew2[2 * t + 1] = W;
+  }
+}
+
+// Exchange-site forces of FS2 (:477-519 for is/js): F_s = -(sum_k f f T) dS_ij/ds.
+// One thread per pair (i<j).
+__global__ __launch_bounds__(256) void k_fs2_pair_sites(Dev D, Consts K) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= D.totals[T_NP2]) return;
+  const double WT = D.wt2[p];
+  if (WT == 0.0) return;
   const int2 pr = D.pairs2[p];
   const int i = pr.x, j = pr.y;
   const double L = D.boxL[D.mol_box[i]];
@@ -457,205 +506,82 @@ __global__ __launch_bounds__(256) void k_fs2_pair(Dev D, Consts K) {
   xij[0] = minimg_e(D.cx[i] - D.cx[j], L);
   xij[1] = minimg_e(D.cy[i] - D.cy[j], L);
   xij[2] = minimg_e(D.cz[i] - D.cz[j], L);
-  // S_ij and dS/d(site) over the 9 site pairs
-  double e = 0.0, el = 0.0, gv[3] = {0.0, 0.0, 0.0};
-  if (lane < 9) {
-    const int is = lane / 3, js = lane - is * 3;
+  double Fi[3][3] = {}, Fj[3][3] = {};
+  for (int is = 0; is < 3; ++is) {
     const double* si = D.site + ((long)i * NS3 + is) * 3;
-    const double* sj = D.site + ((long)j * NS3 + js) * 3;
-    double v[3];
-    for (int c = 0; c < 3; ++c) v[c] = xij[c] + (si[c] - sj[c]);
-    const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    e = ETA3 * exp(-BETA3 * r);
-    el = -BETA3 * r * e;
-    for (int c = 0; c < 3; ++c) gv[c] = -BETA3 * e * v[c] / r;  // dS/dv
-  }
-  const double S = wave_sum(e);
-  const double Sl = wave_sum(el);
-  // dS/d(site is of i) = sum_js gv ; dS/d(site js of j) = -sum_is gv  (lanes 0..8)
-  double dSi[3], dSj[3];
-  {
-    const int is = lane / 3, js = lane - (lane / 3) * 3;
-    for (int c = 0; c < 3; ++c) {
-      const double g0 = __shfl(gv[c], 3 * (lane % 3) + 0, 64);
-      const double g1 = __shfl(gv[c], 3 * (lane % 3) + 1, 64);
-      const double g2 = __shfl(gv[c], 3 * (lane % 3) + 2, 64);
-      dSi[c] = (g0 + g1) + g2;  // lane%3 plays the role of is
-      const double h0 = __shfl(gv[c], 0 * 3 + (lane % 3), 64);
-      const double h1 = __shfl(gv[c], 1 * 3 + (lane % 3), 64);
-      const double h2 = __shfl(gv[c], 2 * 3 + (lane % 3), 64);
-      dSj[c] = -((h0 + h1) + h2);  // lane%3 plays the role of js
-    }
-    (void)is; (void)js;
-  }
-  // spectators
-  const int* nbi = D.nb2 + (long)i * D.cap_nb;
-  const int ni = D.nb2_cnt[i];
-  double Fi[3] = {0, 0, 0}, Fj[3] = {0, 0, 0}, WT = 0.0;
-  for (int base = 0; base < ni; base += 64) {
-    const int q = base + lane;
-    if (q < ni) {
-      const int k = nbi[q];
-      if (k != j) {
-        Fs2Geo G;
-        fs2_geometry(D, k, i, j, G);
-        const double dkj2 = G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2];
-        if (dkj2 < K.rc2_2) {
-          double sk[NS2][3];
-          for (int s = 0; s < NS2; ++s)
-            for (int c = 0; c < 3; ++c) sk[s][c] = D.site[((long)k * NS3 + s) * 3 + c];
-          double dTdRi[3], dTdRj[3], Tl;
-          const double T = fs2_T<false, true>(G, sk, nullptr, dTdRi, dTdRj, Tl);
-          double fki, fpki, fkj, fpkj;
-          fs2_damp(G.Rki, K, fki, fpki);
-          fs2_damp(G.Rkj, K, fkj, fpkj);
-          const double ff = fki * fkj;
-          WT += ff * T;
-          for (int c = 0; c < 3; ++c) {
-            Fi[c] += -ff * S * dTdRi[c] + S * T * fpki * fkj * G.xki[c] / G.Rki;
-            Fj[c] += -ff * S * dTdRj[c] + S * T * fki * fpkj * G.xkj[c] / G.Rkj;
-          }
-        }
+    for (int js = 0; js < 3; ++js) {
+      const double* sj = D.site + ((long)j * NS3 + js) * 3;
+      double v[3];
+      for (int c = 0; c < 3; ++c) v[c] = xij[c] + (si[c] - sj[c]);
+      const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      const double g = -BETA3 * ETA3 * exp(-BETA3 * r) / r;  // dS/dv = g v
+      for (int c = 0; c < 3; ++c) {
+        Fi[is][c] -= WT * g * v[c];
+        Fj[js][c] += WT * g * v[c];
       }
     }
   }
-  WT = wave_sum(WT);
-  for (int c = 0; c < 3; ++c) {
-    Fi[c] = wave_sum(Fi[c]);
-    Fj[c] = wave_sum(Fj[c]);
-  }
-  double* slot = D.pslot2 + p * PSLOT2;
-  if (lane == 0) {
-    slot[0] = S;
-    slot[1] = Sl;
+  for (int s2 = 0; s2 < 3; ++s2)
     for (int c = 0; c < 3; ++c) {
-      slot[2 + 9 + c] = Fi[c];
-      slot[14 + 9 + c] = Fj[c];
+      unsafeAtomicAdd(D.acc + (long)i * ACC_W + s2 * 3 + c, Fi[s2][c]);
+      unsafeAtomicAdd(D.acc + (long)j * ACC_W + s2 * 3 + c, Fj[s2][c]);
     }
-  }
-  if (lane < 3) {  // site forces of i and j: F = -WT dS/ds
-    for (int c = 0; c < 3; ++c) {
-      slot[2 + lane * 3 + c] = -WT * dSi[c];
-      slot[14 + lane * 3 + c] = -WT * dSj[c];
-    }
-  }
 }
 
-// Spectator pass: one wavefront per spectator k over its items (k-major list).
-// Output spec2[k] = [site forces of sites 0..4 (15), COM force (3), E, W].
-__global__ __launch_bounds__(256) void k_fs2_spec(Dev D, Consts K) {
-  const int k = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (k >= D.nm) return;
-  const int s0 = D.off2[k], s1 = D.off2[k + 1];
-  double sk[NS2][3];
-  for (int s = 0; s < NS2; ++s)
-    for (int c = 0; c < 3; ++c) sk[s][c] = D.site[((long)k * NS3 + s) * 3 + c];
-  double acc[SPEC2];
-  for (int a = 0; a < SPEC2; ++a) acc[a] = 0.0;
-  for (int base = s0; base < s1; base += 64) {
-    const int t = base + lane;
-    if (t < s1) {
-      const int4 it = D.items2[t];
-      const int i = it.y, j = it.z;
-      const double S = D.pslot2[(long)it.w * PSLOT2 + 0];
-      const double Sl = D.pslot2[(long)it.w * PSLOT2 + 1];
-      Fs2Geo G;
-      fs2_geometry(D, k, i, j, G);
-      double dTdsk[NS2][3], Tl;
-      const double T = fs2_T<true, false>(G, sk, dTdsk, nullptr, nullptr, Tl);
-      double fki, fpki, fkj, fpkj;
-      fs2_damp(G.Rki, K, fki, fpki);
-      fs2_damp(G.Rkj, K, fkj, fpkj);
-      const double ff = fki * fkj;
-      const double E = ff * S * T;
-      for (int s = 0; s < NS2; ++s)
-        for (int c = 0; c < 3; ++c) acc[s * 3 + c] -= ff * S * dTdsk[s][c];
-      for (int c = 0; c < 3; ++c)
-        acc[15 + c] -= S * T * (fpki * fkj * G.xki[c] / G.Rki + fki * fpkj * G.xkj[c] / G.Rkj);
-      acc[18] += E;
-      acc[19] += ff * (Sl * T + S * Tl) + S * T * (G.Rki * fpki * fkj + G.Rkj * fki * fpkj);
-    }
-  }
-  for (int a = 0; a < SPEC2; ++a) acc[a] = wave_sum(acc[a]);
-  if (lane < SPEC2) {
-    double v = 0.0;
-    for (int a = 0; a < SPEC2; ++a) v = a == lane ? acc[a] : v;
-    D.spec2[(long)k * SPEC2 + lane] = v;
-  }
-}
-
-__device__ __forceinline__ int find_sorted(const int* list, int n, int v) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (list[mid] < v) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
-// FS2 gather: one wavefront per molecule; lanes 0..17 own (sites 0..4, COM).
-// Order: spectator result, pairs (m, j) ascending j, pairs (i, m) ascending i.
-__global__ __launch_bounds__(256) void k_fs2_gather(Dev D, int ld, double* fxs, double* fys,
-                                                     double* fzs, double* fcom) {
-  const int m = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (m >= D.nm) return;
-  double acc = lane < 18 ? D.spec2[(long)m * SPEC2 + lane] : 0.0;
-  // slot component feeding this lane (sites 0..2 and COM only)
-  int sc = -1;
-  if (lane < 9) sc = lane;
-  else if (lane >= 15 && lane < 18) sc = 9 + (lane - 15);
-  const int p0 = D.poff2[m], p1 = D.poff2[m + 1];
-  for (int p = p0; p < p1; ++p)
-    if (sc >= 0) acc += D.pslot2[(long)p * PSLOT2 + 2 + sc];
-  const int* nb = D.nb2 + (long)m * D.cap_nb;
-  const int n = D.nb2_cnt[m];
-  for (int a = 0; a < n && nb[a] < m; ++a) {
-    const int i = nb[a];
-    const int* nbi = D.nb2 + (long)i * D.cap_nb;
-    const int ni = D.nb2_cnt[i];
-    const int u0 = find_sorted(nbi, ni, i + 1);
-    const int pos = find_sorted(nbi, ni, m);
-    const long p = (long)D.poff2[i] + (pos - u0);
-    if (sc >= 0) acc += D.pslot2[p * PSLOT2 + 14 + sc];
-  }
+// Accumulators (hartree/bohr) -> the caller's arrays in 10 J/mol/A: site
+// forces added (:574-585, 1964-1977), COM forces overwritten.
+__global__ void k_finish(Dev D, int ld, int nsite, int do2, int do3, double* fxs, double* fys,
+                         double* fzs, double* fcom2, double* fcom3) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)D.nm * ACC_W) return;
+  const int m = (int)(t / ACC_W), a = (int)(t - (long)m * ACC_W);
   const double conv = H2KCAL * 418.4 / BOHR2A;
-  if (lane < 15) {
-    const int s = lane / 3, c = lane - s * 3;
-    double* f = c == 0 ? fxs : c == 1 ? fys : fzs;
-    f[(long)ld * m + s] += acc * conv;
-  } else if (lane < 18) {
-    fcom[(long)(lane - 15) * D.nm + m] = acc * conv;
+  const double v = D.acc[t] * conv;
+  if (a < NS3 * 3) {
+    const int s = a / 3, c = a - s * 3;
+    if (s < nsite) {
+      double* f = c == 0 ? fxs : c == 1 ? fys : fzs;
+      f[(long)ld * m + s] += v;
+    }
+  } else if (a < ACC_C3) {
+    if (do2) fcom2[(long)(a - ACC_C2) * D.nm + m] = v;
+  } else if (do3) {
+    fcom3[(long)(a - ACC_C3) * D.nm + m] = v;
   }
 }
 
-// Per-box totals: {qfs2 (kcal/mol), qfs3 (kcal/mol), vir increment (10 J/mol)}.
-__global__ __launch_bounds__(256) void k_energy(Dev D, int do2, int do3, double* out) {
-  __shared__ double red[4];
+// Per-box totals {qfs2, qfs3 (kcal/mol), vir increment (10 J/mol)}; a fixed
+// reduction tree, so energies and virials are bitwise reproducible.
+__global__ __launch_bounds__(1024) void k_energy(Dev D, int do2, int do3, double* out) {
+  __shared__ double red[4][16];
   const int b = blockIdx.x;
-  const int m0 = D.box_start[b], m1 = D.box_start[b + 1];
-  double e2 = 0.0, w2 = 0.0, e3 = 0.0, w3 = 0.0;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (do2)
-    for (int m = m0 + threadIdx.x; m < m1; m += 256) {
-      e2 += D.spec2[(long)m * SPEC2 + 18];
-      w2 += D.spec2[(long)m * SPEC2 + 19];
+    for (long t = D.box_t2[b] + threadIdx.x; t < D.box_t2[b + 1]; t += 1024) {
+      v[0] += D.ew2[2 * t];
+      v[1] += D.ew2[2 * t + 1];
     }
-  if (do3) {
-    const long t0 = D.off3[m0], t1 = D.off3[m1];
-    for (long t = t0 + threadIdx.x; t < t1; t += 256) {
-      e3 += D.ew3[2 * t];
-      w3 += D.ew3[2 * t + 1];
+  if (do3)
+    for (long t = D.box_t3[b] + threadIdx.x; t < D.box_t3[b + 1]; t += 1024) {
+      v[2] += D.ew3[2 * t];
+      v[3] += D.ew3[2 * t + 1];
     }
+  const int w = threadIdx.x >> 6;
+  for (int q = 0; q < 4; ++q) {
+    const double s = wave_sum(v[q]);
+    if ((threadIdx.x & 63) == 0) red[q][w] = s;
   }
-  e2 = block_sum256(e2, red);
-  w2 = block_sum256(w2, red);
-  e3 = block_sum256(e3, red);
-  w3 = block_sum256(w3, red);
+  __syncthreads();
   if (threadIdx.x == 0) {
-    out[3 * b + 0] = e2 * H2KCAL;
-    out[3 * b + 1] = H2KCAL * e3;
-    out[3 * b + 2] = w2 * H2KCAL * 418.4 + w3 * H2KCAL * 418.4;
+    double tot[4];
+    for (int q = 0; q < 4; ++q) {
+      double s = 0.0;
+      for (int u = 0; u < 16; ++u) s += red[q][u];
+      tot[q] = s;
+    }
+    out[3 * b + 0] = tot[0] * H2KCAL;
+    out[3 * b + 1] = H2KCAL * tot[2];
+    out[3 * b + 2] = tot[1] * H2KCAL * 418.4 + tot[3] * H2KCAL * 418.4;
   }
 }
 

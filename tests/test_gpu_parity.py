@@ -145,21 +145,27 @@ def test_drop_in_semantics(tb_fs2fs3, params_fs2fs3):
     q2, q3, vir = tb_fs2fs3.threebody_term(box.x, box.y, box.z, box.xs, box.ys, box.zs, box.box, fx, fy, fz,
                                            *c, vir=5.0, txfs2=t[0], tyfs2=t[1], tzfs2=t[2], txfs3=t[3],
                                            tyfs3=t[4], tzfs3=t[5], ld=ld)
-    np.testing.assert_allclose(fx - f0[0], base["fxs"].reshape(-1), rtol=0, atol=1e-12)
-    np.testing.assert_array_equal(c[0], base["fxfs2"])
-    np.testing.assert_array_equal(c[3], base["fxfs3"])
+    # forces are summed with FP64 atomics: reproducible to round-off, not bitwise
+    scale = np.abs(base["fxs"]).max()
+    np.testing.assert_allclose(fx - f0[0], base["fxs"].reshape(-1), rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(c[0], base["fxfs2"], rtol=1e-12, atol=1e-12 * np.abs(base["fxfs2"]).max())
+    np.testing.assert_allclose(c[3], base["fxfs3"], rtol=1e-12, atol=1e-12 * np.abs(base["fxfs3"]).max())
     for a in t:
         assert not a.any()
     assert abs(vir - 5.0 - base["vir"]) < 1e-12 * max(1.0, abs(base["vir"]))
     assert q2 == base["qfs2"] and q3 == base["qfs3"]
 
 
-def test_repeatable_bitwise(tb_fs2fs3):
+def test_repeatable(tb_fs2fs3):
+    """Energies and virial use fixed reduction trees: bitwise repeatable.  Forces
+    are scattered with FP64 atomics: repeatable to round-off."""
     box = lattice_box(64, seed=9)
     a = tb_fs2fs3.evaluate(box)
     b = tb_fs2fs3.evaluate(box)
-    for k in ("fxs", "fys", "fzs", "fxfs2", "fxfs3", "qfs2", "qfs3", "vir"):
-        np.testing.assert_array_equal(a[k], b[k])
+    for k in ("qfs2", "qfs3", "vir"):
+        assert a[k] == b[k]
+    for k in ("fxs", "fys", "fzs", "fxfs2", "fxfs3"):
+        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-13 * np.abs(a[k]).max())
 
 
 def test_mixed_precision_error_512(params_fs2fs3, oracle):
@@ -211,6 +217,7 @@ def test_batch_matches_single(params_fs2fs3):
         fs1, c21, c31, en1 = tb.eval_device(cb, sb, b.box)
         torch.cuda.synchronize()
         sl = slice(starts[bi], starts[bi + 1])
-        assert torch.equal(fs[:, sl], fs1)
-        assert torch.equal(c2[:, sl], c21) and torch.equal(c3[:, sl], c31)
+        # forces: FP64 atomics, equal to round-off; energies: fixed trees, bitwise
+        for a, b_ in ((fs[:, sl], fs1), (c2[:, sl], c21), (c3[:, sl], c31)):
+            assert torch.allclose(a, b_, rtol=0, atol=1e-12 * float(b_.abs().max()))
         assert torch.equal(en[bi], en1)
