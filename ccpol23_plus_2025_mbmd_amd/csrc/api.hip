@@ -24,12 +24,14 @@ __global__ void k_fs2_pairs(Dev);
 __global__ void k_fs2_items(Dev, Consts, int);
 __global__ void k_scan(Dev, int, int, int);
 __global__ void k_box_ranges(Dev, int, int);
-template <typename T>
-__global__ void k_fs3_eval(Dev, Consts, const double4*, const T*);
+__global__ void k_fs3_eval_f64(Dev, Consts, const double4*, const double*);
+__global__ void k_fs3_eval_f32(Dev, Consts, const double4*, const float*);
 __global__ void k_fs2_items_eval(Dev, Consts);
+__global__ void k_fs2_pair_prep(Dev);
 __global__ void k_fs2_pair_sites(Dev, Consts);
 __global__ void k_finish(Dev, int, int, int, int, double*, double*, double*, double*, double*);
-__global__ void k_energy(Dev, int, int, double*);
+__global__ void k_energy_chunks(Dev, int, long);
+__global__ void k_energy_boxes(Dev, int, int, double*);
 }  // namespace ccpol3b
 
 using namespace ccpol3b;
@@ -159,6 +161,8 @@ void free_dev(ccpol3b_ctx* c) {
   dev_free(D.tri); dev_free(D.pcnt2); dev_free(D.poff2); dev_free(D.pairs2); dev_free(D.items2);
   dev_free(D.wt2); dev_free(D.acc); dev_free(D.ew3); dev_free(D.ew2);
   dev_free(D.box_t3); dev_free(D.box_t2); dev_free(D.totals);
+  dev_free(D.box_c3); dev_free(D.box_c2); dev_free(D.red3); dev_free(D.red2); dev_free(D.s2pair);
+  D.cap_red = 0;
 }
 
 // Per-molecule arrays for nm molecules and nbox boxes.  Sub-list and pair
@@ -180,6 +184,7 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
         (rc = dev_alloc(c, D.cnt3s, nn)) || (rc = dev_alloc(c, D.off3s, nn + 1)) ||
         (rc = dev_alloc(c, D.cnt2s, nn)) || (rc = dev_alloc(c, D.off2s, nn + 1)) ||
         (rc = dev_alloc(c, D.pairs2, nn)) || (rc = dev_alloc(c, D.wt2, nn)) ||
+        (rc = dev_alloc(c, D.s2pair, 2 * nn)) ||
         (rc = dev_alloc(c, D.acc, (size_t)n * ACC_W)))
       return rc;
     D.cap_sl3 = D.cap_sl2 = D.cap_pairs2 = (long)nn;
@@ -187,7 +192,8 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
   }
   if (nbox > c->cap_box) {
     if ((rc = dev_alloc(c, D.boxL, nbox)) || (rc = dev_alloc(c, D.box_start, nbox + 1)) ||
-        (rc = dev_alloc(c, D.box_t3, nbox + 1)) || (rc = dev_alloc(c, D.box_t2, nbox + 1)))
+        (rc = dev_alloc(c, D.box_t3, nbox + 1)) || (rc = dev_alloc(c, D.box_t2, nbox + 1)) ||
+        (rc = dev_alloc(c, D.box_c3, nbox + 1)) || (rc = dev_alloc(c, D.box_c2, nbox + 1)))
       return rc;
     c->cap_box = nbox;
   }
@@ -207,6 +213,11 @@ int ensure_lists(ccpol3b_ctx* c, long n3, long ni2) {
     const long n = std::max<long>(ni2 + ni2 / 8, 1024);
     if ((rc = dev_alloc(c, D.items2, n)) || (rc = dev_alloc(c, D.ew2, (size_t)n * 2))) return rc;
     D.cap_items2 = n;
+  }
+  const long nred = std::max(D.cap3, D.cap_items2) / RED_CHUNK + D.nbox + 2;
+  if (nred > D.cap_red) {
+    if ((rc = dev_alloc(c, D.red3, (size_t)nred * 2)) || (rc = dev_alloc(c, D.red2, (size_t)nred * 2))) return rc;
+    D.cap_red = nred;
   }
   return CCPOL3B_OK;
 }
@@ -277,22 +288,23 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     if (nsl2 > 0) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nsl2)), dim3(256), 0, st, D, c->K, 1);
     if (np2 > 0) HIPCHK(c, hipMemsetAsync(D.wt2, 0, sizeof(double) * np2, st));
   }
-  hipLaunchKernelGGL(k_box_ranges, dim3((nbox + 1 + 255) / 256), dim3(256), 0, st, D, do3, do2);
+  hipLaunchKernelGGL(k_box_ranges, dim3(1), dim3(64), 0, st, D, do3, do2);
   LAUNCHCHK(c, "list fill");
   HIPCHK(c, hipEventRecord(c->ev[1], st));
   if (do2 && ni2 > 0) {
+    hipLaunchKernelGGL(k_fs2_pair_prep, dim3((np2 * 16 + 255) / 256), dim3(256), 0, st, D);
     hipLaunchKernelGGL(k_fs2_items_eval, dim3((ni2 + 255) / 256), dim3(256), 0, st, D, c->K);
-    hipLaunchKernelGGL(k_fs2_pair_sites, dim3((np2 + 255) / 256), dim3(256), 0, st, D, c->K);
+    hipLaunchKernelGGL(k_fs2_pair_sites, dim3((np2 * 9 + 255) / 256), dim3(256), 0, st, D, c->K);
     LAUNCHCHK(c, "FS2");
   }
   HIPCHK(c, hipEventRecord(c->ev[2], st));
   if (do3 && n3 > 0) {
     const int grid = (int)std::min<long>(n3, (long)c->cu_count * 8);
     if (c->precision == CCPOL3B_MIXED)
-      hipLaunchKernelGGL(k_fs3_eval<float>, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+      hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
                          (const float*)c->d_coef32);
     else
-      hipLaunchKernelGGL(k_fs3_eval<double>, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+      hipLaunchKernelGGL(k_fs3_eval_f64, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
                          (const double*)c->d_coef64);
     LAUNCHCHK(c, "k_fs3_eval");
   }
@@ -300,7 +312,10 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   const long nfin = (long)nm * ACC_W;
   hipLaunchKernelGGL(k_finish, dim3((nfin + 255) / 256), dim3(256), 0, st, D, ld, do3 ? NS3 : (do2 ? NS2 : 0),
                      do2, do3, fxs, fys, fzs, fcom2, fcom3);
-  hipLaunchKernelGGL(k_energy, dim3(nbox), dim3(1024), 0, st, D, do2, do3, energies);
+  const long nch3 = n3 / RED_CHUNK + nbox + 1, nch2 = ni2 / RED_CHUNK + nbox + 1;
+  if (do3) hipLaunchKernelGGL(k_energy_chunks, dim3(nch3), dim3(256), 0, st, D, 3, nch3);
+  if (do2) hipLaunchKernelGGL(k_energy_chunks, dim3(nch2), dim3(256), 0, st, D, 2, nch2);
+  hipLaunchKernelGGL(k_energy_boxes, dim3((nbox + 63) / 64), dim3(64), 0, st, D, do2, do3, energies);
   LAUNCHCHK(c, "finish");
   HIPCHK(c, hipEventRecord(c->ev[4], st));
   HIPCHK(c, hipMemcpyAsync(c->h_totals + T_HIST, D.totals + T_HIST, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -427,6 +442,8 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
   Consts& K = c->K;
   K.alp3 = p->alpha3 * BOHR2A;
   K.delta33 = p->delta33;
+  K.idelta = (p->delta33 >= 1.0 && p->delta33 <= 64.0 && p->delta33 == std::floor(p->delta33))
+                 ? (int)p->delta33 : 0;
   K.r0dmp3 = 6.20446 / BOHR2A;
   K.r0dmp2 = 7.40446 / BOHR2A;
   K.rc2_3 = std::pow(6.3, 2.0) * std::pow(BOHR2A, -2.0);

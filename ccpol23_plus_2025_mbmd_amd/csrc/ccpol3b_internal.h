@@ -25,6 +25,7 @@ constexpr int T_HIST = 4;    // [4..6] FS3 region histogram SR, switch, LR
 constexpr int T_SL3 = 7;     // FS3 sub-lists (i; j)
 constexpr int T_SL2 = 8;     // FS2 sub-lists (k; i)
 constexpr int T_COUNT = 16;
+constexpr int RED_CHUNK = 2048;  // entries per block in the first level of the energy reduction
 
 constexpr double BOHR2A = 0.529177249;  // common_arrays.f90:34
 constexpr double H2KCAL = 627.510;      // common_arrays.f90:35
@@ -38,6 +39,7 @@ __host__ __device__ constexpr int site_type(int s) {
 struct Consts {
   double alp3;     // alpha3 * bohr2a (1/bohr)
   double delta33;
+  int idelta;      // delta33 when it is a whole number in [1, 64], else 0
   double r0dmp3;   // 6.20446 A in bohr (FS3 damping centre, :772)
   double r0dmp2;   // 7.40446 A in bohr (FS2 damping centre, :191)
   double rc2_3;    // FS3 COM cutoff^2 (bohr^2), :770
@@ -75,6 +77,10 @@ struct Dev {
   double *ew3;                // [cap3][2] per-triplet energy, virial
   double *ew2;                // [cap_items2/64+1][2] per-wavefront FS2 energy, virial
   int *box_t3, *box_t2;       // [nbox+1] triplet / item ranges of each box
+  int *box_c3, *box_c2;       // [nbox+1] first reduction chunk of each box
+  double *red3, *red2;        // [chunks][2] first-level energy/virial partial sums
+  long cap_red;               // capacity of red3/red2 in chunks
+  double *s2pair;             // [cap_pairs2][2] S_ij and dS/dlambda of each FS2 pair
   int *totals;                // [T_COUNT]
 };
 

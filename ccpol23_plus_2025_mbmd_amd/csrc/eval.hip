@@ -50,6 +50,20 @@ template <typename T> __device__ __forceinline__ T t_exp(T x);
 template <> __device__ __forceinline__ double t_exp<double>(double x) { return exp(x); }
 template <> __device__ __forceinline__ float t_exp<float>(float x) { return __expf(x); }
 
+// x**delta for the damping exponent: exponentiation by squaring when delta is
+// a small whole number (10 in both shipped CONTROL files), pow otherwise.
+__device__ __forceinline__ double pow_delta(double x, const Consts& K) {
+  if (K.idelta > 0) {
+    double r = 1.0, b = x;
+    for (int e = K.idelta; e; e >>= 1) {
+      if (e & 1) r *= b;
+      b *= b;
+    }
+    return r;
+  }
+  return pow(x, K.delta33);
+}
+
 // 2x2 site tiles of one molecule's 17 sites, aligned to the type blocks
 // (O | H H | D1 D1 | N1 N1 | N1 N1 | N2 N2 | N2 N2 | N3 N3 | N3 N3); -1 pads O.
 __constant__ int c_tile0[9] = {0, 1, 3, 5, 7, 9, 11, 13, 15};
@@ -61,10 +75,19 @@ __constant__ int c_z1[6] = {1, 3, 5, 9, 13, 17};
 // ====================================================================== FS3
 // One 256-thread block per triplet (grid-stride).  pairpar: [2][289] (beta,
 // gamma, r0, -) for SR and LR; coef: [2][NCOEF] dense [tx][ty][tz][p][q][s].
+// Block-uniform data of the current triplet, kept in LDS rather than in every
+// thread's registers.
+struct TriGeo {
+  double c[3][3];    // COM vectors x_ij, x_ik, x_jk (bohr), as the reference forms them
+  double R[3];       // |x_ij|, |x_ik|, |x_jk|
+  double Ravg, fd, dfd;
+  double w[2], wp[2];  // region weights and their R_avg derivatives
+  int nreg, reg0, m[3];
+};
+
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
-                                                      const double4* __restrict__ pairpar,
-                                                      const T* __restrict__ coef) {
+__device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __restrict__ pairpar,
+                                              const T* __restrict__ coef) {
   using T2 = typename Pair2<T>::type;
   __shared__ T sC[2 * NCOEF];
   __shared__ double sP[3][NS3][3];
@@ -73,67 +96,67 @@ __global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
   __shared__ double sR[NPAIR3];
   __shared__ T sG[NPAIR3];
   __shared__ double sRed[4];
+  __shared__ TriGeo sT;
 
   const int tid = threadIdx.x;
   for (int q = tid; q < 2 * NCOEF; q += 256) sC[q] = coef[q];
 
   const long n3 = D.totals[T_N3];
   for (long t = blockIdx.x; t < n3; t += gridDim.x) {
-    const int4 tr = D.tri[t];
-    const int mi = tr.x, mj = tr.y, mk = tr.z;
-    const double L = D.boxL[D.mol_box[mi]];
-    // COM vectors as the reference forms them (:862-884)
-    double cij[3], cik[3], cjk[3];
-    {
+    if (tid == 0) {
+      const int4 tr = D.tri[t];
+      const int mi = tr.x, mj = tr.y, mk = tr.z;
+      const double L = D.boxL[D.mol_box[mi]];
+      // COM vectors as the reference forms them (:862-884)
       const double pi_[3] = {D.cx[mi], D.cy[mi], D.cz[mi]};
       const double pj_[3] = {D.cx[mj], D.cy[mj], D.cz[mj]};
       const double pk_[3] = {D.cx[mk], D.cy[mk], D.cz[mk]};
-#pragma unroll
       for (int c = 0; c < 3; ++c) {
         const double dij = pi_[c] - pj_[c], dik = pi_[c] - pk_[c], djk = pj_[c] - pk_[c];
         if (L > 0.0) {
           const double nij = round(dij / L), nik = round(dik / L);
-          cij[c] = dij - L * nij;
-          cik[c] = dik - L * nik;
-          cjk[c] = djk + L * nij - L * nik;
+          sT.c[0][c] = dij - L * nij;
+          sT.c[1][c] = dik - L * nik;
+          sT.c[2][c] = djk + L * nij - L * nik;
         } else {
-          cij[c] = dij;
-          cik[c] = dik;
-          cjk[c] = djk;
+          sT.c[0][c] = dij;
+          sT.c[1][c] = dik;
+          sT.c[2][c] = djk;
         }
       }
-    }
-    const double Rij = sqrt(cij[0] * cij[0] + cij[1] * cij[1] + cij[2] * cij[2]);
-    const double Rik = sqrt(cik[0] * cik[0] + cik[1] * cik[1] + cik[2] * cik[2]);
-    const double Rjk = sqrt(cjk[0] * cjk[0] + cjk[1] * cjk[1] + cjk[2] * cjk[2]);
-    const double Ravg = (Rij + Rjk + Rik) / 3.0;
-    const double pom = exp(K.alp3 * (Ravg - K.r0dmp3));
-    const double pp = 1.0 / (1.0 + pom);
-    const double fd = pow(pp, K.delta33);
-    const double dfd = -K.alp3 * K.delta33 * pom * pp * fd;  // d f_d / d R_avg
-    int nreg, reg0;
-    double w0 = 1.0, wp0 = 0.0, w1 = 0.0, wp1 = 0.0;
-    if (Ravg <= K.rmin) {
-      nreg = 1; reg0 = 0;
-    } else if (Ravg >= K.rmax) {
-      nreg = 1; reg0 = 1;
-    } else {
-      const double span = K.rmax - K.rmin;
-      const double c = (Ravg - K.rmin) / span;
-      const double sm = 1.0 + 2.0 * c * c * c - 3.0 * c * c;
-      const double dsm = (6.0 * c * c - 6.0 * c) / span;
-      nreg = 2; reg0 = 0;
-      w0 = sm; wp0 = dsm;
-      w1 = 1.0 - sm; wp1 = -dsm;
-    }
-    if (tid == 0) atomicAdd(D.totals + T_HIST + (nreg == 2 ? 1 : 2 * reg0), 1);
-
-    if (tid < 3 * NS3 * 3) {
-      const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
-      const int m = role == 0 ? mi : role == 1 ? mj : mk;
-      (&sP[0][0][0])[tid] = D.site[(long)m * NS3 * 3 + rem];
+      for (int b = 0; b < 3; ++b)
+        sT.R[b] = sqrt(sT.c[b][0] * sT.c[b][0] + sT.c[b][1] * sT.c[b][1] + sT.c[b][2] * sT.c[b][2]);
+      const double Ravg = (sT.R[0] + sT.R[2] + sT.R[1]) / 3.0;
+      const double pom = exp(K.alp3 * (Ravg - K.r0dmp3));
+      const double pp = 1.0 / (1.0 + pom);
+      const double fd = pow_delta(pp, K);
+      sT.Ravg = Ravg;
+      sT.fd = fd;
+      sT.dfd = -K.alp3 * K.delta33 * pom * pp * fd;  // d f_d / d R_avg
+      sT.w[0] = 1.0; sT.wp[0] = 0.0; sT.w[1] = 0.0; sT.wp[1] = 0.0;
+      if (Ravg <= K.rmin) {
+        sT.nreg = 1; sT.reg0 = 0;
+      } else if (Ravg >= K.rmax) {
+        sT.nreg = 1; sT.reg0 = 1;
+      } else {  // cubic switch (:1420-1423)
+        const double span = K.rmax - K.rmin;
+        const double c = (Ravg - K.rmin) / span;
+        const double sm = 1.0 + 2.0 * c * c * c - 3.0 * c * c;
+        const double dsm = (6.0 * c * c - 6.0 * c) / span;
+        sT.nreg = 2; sT.reg0 = 0;
+        sT.w[0] = sm; sT.wp[0] = dsm;
+        sT.w[1] = 1.0 - sm; sT.wp[1] = -dsm;
+      }
+      sT.m[0] = mi; sT.m[1] = mj; sT.m[2] = mk;
+      atomicAdd(D.totals + T_HIST + (sT.nreg == 2 ? 1 : 2 * sT.reg0), 1);
     }
     for (int p = tid; p < NPAIR3; p += 256) sG[p] = T(0);
+    __syncthreads();
+    if (tid < 3 * NS3 * 3) {
+      const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
+      (&sP[0][0][0])[tid] = D.site[(long)sT.m[role] * NS3 * 3 + rem];
+    }
+    const int nreg = sT.nreg, reg0 = sT.reg0;
     __syncthreads();
 
     double eacc0 = 0.0, eacc1 = 0.0;
@@ -144,12 +167,9 @@ __global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
         const int blk = p / NPAIR, rem = p - blk * NPAIR;
         const int x = rem / NS3, y = rem - x * NS3;
         const int ra = blk == 2 ? 1 : 0, rb = blk == 0 ? 1 : 2;
-        const double cvx = blk == 0 ? cij[0] : blk == 1 ? cik[0] : cjk[0];
-        const double cvy = blk == 0 ? cij[1] : blk == 1 ? cik[1] : cjk[1];
-        const double cvz = blk == 0 ? cij[2] : blk == 1 ? cik[2] : cjk[2];
-        const double vx = cvx + (sP[ra][x][0] - sP[rb][y][0]);
-        const double vy = cvy + (sP[ra][x][1] - sP[rb][y][1]);
-        const double vz = cvz + (sP[ra][x][2] - sP[rb][y][2]);
+        const double vx = sT.c[blk][0] + (sP[ra][x][0] - sP[rb][y][0]);
+        const double vy = sT.c[blk][1] + (sP[ra][x][1] - sP[rb][y][1]);
+        const double vz = sT.c[blk][2] + (sP[ra][x][2] - sP[rb][y][2]);
         const double r = sqrt(vx * vx + vy * vy + vz * vz);
         const double4 par = pairpar[X * NPAIR + rem];
         const T rt = T(r);
@@ -222,7 +242,7 @@ __global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
                   for (int s = 0; s < 2; ++s)
                     G[xs][ys][s] += cc[(p * 2 + q) * 2 + s] * M[xs][ys][p][q];
         }
-        const T fdw = T(fd * (rr == 0 ? w0 : w1));
+        const T fdw = T(sT.fd * sT.w[rr]);
 #pragma unroll
         for (int xs = 0; xs < 2; ++xs) {
           if (xs == 1 && x1r < 0) continue;
@@ -248,10 +268,9 @@ __global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
     // region energies (FP64 reduction)
     const double EX0 = block_sum256(eacc0, sRed);
     const double EX1 = nreg == 2 ? block_sum256(eacc1, sRed) : 0.0;
-    const double Eb = w0 * EX0 + w1 * EX1;
-    double dEdR = wp0 * EX0 + wp1 * EX1;
-    const double Et = fd * Eb;
-    dEdR = dfd * Eb + fd * dEdR;  // dE/dR_avg
+    const double Eb = sT.w[0] * EX0 + sT.w[1] * EX1;
+    const double Et = sT.fd * Eb;
+    const double dEdR = sT.dfd * Eb + sT.fd * (sT.wp[0] * EX0 + sT.wp[1] * EX1);  // dE/dR_avg
     // ---------------- chain rule: h = (dE/dr)/r; site virial sum r dE/dr
     double wsum = 0.0;
     for (int p = tid; p < NPAIR3; p += 256) {
@@ -264,45 +283,45 @@ __global__ __launch_bounds__(256, 2) void k_fs3_eval(Dev D, Consts K,
     if (tid < 3 * NS3 * 3) {
       const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
       const int s = rem / 3, c = rem - s * 3;
-      const double aij = c == 0 ? cij[0] : c == 1 ? cij[1] : cij[2];
-      const double aik = c == 0 ? cik[0] : c == 1 ? cik[1] : cik[2];
-      const double ajk = c == 0 ? cjk[0] : c == 1 ? cjk[1] : cjk[2];
+      const double aij = sT.c[0][c], aik = sT.c[1][c], ajk = sT.c[2][c];
       double F = 0.0;
       if (role == 0) {
+#pragma unroll 4
         for (int y = 0; y < NS3; ++y)
           F -= sR[s * NS3 + y] * (aij + (sP[0][s][c] - sP[1][y][c]));
+#pragma unroll 4
         for (int z = 0; z < NS3; ++z)
           F -= sR[NPAIR + s * NS3 + z] * (aik + (sP[0][s][c] - sP[2][z][c]));
       } else if (role == 1) {
+#pragma unroll 4
         for (int x = 0; x < NS3; ++x)
           F += sR[x * NS3 + s] * (aij + (sP[0][x][c] - sP[1][s][c]));
+#pragma unroll 4
         for (int z = 0; z < NS3; ++z)
           F -= sR[2 * NPAIR + s * NS3 + z] * (ajk + (sP[1][s][c] - sP[2][z][c]));
       } else {
+#pragma unroll 4
         for (int x = 0; x < NS3; ++x)
           F += sR[NPAIR + x * NS3 + s] * (aik + (sP[0][x][c] - sP[2][s][c]));
+#pragma unroll 4
         for (int y = 0; y < NS3; ++y)
           F += sR[2 * NPAIR + y * NS3 + s] * (ajk + (sP[1][y][c] - sP[2][s][c]));
       }
-      const int m = role == 0 ? mi : role == 1 ? mj : mk;
-      unsafeAtomicAdd(D.acc + (long)m * ACC_W + rem, F);
+      unsafeAtomicAdd(D.acc + (long)sT.m[role] * ACC_W + rem, F);
     } else if (tid < 3 * NS3 * 3 + 9) {
       // COM forces from the R_avg dependence of f_d and w (:1130-1166, 1613-1665)
       const int q = tid - 3 * NS3 * 3, role = q / 3, c = q - role * 3;
-      const double aij = c == 0 ? cij[0] : c == 1 ? cij[1] : cij[2];
-      const double aik = c == 0 ? cik[0] : c == 1 ? cik[1] : cik[2];
-      const double ajk = c == 0 ? cjk[0] : c == 1 ? cjk[1] : cjk[2];
+      const double uij = sT.c[0][c] / sT.R[0], uik = sT.c[1][c] / sT.R[1], ujk = sT.c[2][c] / sT.R[2];
       const double g3 = -dEdR / 3.0;
       double F;
-      if (role == 0) F = g3 * (aij / Rij + aik / Rik);
-      else if (role == 1) F = g3 * (-aij / Rij + ajk / Rjk);
-      else F = g3 * (-aik / Rik - ajk / Rjk);
-      const int m = role == 0 ? mi : role == 1 ? mj : mk;
-      unsafeAtomicAdd(D.acc + (long)m * ACC_W + ACC_C3 + c, F);
+      if (role == 0) F = g3 * (uij + uik);
+      else if (role == 1) F = g3 * (-uij + ujk);
+      else F = g3 * (-uik - ujk);
+      unsafeAtomicAdd(D.acc + (long)sT.m[role] * ACC_W + ACC_C3 + c, F);
     }
     if (tid == 0) {
       D.ew3[2 * t] = Et;
-      D.ew3[2 * t + 1] = Wsites + Ravg * dEdR;
+      D.ew3[2 * t + 1] = Wsites + sT.Ravg * dEdR;
     }
     __syncthreads();
   }
@@ -330,7 +349,7 @@ struct Fs2Geo {
 __device__ __forceinline__ void fs2_damp(double R, const Consts& K, double& f, double& fp) {
   const double pom = exp(K.alp3 * (R - K.r0dmp2));
   const double pp = 1.0 / (1.0 + pom);
-  f = pow(pp, K.delta33);
+  f = pow_delta(pp, K);
   fp = -K.alp3 * K.delta33 * pom * pp * f;  // df/dR
 }
 
@@ -428,21 +447,7 @@ __global__ __launch_bounds__(256) void k_fs2_items_eval(Dev D, Consts K) {
     k = it.x; i = it.y; j = it.z; pr = it.w;
     Fs2Geo G;
     fs2_geometry(D, k, i, j, G);
-    // exchange sum S_ij = eta sum exp(-beta r) over the O/H site pairs (:361-372)
-    double S = 0.0, Sl = 0.0;
-    for (int is = 0; is < 3; ++is) {
-      const double* si = D.site + ((long)i * NS3 + is) * 3;
-      for (int js = 0; js < 3; ++js) {
-        const double* sj = D.site + ((long)j * NS3 + js) * 3;
-        const double vx = G.xij[0] + (si[0] - sj[0]);
-        const double vy = G.xij[1] + (si[1] - sj[1]);
-        const double vz = G.xij[2] + (si[2] - sj[2]);
-        const double r = sqrt(vx * vx + vy * vy + vz * vz);
-        const double e = ETA3 * exp(-BETA3 * r);
-        S += e;
-        Sl += -BETA3 * r * e;
-      }
-    }
+    const double S = D.s2pair[2 * (long)pr], Sl = D.s2pair[2 * (long)pr + 1];
     double sk[NS2][3];
     for (int s2 = 0; s2 < NS2; ++s2)
       for (int c = 0; c < 3; ++c) sk[s2][c] = D.site[((long)k * NS3 + s2) * 3 + c];
@@ -492,40 +497,63 @@ __global__ __launch_bounds__(256) void k_fs2_items_eval(Dev D, Consts K) {
   }
 }
 
+// Exchange sum of every FS2 pair (i<j): S_ij = eta sum_{is,js in O,H,H} exp(-beta r)
+// (:361-372) and its scaling derivative sum -beta r eta exp(-beta r); one
+// thread per (pair, site pair) and a 9-lane sum.
+__global__ __launch_bounds__(256) void k_fs2_pair_prep(Dev D) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long p = g / 16;
+  const int q = (int)(g - p * 16);
+  const long np = D.totals[T_NP2];
+  double e = 0.0, el = 0.0;
+  if (p < np && q < 9) {
+    const int2 pr = D.pairs2[p];
+    const int is = q / 3, js = q - is * 3;
+    const double L = D.boxL[D.mol_box[pr.x]];
+    const double* si = D.site + ((long)pr.x * NS3 + is) * 3;
+    const double* sj = D.site + ((long)pr.y * NS3 + js) * 3;
+    const double vx = minimg_e(D.cx[pr.x] - D.cx[pr.y], L) + (si[0] - sj[0]);
+    const double vy = minimg_e(D.cy[pr.x] - D.cy[pr.y], L) + (si[1] - sj[1]);
+    const double vz = minimg_e(D.cz[pr.x] - D.cz[pr.y], L) + (si[2] - sj[2]);
+    const double r = sqrt(vx * vx + vy * vy + vz * vz);
+    e = ETA3 * exp(-BETA3 * r);
+    el = -BETA3 * r * e;
+  }
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) {
+    e += __shfl_xor(e, o, 16);
+    el += __shfl_xor(el, o, 16);
+  }
+  if (p < np && q == 0) {
+    D.s2pair[2 * p] = e;
+    D.s2pair[2 * p + 1] = el;
+  }
+}
+
 // Exchange-site forces of FS2 (:477-519 for is/js): F_s = -(sum_k f f T) dS_ij/ds.
-// One thread per pair (i<j).
+// One thread per (pair, site pair).
 __global__ __launch_bounds__(256) void k_fs2_pair_sites(Dev D, Consts K) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long p = g / 9;
+  const int q = (int)(g - p * 9);
   if (p >= D.totals[T_NP2]) return;
   const double WT = D.wt2[p];
   if (WT == 0.0) return;
   const int2 pr = D.pairs2[p];
-  const int i = pr.x, j = pr.y;
+  const int i = pr.x, j = pr.y, is = q / 3, js = q - is * 3;
   const double L = D.boxL[D.mol_box[i]];
-  double xij[3];
-  xij[0] = minimg_e(D.cx[i] - D.cx[j], L);
-  xij[1] = minimg_e(D.cy[i] - D.cy[j], L);
-  xij[2] = minimg_e(D.cz[i] - D.cz[j], L);
-  double Fi[3][3] = {}, Fj[3][3] = {};
-  for (int is = 0; is < 3; ++is) {
-    const double* si = D.site + ((long)i * NS3 + is) * 3;
-    for (int js = 0; js < 3; ++js) {
-      const double* sj = D.site + ((long)j * NS3 + js) * 3;
-      double v[3];
-      for (int c = 0; c < 3; ++c) v[c] = xij[c] + (si[c] - sj[c]);
-      const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-      const double g = -BETA3 * ETA3 * exp(-BETA3 * r) / r;  // dS/dv = g v
-      for (int c = 0; c < 3; ++c) {
-        Fi[is][c] -= WT * g * v[c];
-        Fj[js][c] += WT * g * v[c];
-      }
-    }
+  const double* si = D.site + ((long)i * NS3 + is) * 3;
+  const double* sj = D.site + ((long)j * NS3 + js) * 3;
+  double v[3];
+  v[0] = minimg_e(D.cx[i] - D.cx[j], L) + (si[0] - sj[0]);
+  v[1] = minimg_e(D.cy[i] - D.cy[j], L) + (si[1] - sj[1]);
+  v[2] = minimg_e(D.cz[i] - D.cz[j], L) + (si[2] - sj[2]);
+  const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const double g1 = -WT * (-BETA3 * ETA3 * exp(-BETA3 * r) / r);  // -WT dS/dv = F on site is
+  for (int c = 0; c < 3; ++c) {
+    unsafeAtomicAdd(D.acc + (long)i * ACC_W + is * 3 + c, g1 * v[c]);
+    unsafeAtomicAdd(D.acc + (long)j * ACC_W + js * 3 + c, -g1 * v[c]);
   }
-  for (int s2 = 0; s2 < 3; ++s2)
-    for (int c = 0; c < 3; ++c) {
-      unsafeAtomicAdd(D.acc + (long)i * ACC_W + s2 * 3 + c, Fi[s2][c]);
-      unsafeAtomicAdd(D.acc + (long)j * ACC_W + s2 * 3 + c, Fj[s2][c]);
-    }
 }
 
 // Accumulators (hartree/bohr) -> the caller's arrays in 10 J/mol/A: site
@@ -550,43 +578,73 @@ __global__ void k_finish(Dev D, int ld, int nsite, int do2, int do3, double* fxs
   }
 }
 
-// Per-box totals {qfs2, qfs3 (kcal/mol), vir increment (10 J/mol)}; a fixed
-// reduction tree, so energies and virials are bitwise reproducible.
-__global__ __launch_bounds__(1024) void k_energy(Dev D, int do2, int do3, double* out) {
-  __shared__ double red[4][16];
-  const int b = blockIdx.x;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  if (do2)
-    for (long t = D.box_t2[b] + threadIdx.x; t < D.box_t2[b + 1]; t += 1024) {
-      v[0] += D.ew2[2 * t];
-      v[1] += D.ew2[2 * t + 1];
-    }
-  if (do3)
-    for (long t = D.box_t3[b] + threadIdx.x; t < D.box_t3[b + 1]; t += 1024) {
-      v[2] += D.ew3[2 * t];
-      v[3] += D.ew3[2 * t + 1];
-    }
-  const int w = threadIdx.x >> 6;
-  for (int q = 0; q < 4; ++q) {
-    const double s = wave_sum(v[q]);
-    if ((threadIdx.x & 63) == 0) red[q][w] = s;
+// Energy/virial reduction, level 1: one 256-thread block per box-aligned chunk
+// of RED_CHUNK entries (per-triplet FS3 or per-item FS2 values), fixed tree.
+__global__ __launch_bounds__(256) void k_energy_chunks(Dev D, int which, long nchunks) {
+  __shared__ double red[2][4];
+  const long q = blockIdx.x;
+  if (q >= nchunks) return;
+  const int* bt = which == 3 ? D.box_t3 : D.box_t2;
+  const int* bc = which == 3 ? D.box_c3 : D.box_c2;
+  const double* ew = which == 3 ? D.ew3 : D.ew2;
+  double* out = which == 3 ? D.red3 : D.red2;
+  int lo = 0, hi = D.nbox;  // box owning chunk q: bc[b] <= q < bc[b+1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bc[mid] <= q) lo = mid; else hi = mid;
+  }
+  const int b = lo;
+  const long t0 = bt[b] + (q - bc[b]) * (long)RED_CHUNK;
+  const long t1 = t0 + RED_CHUNK < bt[b + 1] ? t0 + RED_CHUNK : bt[b + 1];
+  double e = 0.0, w = 0.0;
+  for (long t = t0 + threadIdx.x; t < t1; t += 256) {
+    e += ew[2 * t];
+    w += ew[2 * t + 1];
+  }
+  e = wave_sum(e);
+  w = wave_sum(w);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = e;
+    red[1][threadIdx.x >> 6] = w;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double tot[4];
-    for (int q = 0; q < 4; ++q) {
-      double s = 0.0;
-      for (int u = 0; u < 16; ++u) s += red[q][u];
-      tot[q] = s;
-    }
-    out[3 * b + 0] = tot[0] * H2KCAL;
-    out[3 * b + 1] = H2KCAL * tot[2];
-    out[3 * b + 2] = tot[1] * H2KCAL * 418.4 + tot[3] * H2KCAL * 418.4;
+    out[2 * q] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    out[2 * q + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
   }
 }
 
-// explicit instantiations
-template __global__ void k_fs3_eval<double>(Dev, Consts, const double4*, const double*);
-template __global__ void k_fs3_eval<float>(Dev, Consts, const double4*, const float*);
+// Level 2: per box, the chunk partial sums in chunk order ->
+// {qfs2, qfs3 (kcal/mol), vir increment (10 J/mol)}.
+__global__ void k_energy_boxes(Dev D, int do2, int do3, double* out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= D.nbox) return;
+  double e2 = 0.0, w2 = 0.0, e3 = 0.0, w3 = 0.0;
+  if (do2)
+    for (int q = D.box_c2[b]; q < D.box_c2[b + 1]; ++q) {
+      e2 += D.red2[2 * q];
+      w2 += D.red2[2 * q + 1];
+    }
+  if (do3)
+    for (int q = D.box_c3[b]; q < D.box_c3[b + 1]; ++q) {
+      e3 += D.red3[2 * q];
+      w3 += D.red3[2 * q + 1];
+    }
+  out[3 * b + 0] = e2 * H2KCAL;
+  out[3 * b + 1] = H2KCAL * e3;
+  out[3 * b + 2] = w2 * H2KCAL * 418.4 + w3 * H2KCAL * 418.4;
+}
+
+// The FP64 variant is held to two 256-thread blocks per CU by its 63 KB of LDS;
+// the mixed variant (36 KB of LDS) is compiled for three.
+__global__ __launch_bounds__(256, 2) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
+                                                         const double* __restrict__ coef) {
+  fs3_eval_body<double>(D, K, pairpar, coef);
+}
+
+__global__ __launch_bounds__(256, 3) void k_fs3_eval_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
+                                                         const float* __restrict__ coef) {
+  fs3_eval_body<float>(D, K, pairpar, coef);
+}
 
 }  // namespace ccpol3b

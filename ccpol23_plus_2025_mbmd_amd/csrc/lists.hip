@@ -241,30 +241,56 @@ __global__ __launch_bounds__(256) void k_fs2_items(Dev D, Consts K, int mode) {
 }
 
 // ------------------------------------------------------------------ scans
-// Exclusive scan of n ints by one 1024-thread block; off[n] and *total get the sum.
+// Exclusive scan of n ints by one 1024-thread block, tile by tile through LDS
+// (coalesced loads, 8 consecutive elements per thread); off[n] and *total get
+// the sum.
+constexpr int SCAN_PER = 8;
+constexpr int SCAN_TILE = 1024 * SCAN_PER;
+
 __device__ void block_exclusive_scan(const int* cnt, int* off, long n, int* total) {
-  __shared__ long part[1024];
-  const int t = threadIdx.x;
-  const long per = (n + 1023) / 1024;
-  const long b0 = t * per, b1 = b0 + per < n ? b0 + per : n;
-  long s = 0;
-  for (long q = b0; q < b1; ++q) s += cnt[q];
-  part[t] = s;
+  __shared__ int tile[SCAN_TILE];
+  __shared__ long wsum[16];
+  __shared__ long carry_s;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) carry_s = 0;
   __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const long v = t >= d ? part[t - d] : 0;
+  for (long base = 0; base < n; base += SCAN_TILE) {
+    for (int r = 0; r < SCAN_PER; ++r) {
+      const long q = base + r * 1024 + t;
+      tile[r * 1024 + t] = q < n ? cnt[q] : 0;
+    }
     __syncthreads();
-    part[t] += v;
+    long loc[SCAN_PER];
+    long s = 0;
+    for (int r = 0; r < SCAN_PER; ++r) {
+      loc[r] = s;
+      s += tile[t * SCAN_PER + r];
+    }
+    // inclusive scan of the per-thread sums: wavefront shuffles, then wavefront totals
+    long incl = s;
+    for (int d = 1; d < 64; d <<= 1) {
+      const long u = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += u;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    long wbase = 0;
+    for (int u = 0; u < w; ++u) wbase += wsum[u];
+    const long carry = carry_s;
+    const long excl = carry + wbase + incl - s;
+    __syncthreads();
+    for (int r = 0; r < SCAN_PER; ++r) tile[t * SCAN_PER + r] = (int)(excl + loc[r]);
+    if (t == 1023) carry_s = excl + s;
+    __syncthreads();
+    for (int r = 0; r < SCAN_PER; ++r) {
+      const long q = base + r * 1024 + t;
+      if (q < n) off[q] = tile[r * 1024 + t];
+    }
     __syncthreads();
   }
-  long run = part[t] - s;
-  for (long q = b0; q < b1; ++q) {
-    off[q] = (int)run;
-    run += cnt[q];
-  }
-  if (t == 1023) {
-    off[n] = (int)part[1023];
-    *total = (int)part[1023];
+  if (t == 0) {
+    off[n] = (int)carry_s;
+    *total = (int)carry_s;
   }
   __syncthreads();
 }
@@ -284,13 +310,28 @@ __global__ __launch_bounds__(1024) void k_scan(Dev D, int phase, int do3, int do
   }
 }
 
-// Triplet range of each box (triplets are sorted by i, boxes are contiguous in i).
+// Triplet / item range of each box (lists are sorted by their first molecule
+// and boxes are contiguous in molecule index), and the box-aligned chunking of
+// those ranges used by the energy reduction.  One thread.
 __global__ void k_box_ranges(Dev D, int do3, int do2) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b > D.nbox) return;
-  const int m = D.box_start[b];
-  if (do3) D.box_t3[b] = m < D.nm ? D.off3s[D.sl3_off[m]] : D.totals[T_N3];
-  if (do2) D.box_t2[b] = m < D.nm ? D.off2s[D.sl2_off[m]] : D.totals[T_NI2];
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int c3 = 0, c2 = 0;
+  for (int b = 0; b <= D.nbox; ++b) {
+    const int m = D.box_start[b];
+    const int t3 = do3 ? (m < D.nm ? D.off3s[D.sl3_off[m]] : D.totals[T_N3]) : 0;
+    const int t2 = do2 ? (m < D.nm ? D.off2s[D.sl2_off[m]] : D.totals[T_NI2]) : 0;
+    D.box_t3[b] = t3;
+    D.box_t2[b] = t2;
+    D.box_c3[b] = c3;
+    D.box_c2[b] = c2;
+    if (b < D.nbox) {
+      const int m1 = D.box_start[b + 1];
+      const int e3 = do3 ? (m1 < D.nm ? D.off3s[D.sl3_off[m1]] : D.totals[T_N3]) : 0;
+      const int e2 = do2 ? (m1 < D.nm ? D.off2s[D.sl2_off[m1]] : D.totals[T_NI2]) : 0;
+      c3 += (e3 - t3 + RED_CHUNK - 1) / RED_CHUNK;
+      c2 += (e2 - t2 + RED_CHUNK - 1) / RED_CHUNK;
+    }
+  }
 }
 
 }  // namespace ccpol3b
