@@ -299,7 +299,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   }
   HIPCHK(c, hipEventRecord(c->ev[2], st));
   if (do3 && n3 > 0) {
-    const int grid = (int)std::min<long>(n3, (long)c->cu_count * 8);
+    const int grid = (int)n3;  // one block per triplet
     if (c->precision == CCPOL3B_MIXED)
       hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
                          (const float*)c->d_coef32);

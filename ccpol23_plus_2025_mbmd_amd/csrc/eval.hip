@@ -69,8 +69,8 @@ __device__ __forceinline__ double pow_delta(double x, const Consts& K) {
 __constant__ int c_tile0[9] = {0, 1, 3, 5, 7, 9, 11, 13, 15};
 __constant__ int c_tile1[9] = {-1, 2, 4, 6, 8, 10, 12, 14, 16};
 __constant__ int c_tilet[9] = {0, 1, 2, 3, 3, 4, 4, 5, 5};
-__constant__ int c_z0[6] = {0, 1, 3, 5, 9, 13};
-__constant__ int c_z1[6] = {1, 3, 5, 9, 13, 17};
+// first site of each type block, and the end
+__constant__ int c_z0[7] = {0, 1, 3, 5, 9, 13, 17};
 
 // ====================================================================== FS3
 // One 256-thread block per triplet (grid-stride).  pairpar: [2][289] (beta,
@@ -91,18 +91,22 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
   using T2 = typename Pair2<T>::type;
   __shared__ T sC[2 * NCOEF];
   __shared__ double sP[3][NS3][3];
-  __shared__ T2 sV[NPAIR3];
+  // pair vectors (e, e r): blocks ij, ik, jk (first molecule's site major) and
+  // their transposes ji, ki, kj, so every pass reads its rows with unit stride
+  __shared__ T2 sV[2 * NPAIR3];
   __shared__ T sDer[NPAIR3];
   __shared__ double sR[NPAIR3];
   __shared__ T sG[NPAIR3];
   __shared__ double sRed[4];
   __shared__ TriGeo sT;
 
-  const int tid = threadIdx.x;
-  for (int q = tid; q < 2 * NCOEF; q += 256) sC[q] = coef[q];
+  for (int q = threadIdx.x; q < 2 * NCOEF; q += 256) sC[q] = coef[q];
 
   const long n3 = D.totals[T_N3];
-  for (long t = blockIdx.x; t < n3; t += gridDim.x) {
+  const long t = blockIdx.x;  // one block per triplet
+  if (t >= n3) return;
+  {
+    const int tid = threadIdx.x;
     if (tid == 0) {
       const int4 tr = D.tri[t];
       const int mi = tr.x, mj = tr.y, mk = tr.z;
@@ -180,6 +184,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
         v.x = e;
         v.y = e * rt;
         sV[p] = v;
+        sV[(blk + 3) * NPAIR + y * NS3 + x] = v;
         sDer[p] = -T(par.x) + T(par.y) * tg * damp;
         if (rr == 0) sR[p] = r;
       }
@@ -191,30 +196,31 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
         const int x0 = c_tile0[xb], x1r = c_tile1[xb], tx = c_tilet[xb];
         const int y0 = c_tile0[yb], y1r = c_tile1[yb], ty = c_tilet[yb];
         const int x1 = x1r < 0 ? x0 : x1r, y1 = y1r < 0 ? y0 : y1r;
-        // pair-array addressing of v_xz and v_yz for this pass
-        int xzb, xzs, xzz, yzb, yzs, yzz, outb;
-        if (P == 0) {        // X=i, Y=j, Z=k
-          xzb = NPAIR; xzs = NS3; xzz = 1; yzb = 2 * NPAIR; yzs = NS3; yzz = 1; outb = 0;
-        } else if (P == 1) { // X=i, Y=k, Z=j
-          xzb = 0; xzs = NS3; xzz = 1; yzb = 2 * NPAIR; yzs = 1; yzz = NS3; outb = NPAIR;
-        } else {             // X=j, Y=k, Z=i
-          xzb = 0; xzs = 1; xzz = NS3; yzb = NPAIR; yzs = 1; yzz = NS3; outb = 2 * NPAIR;
-        }
+        // rows of v_xz and v_yz: P=0 (X=i,Y=j,Z=k): ik, jk; P=1 (X=i,Y=k,Z=j): ij, kj;
+        // P=2 (X=j,Y=k,Z=i): ji, ki
+        const int xzblk = P == 0 ? 1 : P == 1 ? 0 : 3;
+        const int yzblk = P == 0 ? 2 : P == 1 ? 5 : 4;
+        const int ia0 = xzblk * NPAIR + x0 * NS3, ia1 = xzblk * NPAIR + x1 * NS3;
+        const int ib0 = yzblk * NPAIR + y0 * NS3, ib1 = yzblk * NPAIR + y1 * NS3;
+        const int outb = P * NPAIR;
         T G[2][2][2];
 #pragma unroll
-        for (int a = 0; a < 8; ++a) (&G[0][0][0])[a] = T(0);
-        const T* Cx = sC + X * NCOEF + (tx * NTYPE + ty) * NTYPE * 8;
+        for (int xs = 0; xs < 2; ++xs)
+#pragma unroll
+          for (int ys = 0; ys < 2; ++ys) G[xs][ys][0] = G[xs][ys][1] = T(0);
+        const int ic = X * NCOEF + (tx * NTYPE + ty) * NTYPE * 8;
 #pragma unroll 1
         for (int tz = 0; tz < NTYPE; ++tz) {
           T M[2][2][2][2];
 #pragma unroll
-          for (int a = 0; a < 16; ++a) (&M[0][0][0][0])[a] = T(0);
+          for (int xs = 0; xs < 2; ++xs)
+#pragma unroll
+            for (int ys = 0; ys < 2; ++ys)
+#pragma unroll
+              for (int p = 0; p < 2; ++p) M[xs][ys][p][0] = M[xs][ys][p][1] = T(0);
 #pragma unroll 1
-          for (int z = c_z0[tz]; z < c_z1[tz]; ++z) {
-            const T2 a0 = sV[xzb + x0 * xzs + z * xzz];
-            const T2 a1 = sV[xzb + x1 * xzs + z * xzz];
-            const T2 b0 = sV[yzb + y0 * yzs + z * yzz];
-            const T2 b1 = sV[yzb + y1 * yzs + z * yzz];
+          for (int z = c_z0[tz]; z < c_z0[tz + 1]; ++z) {
+            const T2 a0 = sV[ia0 + z], a1 = sV[ia1 + z], b0 = sV[ib0 + z], b1 = sV[ib1 + z];
             const T av[2][2] = {{a0.x, a0.y}, {a1.x, a1.y}};
             const T bv[2][2] = {{b0.x, b0.y}, {b1.x, b1.y}};
 #pragma unroll
@@ -226,10 +232,9 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
 #pragma unroll
                   for (int q = 0; q < 2; ++q) M[xs][ys][p][q] += bv[ys][p] * av[xs][q];
           }
-          const T* c8 = Cx + tz * 8;
           T cc[8];
 #pragma unroll
-          for (int a = 0; a < 8; ++a) cc[a] = c8[a];
+          for (int a = 0; a < 8; ++a) cc[a] = sC[ic + tz * 8 + a];
 #pragma unroll
           for (int xs = 0; xs < 2; ++xs)
 #pragma unroll
