@@ -24,6 +24,7 @@ __global__ void k_fs2_pairs(Dev);
 __global__ void k_fs2_items(Dev, Consts, int);
 __global__ void k_scan(Dev, int, int, int);
 __global__ void k_box_ranges(Dev, int, int);
+__global__ void k_fs3_geo(Dev, Consts);
 __global__ void k_fs3_eval_f64(Dev, Consts, const double4*, const double*);
 __global__ void k_fs3_eval_f32(Dev, Consts, const double4*, const float*);
 __global__ void k_fs2_items_eval(Dev, Consts);
@@ -158,7 +159,7 @@ void free_dev(ccpol3b_ctx* c) {
   dev_free(D.nb2_cnt); dev_free(D.nb2); dev_free(D.up2);
   dev_free(D.sl3_cnt); dev_free(D.sl3_off); dev_free(D.sl2_cnt); dev_free(D.sl2_off);
   dev_free(D.cnt3s); dev_free(D.off3s); dev_free(D.cnt2s); dev_free(D.off2s);
-  dev_free(D.tri); dev_free(D.pcnt2); dev_free(D.poff2); dev_free(D.pairs2); dev_free(D.items2);
+  dev_free(D.tri); dev_free(D.geo3); dev_free(D.pcnt2); dev_free(D.poff2); dev_free(D.pairs2); dev_free(D.items2);
   dev_free(D.wt2); dev_free(D.acc); dev_free(D.ew3); dev_free(D.ew2);
   dev_free(D.box_t3); dev_free(D.box_t2); dev_free(D.totals);
   dev_free(D.box_c3); dev_free(D.box_c2); dev_free(D.red3); dev_free(D.red2); dev_free(D.s2pair);
@@ -206,7 +207,8 @@ int ensure_lists(ccpol3b_ctx* c, long n3, long ni2) {
   int rc;
   if (n3 > D.cap3) {
     const long n = std::max<long>(n3 + n3 / 8, 1024);
-    if ((rc = dev_alloc(c, D.tri, n)) || (rc = dev_alloc(c, D.ew3, (size_t)n * 2))) return rc;
+    if ((rc = dev_alloc(c, D.tri, n)) || (rc = dev_alloc(c, D.geo3, n)) || (rc = dev_alloc(c, D.ew3, (size_t)n * 2)))
+      return rc;
     D.cap3 = n;
   }
   if (ni2 > D.cap_items2) {
@@ -299,6 +301,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   }
   HIPCHK(c, hipEventRecord(c->ev[2], st));
   if (do3 && n3 > 0) {
+    hipLaunchKernelGGL(k_fs3_geo, dim3((n3 + 255) / 256), dim3(256), 0, st, D, c->K);
     const int grid = (int)n3;  // one block per triplet
     if (c->precision == CCPOL3B_MIXED)
       hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,

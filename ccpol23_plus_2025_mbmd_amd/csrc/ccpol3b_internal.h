@@ -47,6 +47,17 @@ struct Consts {
   double rmin, rmax;  // switch window in bohr, :774-775
 };
 
+// Geometry of one FS3 triplet, computed once per triplet by k_fs3_geo (one
+// thread per triplet) and read by every thread of its evaluation block.
+struct TriGeo {
+  double c[3][3];      // COM vectors x_ij, x_ik, x_jk (bohr), as the reference forms them (:862-884)
+  double R[3];         // |x_ij|, |x_ik|, |x_jk|
+  double Ravg, fd, dfd;
+  double w[2], wp[2];  // region weights (SR, LR) and their R_avg derivatives
+  int nreg, reg0, m[3];
+  int pad;
+};
+
 // Device arrays of one evaluation (sized by the context; see api.hip).
 struct Dev {
   int nm, nbox, cap_nb;
@@ -67,6 +78,7 @@ struct Dev {
   int *cnt2s, *off2s;         // [cap_sl2], [cap_sl2+1] items per FS2 sub-list
   // FS3 triplets (i<j<k, lexicographic)
   int4 *tri;                  // [cap3] (i,j,k,-)
+  TriGeo *geo3;               // [cap3] geometry of each triplet
   // FS2 pairs (i<j within rcut3) and k-major items (k,i,j,pair)
   int *pcnt2, *poff2;         // [nm], [nm+1]
   int2 *pairs2;               // [cap_pairs2]

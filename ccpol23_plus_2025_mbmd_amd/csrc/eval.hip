@@ -69,22 +69,95 @@ __device__ __forceinline__ double pow_delta(double x, const Consts& K) {
 __constant__ int c_tile0[9] = {0, 1, 3, 5, 7, 9, 11, 13, 15};
 __constant__ int c_tile1[9] = {-1, 2, 4, 6, 8, 10, 12, 14, 16};
 __constant__ int c_tilet[9] = {0, 1, 2, 3, 3, 4, 4, 5, 5};
-// first site of each type block, and the end
-__constant__ int c_z0[7] = {0, 1, 3, 5, 9, 13, 17};
 
 // ====================================================================== FS3
-// One 256-thread block per triplet (grid-stride).  pairpar: [2][289] (beta,
-// gamma, r0, -) for SR and LR; coef: [2][NCOEF] dense [tx][ty][tz][p][q][s].
-// Block-uniform data of the current triplet, kept in LDS rather than in every
-// thread's registers.
-struct TriGeo {
-  double c[3][3];    // COM vectors x_ij, x_ik, x_jk (bohr), as the reference forms them
-  double R[3];       // |x_ij|, |x_ik|, |x_jk|
-  double Ravg, fd, dfd;
-  double w[2], wp[2];  // region weights and their R_avg derivatives
-  int nreg, reg0, m[3];
-};
+// pairpar: [2][289] (beta, gamma, r0, -) for SR and LR; coef: [2][NCOEF] dense
+// [tx][ty][tz][p][q][s].
+// Geometry of triplet t (COM vectors, R_avg, damping, region weights).
+__device__ void fs3_geometry(const Dev& D, const Consts& K, long t, TriGeo& g) {
+  const int4 tr = D.tri[t];
+  const int mi = tr.x, mj = tr.y, mk = tr.z;
+  const double L = D.boxL[D.mol_box[mi]];
+  const double pi_[3] = {D.cx[mi], D.cy[mi], D.cz[mi]};
+  const double pj_[3] = {D.cx[mj], D.cy[mj], D.cz[mj]};
+  const double pk_[3] = {D.cx[mk], D.cy[mk], D.cz[mk]};
+  for (int c = 0; c < 3; ++c) {
+    const double dij = pi_[c] - pj_[c], dik = pi_[c] - pk_[c], djk = pj_[c] - pk_[c];
+    if (L > 0.0) {
+      const double nij = round(dij / L), nik = round(dik / L);
+      g.c[0][c] = dij - L * nij;
+      g.c[1][c] = dik - L * nik;
+      g.c[2][c] = djk + L * nij - L * nik;
+    } else {
+      g.c[0][c] = dij;
+      g.c[1][c] = dik;
+      g.c[2][c] = djk;
+    }
+  }
+  for (int b = 0; b < 3; ++b) g.R[b] = sqrt(g.c[b][0] * g.c[b][0] + g.c[b][1] * g.c[b][1] + g.c[b][2] * g.c[b][2]);
+  const double Ravg = (g.R[0] + g.R[2] + g.R[1]) / 3.0;  // (:894-898)
+  const double pom = exp(K.alp3 * (Ravg - K.r0dmp3));
+  const double pp = 1.0 / (1.0 + pom);
+  const double fd = pow_delta(pp, K);
+  g.Ravg = Ravg;
+  g.fd = fd;
+  g.dfd = -K.alp3 * K.delta33 * pom * pp * fd;  // d f_d / d R_avg (:900-904)
+  g.w[0] = 1.0; g.wp[0] = 0.0; g.w[1] = 0.0; g.wp[1] = 0.0;
+  if (Ravg <= K.rmin) {
+    g.nreg = 1; g.reg0 = 0;
+  } else if (Ravg >= K.rmax) {
+    g.nreg = 1; g.reg0 = 1;
+  } else {  // cubic switch (:1420-1423)
+    const double span = K.rmax - K.rmin;
+    const double c = (Ravg - K.rmin) / span;
+    const double sm = 1.0 + 2.0 * c * c * c - 3.0 * c * c;
+    const double dsm = (6.0 * c * c - 6.0 * c) / span;
+    g.nreg = 2; g.reg0 = 0;
+    g.w[0] = sm; g.wp[0] = dsm;
+    g.w[1] = 1.0 - sm; g.wp[1] = -dsm;
+  }
+  g.m[0] = mi; g.m[1] = mj; g.m[2] = mk;
+  g.pad = 0;
+}
 
+__global__ __launch_bounds__(256) void k_fs3_geo(Dev D, Consts K) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n3 = D.totals[T_N3];
+  if (t - (threadIdx.x & 63) >= n3) return;  // whole waves past the end
+  int hist = -1;
+  if (t < n3) {
+    TriGeo g;
+    fs3_geometry(D, K, t, g);
+    D.geo3[t] = g;
+    hist = g.nreg == 2 ? 1 : 2 * g.reg0;
+  }
+  // region histogram: one atomic per wave and region (same-address atomics
+  // from every triplet would serialise)
+  const int lane = threadIdx.x & 63;
+  for (int h = 0; h < 3; ++h) {
+    const int n = __popcll(__ballot(hist == h));
+    if (lane == 0 && n > 0) atomicAdd(D.totals + T_HIST + h, n);
+  }
+}
+
+// Mixed-precision helpers: r from the FP64 r^2 with a FP32 square root, and a
+// FP32 reciprocal; the FP64 variant keeps sqrt and the IEEE division.
+template <typename T> __device__ __forceinline__ double t_dist(double r2);
+template <> __device__ __forceinline__ double t_dist<double>(double r2) { return sqrt(r2); }
+template <> __device__ __forceinline__ double t_dist<float>(double r2) { return double(__fsqrt_rn(float(r2))); }
+template <typename T> __device__ __forceinline__ T t_rcp(T x);
+template <> __device__ __forceinline__ double t_rcp<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ float t_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// sites of each type block (O, H, D1, N1, N2, N3)
+__constant__ int c_nz[NTYPE] = {1, 2, 2, 4, 4, 4};
+
+// One 256-thread block per triplet:
+//  pair stage:   e, e r and d ln e/dr of the 867 site pairs of a region -> LDS;
+//  triple stage: 3 passes x 9 x 9 tiles of 2x2 site pairs (243 threads), each
+//                accumulating G[x,y,s] = sum_tz sum_pq C M_tz over the type
+//                blocks of the third molecule's sites;
+//  chain rule:   dE/dr of each site pair -> site forces, site virial, COM forces.
 template <typename T>
 __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __restrict__ pairpar,
                                               const T* __restrict__ coef) {
@@ -98,143 +171,83 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
   __shared__ double sR[NPAIR3];
   __shared__ T sG[NPAIR3];
   __shared__ double sRed[4];
-  __shared__ TriGeo sT;
 
-  for (int q = threadIdx.x; q < 2 * NCOEF; q += 256) sC[q] = coef[q];
-
-  const long n3 = D.totals[T_N3];
   const long t = blockIdx.x;  // one block per triplet
-  if (t >= n3) return;
-  {
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-      const int4 tr = D.tri[t];
-      const int mi = tr.x, mj = tr.y, mk = tr.z;
-      const double L = D.boxL[D.mol_box[mi]];
-      // COM vectors as the reference forms them (:862-884)
-      const double pi_[3] = {D.cx[mi], D.cy[mi], D.cz[mi]};
-      const double pj_[3] = {D.cx[mj], D.cy[mj], D.cz[mj]};
-      const double pk_[3] = {D.cx[mk], D.cy[mk], D.cz[mk]};
-      for (int c = 0; c < 3; ++c) {
-        const double dij = pi_[c] - pj_[c], dik = pi_[c] - pk_[c], djk = pj_[c] - pk_[c];
-        if (L > 0.0) {
-          const double nij = round(dij / L), nik = round(dik / L);
-          sT.c[0][c] = dij - L * nij;
-          sT.c[1][c] = dik - L * nik;
-          sT.c[2][c] = djk + L * nij - L * nik;
-        } else {
-          sT.c[0][c] = dij;
-          sT.c[1][c] = dik;
-          sT.c[2][c] = djk;
-        }
-      }
-      for (int b = 0; b < 3; ++b)
-        sT.R[b] = sqrt(sT.c[b][0] * sT.c[b][0] + sT.c[b][1] * sT.c[b][1] + sT.c[b][2] * sT.c[b][2]);
-      const double Ravg = (sT.R[0] + sT.R[2] + sT.R[1]) / 3.0;
-      const double pom = exp(K.alp3 * (Ravg - K.r0dmp3));
-      const double pp = 1.0 / (1.0 + pom);
-      const double fd = pow_delta(pp, K);
-      sT.Ravg = Ravg;
-      sT.fd = fd;
-      sT.dfd = -K.alp3 * K.delta33 * pom * pp * fd;  // d f_d / d R_avg
-      sT.w[0] = 1.0; sT.wp[0] = 0.0; sT.w[1] = 0.0; sT.wp[1] = 0.0;
-      if (Ravg <= K.rmin) {
-        sT.nreg = 1; sT.reg0 = 0;
-      } else if (Ravg >= K.rmax) {
-        sT.nreg = 1; sT.reg0 = 1;
-      } else {  // cubic switch (:1420-1423)
-        const double span = K.rmax - K.rmin;
-        const double c = (Ravg - K.rmin) / span;
-        const double sm = 1.0 + 2.0 * c * c * c - 3.0 * c * c;
-        const double dsm = (6.0 * c * c - 6.0 * c) / span;
-        sT.nreg = 2; sT.reg0 = 0;
-        sT.w[0] = sm; sT.wp[0] = dsm;
-        sT.w[1] = 1.0 - sm; sT.wp[1] = -dsm;
-      }
-      sT.m[0] = mi; sT.m[1] = mj; sT.m[2] = mk;
-      atomicAdd(D.totals + T_HIST + (sT.nreg == 2 ? 1 : 2 * sT.reg0), 1);
-    }
-    for (int p = tid; p < NPAIR3; p += 256) sG[p] = T(0);
-    __syncthreads();
-    if (tid < 3 * NS3 * 3) {
-      const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
-      (&sP[0][0][0])[tid] = D.site[(long)sT.m[role] * NS3 * 3 + rem];
-    }
-    const int nreg = sT.nreg, reg0 = sT.reg0;
-    __syncthreads();
+  if (t >= D.totals[T_N3]) return;
+  const int tid = threadIdx.x;
+  __shared__ TriGeo g;  // block-uniform geometry, read with lane-varying indices
+  static_assert(sizeof(TriGeo) % 8 == 0 && sizeof(TriGeo) / 8 <= 256, "TriGeo copy");
+  if (tid < (int)(sizeof(TriGeo) / 8))
+    reinterpret_cast<double*>(&g)[tid] = reinterpret_cast<const double*>(D.geo3 + t)[tid];
+  const int nreg = D.geo3[t].nreg, reg0 = D.geo3[t].reg0;
+  // coefficients of the regions this triplet needs
+  for (int q = tid; q < nreg * NCOEF; q += 256) sC[q] = coef[reg0 * NCOEF + q];
+  for (int p = tid; p < NPAIR3; p += 256) sG[p] = T(0);
+  if (tid < 3 * NS3 * 3) {
+    const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
+    const int4 tr = D.tri[t];
+    const int m = role == 0 ? tr.x : role == 1 ? tr.y : tr.z;
+    (&sP[0][0][0])[tid] = D.site[(long)m * NS3 * 3 + rem];
+  }
+  __syncthreads();
 
-    double eacc0 = 0.0, eacc1 = 0.0;
-    for (int rr = 0; rr < nreg; ++rr) {
-      const int X = reg0 + rr;
-      // ---------------- pair stage: e, e*r and d ln e / dr for the 867 site pairs
-      for (int p = tid; p < NPAIR3; p += 256) {
-        const int blk = p / NPAIR, rem = p - blk * NPAIR;
-        const int x = rem / NS3, y = rem - x * NS3;
-        const int ra = blk == 2 ? 1 : 0, rb = blk == 0 ? 1 : 2;
-        const double vx = sT.c[blk][0] + (sP[ra][x][0] - sP[rb][y][0]);
-        const double vy = sT.c[blk][1] + (sP[ra][x][1] - sP[rb][y][1]);
-        const double vz = sT.c[blk][2] + (sP[ra][x][2] - sP[rb][y][2]);
-        const double r = sqrt(vx * vx + vy * vy + vz * vz);
-        const double4 par = pairpar[X * NPAIR + rem];
-        const T rt = T(r);
-        const T tg = t_exp<T>(-T(par.y) * (rt - T(par.z)));
-        const T damp = T(1) / (T(1) + tg);
-        const T e = t_exp<T>(-T(par.x) * rt) * damp;
-        T2 v;
-        v.x = e;
-        v.y = e * rt;
-        sV[p] = v;
-        sV[(blk + 3) * NPAIR + y * NS3 + x] = v;
-        sDer[p] = -T(par.x) + T(par.y) * tg * damp;
-        if (rr == 0) sR[p] = r;
-      }
-      __syncthreads();
-      // ---------------- site-triple stage: 3 passes x 9 x 9 tiles of 2x2 site pairs
-      if (tid < 243) {
-        const int P = tid / 81, rem = tid - P * 81;
-        const int xb = rem / 9, yb = rem - xb * 9;
-        const int x0 = c_tile0[xb], x1r = c_tile1[xb], tx = c_tilet[xb];
-        const int y0 = c_tile0[yb], y1r = c_tile1[yb], ty = c_tilet[yb];
-        const int x1 = x1r < 0 ? x0 : x1r, y1 = y1r < 0 ? y0 : y1r;
-        // rows of v_xz and v_yz: P=0 (X=i,Y=j,Z=k): ik, jk; P=1 (X=i,Y=k,Z=j): ij, kj;
-        // P=2 (X=j,Y=k,Z=i): ji, ki
-        const int xzblk = P == 0 ? 1 : P == 1 ? 0 : 3;
-        const int yzblk = P == 0 ? 2 : P == 1 ? 5 : 4;
-        const int ia0 = xzblk * NPAIR + x0 * NS3, ia1 = xzblk * NPAIR + x1 * NS3;
-        const int ib0 = yzblk * NPAIR + y0 * NS3, ib1 = yzblk * NPAIR + y1 * NS3;
-        const int outb = P * NPAIR;
-        T G[2][2][2];
+  double eacc0 = 0.0, eacc1 = 0.0;
+  for (int rr = 0; rr < nreg; ++rr) {
+    const int X = reg0 + rr;
+    // ---------------- pair stage: e, e*r and d ln e / dr for the 867 site pairs
+    for (int p = tid; p < NPAIR3; p += 256) {
+      const int blk = p / NPAIR, rem = p - blk * NPAIR;
+      const int x = rem / NS3, y = rem - x * NS3;
+      const int ra = blk == 2 ? 1 : 0, rb = blk == 0 ? 1 : 2;
+      const double vx = g.c[blk][0] + (sP[ra][x][0] - sP[rb][y][0]);
+      const double vy = g.c[blk][1] + (sP[ra][x][1] - sP[rb][y][1]);
+      const double vz = g.c[blk][2] + (sP[ra][x][2] - sP[rb][y][2]);
+      const double r = t_dist<T>(vx * vx + vy * vy + vz * vz);
+      const double4 par = pairpar[X * NPAIR + rem];
+      const T rt = T(r);
+      const T tg = t_exp<T>(-T(par.y) * (rt - T(par.z)));
+      const T damp = t_rcp<T>(T(1) + tg);
+      const T e = t_exp<T>(-T(par.x) * rt) * damp;
+      T2 v;
+      v.x = e;
+      v.y = e * rt;
+      sV[p] = v;
+      sV[(blk + 3) * NPAIR + y * NS3 + x] = v;
+      sDer[p] = -T(par.x) + T(par.y) * tg * damp;
+      if (rr == 0) sR[p] = r;
+    }
+    __syncthreads();
+    // ---------------- site-triple stage
+    if (tid < 243) {
+      const int P = tid / 81, rem = tid - P * 81;
+      const int xb = rem / 9, yb = rem - xb * 9;
+      const int x0 = c_tile0[xb], x1r = c_tile1[xb], tx = c_tilet[xb];
+      const int y0 = c_tile0[yb], y1r = c_tile1[yb], ty = c_tilet[yb];
+      const int x1 = x1r < 0 ? x0 : x1r, y1 = y1r < 0 ? y0 : y1r;
+      // rows of v_xz and v_yz: P=0 (X=i,Y=j,Z=k): ik, jk; P=1 (X=i,Y=k,Z=j): ij, kj;
+      // P=2 (X=j,Y=k,Z=i): ji, ki.  The second row of a tile is always the next
+      // row in LDS (the O tile reads the first H row and discards it), so two
+      // pointers and immediate offsets address all four rows.
+      const int xzblk = P == 0 ? 1 : P == 1 ? 0 : 3;
+      const int yzblk = P == 0 ? 2 : P == 1 ? 5 : 4;
+      const T2* ap = sV + xzblk * NPAIR + x0 * NS3;
+      const T2* bp = sV + yzblk * NPAIR + y0 * NS3;
+      const T* cp = sC + rr * NCOEF + (tx * NTYPE + ty) * NTYPE * 8;
+      T G[2][2][2];
 #pragma unroll
-        for (int xs = 0; xs < 2; ++xs)
+      for (int xs = 0; xs < 2; ++xs)
 #pragma unroll
-          for (int ys = 0; ys < 2; ++ys) G[xs][ys][0] = G[xs][ys][1] = T(0);
-        const int ic = X * NCOEF + (tx * NTYPE + ty) * NTYPE * 8;
+        for (int ys = 0; ys < 2; ++ys) G[xs][ys][0] = G[xs][ys][1] = T(0);
 #pragma unroll 1
-        for (int tz = 0; tz < NTYPE; ++tz) {
-          T M[2][2][2][2];
-#pragma unroll
-          for (int xs = 0; xs < 2; ++xs)
-#pragma unroll
-            for (int ys = 0; ys < 2; ++ys)
-#pragma unroll
-              for (int p = 0; p < 2; ++p) M[xs][ys][p][0] = M[xs][ys][p][1] = T(0);
-#pragma unroll 1
-          for (int z = c_z0[tz]; z < c_z0[tz + 1]; ++z) {
-            const T2 a0 = sV[ia0 + z], a1 = sV[ia1 + z], b0 = sV[ib0 + z], b1 = sV[ib1 + z];
-            const T av[2][2] = {{a0.x, a0.y}, {a1.x, a1.y}};
-            const T bv[2][2] = {{b0.x, b0.y}, {b1.x, b1.y}};
-#pragma unroll
-            for (int xs = 0; xs < 2; ++xs)
-#pragma unroll
-              for (int ys = 0; ys < 2; ++ys)
-#pragma unroll
-                for (int p = 0; p < 2; ++p)
-#pragma unroll
-                  for (int q = 0; q < 2; ++q) M[xs][ys][p][q] += bv[ys][p] * av[xs][q];
-          }
-          T cc[8];
-#pragma unroll
-          for (int a = 0; a < 8; ++a) cc[a] = sC[ic + tz * 8 + a];
+      for (int tz = 0; tz < NTYPE; ++tz) {
+        // M[xs][ys][p][q] = sum_{z in type tz} v_yz[p] v_xz[q]; every type block
+        // holds at least one site, so the first term initialises M
+        T M[2][2][2][2];
+        const int nz = c_nz[tz];
+        {
+          const T2 a0 = ap[0], a1 = ap[NS3], b0 = bp[0], b1 = bp[NS3];
+          const T av[2][2] = {{a0.x, a0.y}, {a1.x, a1.y}};
+          const T bv[2][2] = {{b0.x, b0.y}, {b1.x, b1.y}};
 #pragma unroll
           for (int xs = 0; xs < 2; ++xs)
 #pragma unroll
@@ -242,93 +255,119 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
 #pragma unroll
               for (int p = 0; p < 2; ++p)
 #pragma unroll
-                for (int q = 0; q < 2; ++q)
-#pragma unroll
-                  for (int s = 0; s < 2; ++s)
-                    G[xs][ys][s] += cc[(p * 2 + q) * 2 + s] * M[xs][ys][p][q];
+                for (int q = 0; q < 2; ++q) M[xs][ys][p][q] = bv[ys][p] * av[xs][q];
         }
-        const T fdw = T(sT.fd * sT.w[rr]);
+#pragma unroll 1
+        for (int z = 1; z < nz; ++z) {
+          const T2 a0 = ap[z], a1 = ap[NS3 + z], b0 = bp[z], b1 = bp[NS3 + z];
+          const T av[2][2] = {{a0.x, a0.y}, {a1.x, a1.y}};
+          const T bv[2][2] = {{b0.x, b0.y}, {b1.x, b1.y}};
 #pragma unroll
-        for (int xs = 0; xs < 2; ++xs) {
-          if (xs == 1 && x1r < 0) continue;
-          const int x = xs ? x1 : x0;
+          for (int xs = 0; xs < 2; ++xs)
 #pragma unroll
-          for (int ys = 0; ys < 2; ++ys) {
-            if (ys == 1 && y1r < 0) continue;
-            const int y = ys ? y1 : y0;
-            const int po = outb + x * NS3 + y;
-            const T2 v = sV[po];
-            const T der = sDer[po];
-            const T ev = v.x * G[xs][ys][0] + v.y * G[xs][ys][1];
-            const T dEdr = der * ev + v.x * G[xs][ys][1];
-            sG[po] += fdw * dEdr;
-            if (P == 0) {
-              if (rr == 0) eacc0 += double(ev); else eacc1 += double(ev);
-            }
+            for (int ys = 0; ys < 2; ++ys)
+#pragma unroll
+              for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) M[xs][ys][p][q] += bv[ys][p] * av[xs][q];
+        }
+        ap += nz;
+        bp += nz;
+        T cc[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) cc[a] = cp[tz * 8 + a];
+#pragma unroll
+        for (int xs = 0; xs < 2; ++xs)
+#pragma unroll
+          for (int ys = 0; ys < 2; ++ys)
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+              for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) G[xs][ys][s] += cc[(p * 2 + q) * 2 + s] * M[xs][ys][p][q];
+      }
+      const T fdw = T(g.fd * g.w[rr]);
+      const int outb = P * NPAIR;
+#pragma unroll
+      for (int xs = 0; xs < 2; ++xs) {
+        if (xs == 1 && x1r < 0) continue;
+        const int x = xs ? x1 : x0;
+#pragma unroll
+        for (int ys = 0; ys < 2; ++ys) {
+          if (ys == 1 && y1r < 0) continue;
+          const int y = ys ? y1 : y0;
+          const int po = outb + x * NS3 + y;
+          const T2 v = sV[po];
+          const T der = sDer[po];
+          const T ev = v.x * G[xs][ys][0] + v.y * G[xs][ys][1];
+          const T dEdr = der * ev + v.x * G[xs][ys][1];
+          sG[po] += fdw * dEdr;
+          if (P == 0) {
+            if (rr == 0) eacc0 += double(ev); else eacc1 += double(ev);
           }
         }
       }
-      __syncthreads();
-    }
-    // region energies (FP64 reduction)
-    const double EX0 = block_sum256(eacc0, sRed);
-    const double EX1 = nreg == 2 ? block_sum256(eacc1, sRed) : 0.0;
-    const double Eb = sT.w[0] * EX0 + sT.w[1] * EX1;
-    const double Et = sT.fd * Eb;
-    const double dEdR = sT.dfd * Eb + sT.fd * (sT.wp[0] * EX0 + sT.wp[1] * EX1);  // dE/dR_avg
-    // ---------------- chain rule: h = (dE/dr)/r; site virial sum r dE/dr
-    double wsum = 0.0;
-    for (int p = tid; p < NPAIR3; p += 256) {
-      const double g = double(sG[p]);
-      const double r = sR[p];
-      wsum += g * r;
-      sR[p] = g / r;
-    }
-    const double Wsites = block_sum256(wsum, sRed);  // includes the barrier before reuse of sR
-    if (tid < 3 * NS3 * 3) {
-      const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
-      const int s = rem / 3, c = rem - s * 3;
-      const double aij = sT.c[0][c], aik = sT.c[1][c], ajk = sT.c[2][c];
-      double F = 0.0;
-      if (role == 0) {
-#pragma unroll 4
-        for (int y = 0; y < NS3; ++y)
-          F -= sR[s * NS3 + y] * (aij + (sP[0][s][c] - sP[1][y][c]));
-#pragma unroll 4
-        for (int z = 0; z < NS3; ++z)
-          F -= sR[NPAIR + s * NS3 + z] * (aik + (sP[0][s][c] - sP[2][z][c]));
-      } else if (role == 1) {
-#pragma unroll 4
-        for (int x = 0; x < NS3; ++x)
-          F += sR[x * NS3 + s] * (aij + (sP[0][x][c] - sP[1][s][c]));
-#pragma unroll 4
-        for (int z = 0; z < NS3; ++z)
-          F -= sR[2 * NPAIR + s * NS3 + z] * (ajk + (sP[1][s][c] - sP[2][z][c]));
-      } else {
-#pragma unroll 4
-        for (int x = 0; x < NS3; ++x)
-          F += sR[NPAIR + x * NS3 + s] * (aik + (sP[0][x][c] - sP[2][s][c]));
-#pragma unroll 4
-        for (int y = 0; y < NS3; ++y)
-          F += sR[2 * NPAIR + y * NS3 + s] * (ajk + (sP[1][y][c] - sP[2][s][c]));
-      }
-      unsafeAtomicAdd(D.acc + (long)sT.m[role] * ACC_W + rem, F);
-    } else if (tid < 3 * NS3 * 3 + 9) {
-      // COM forces from the R_avg dependence of f_d and w (:1130-1166, 1613-1665)
-      const int q = tid - 3 * NS3 * 3, role = q / 3, c = q - role * 3;
-      const double uij = sT.c[0][c] / sT.R[0], uik = sT.c[1][c] / sT.R[1], ujk = sT.c[2][c] / sT.R[2];
-      const double g3 = -dEdR / 3.0;
-      double F;
-      if (role == 0) F = g3 * (uij + uik);
-      else if (role == 1) F = g3 * (-uij + ujk);
-      else F = g3 * (-uik - ujk);
-      unsafeAtomicAdd(D.acc + (long)sT.m[role] * ACC_W + ACC_C3 + c, F);
-    }
-    if (tid == 0) {
-      D.ew3[2 * t] = Et;
-      D.ew3[2 * t + 1] = Wsites + sT.Ravg * dEdR;
     }
     __syncthreads();
+  }
+  // region energies (FP64 reduction)
+  const double EX0 = block_sum256(eacc0, sRed);
+  const double EX1 = nreg == 2 ? block_sum256(eacc1, sRed) : 0.0;
+  const double Eb = g.w[0] * EX0 + g.w[1] * EX1;
+  const double Et = g.fd * Eb;
+  const double dEdR = g.dfd * Eb + g.fd * (g.wp[0] * EX0 + g.wp[1] * EX1);  // dE/dR_avg
+  // ---------------- chain rule: h = (dE/dr)/r; site virial sum r dE/dr
+  double wsum = 0.0;
+  for (int p = tid; p < NPAIR3; p += 256) {
+    const double gp = double(sG[p]);
+    const double r = sR[p];
+    wsum += gp * r;
+    sR[p] = gp / r;
+  }
+  const double Wsites = block_sum256(wsum, sRed);  // includes the barrier before reuse of sR
+  if (tid < 3 * NS3 * 3) {
+    const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
+    const int s = rem / 3, c = rem - s * 3;
+    const double aij = g.c[0][c], aik = g.c[1][c], ajk = g.c[2][c];
+    double F = 0.0;
+    if (role == 0) {
+#pragma unroll 4
+      for (int y = 0; y < NS3; ++y)
+        F -= sR[s * NS3 + y] * (aij + (sP[0][s][c] - sP[1][y][c]));
+#pragma unroll 4
+      for (int z = 0; z < NS3; ++z)
+        F -= sR[NPAIR + s * NS3 + z] * (aik + (sP[0][s][c] - sP[2][z][c]));
+    } else if (role == 1) {
+#pragma unroll 4
+      for (int x = 0; x < NS3; ++x)
+        F += sR[x * NS3 + s] * (aij + (sP[0][x][c] - sP[1][s][c]));
+#pragma unroll 4
+      for (int z = 0; z < NS3; ++z)
+        F -= sR[2 * NPAIR + s * NS3 + z] * (ajk + (sP[1][s][c] - sP[2][z][c]));
+    } else {
+#pragma unroll 4
+      for (int x = 0; x < NS3; ++x)
+        F += sR[NPAIR + x * NS3 + s] * (aik + (sP[0][x][c] - sP[2][s][c]));
+#pragma unroll 4
+      for (int y = 0; y < NS3; ++y)
+        F += sR[2 * NPAIR + y * NS3 + s] * (ajk + (sP[1][y][c] - sP[2][s][c]));
+    }
+    unsafeAtomicAdd(D.acc + (long)g.m[role] * ACC_W + rem, F);
+  } else if (tid < 3 * NS3 * 3 + 9) {
+    // COM forces from the R_avg dependence of f_d and w (:1130-1166, 1613-1665)
+    const int q = tid - 3 * NS3 * 3, role = q / 3, c = q - role * 3;
+    const double uij = g.c[0][c] / g.R[0], uik = g.c[1][c] / g.R[1], ujk = g.c[2][c] / g.R[2];
+    const double g3 = -dEdR / 3.0;
+    double F;
+    if (role == 0) F = g3 * (uij + uik);
+    else if (role == 1) F = g3 * (-uij + ujk);
+    else F = g3 * (-uik - ujk);
+    unsafeAtomicAdd(D.acc + (long)g.m[role] * ACC_W + ACC_C3 + c, F);
+  }
+  if (tid == 0) {
+    D.ew3[2 * t] = Et;
+    D.ew3[2 * t + 1] = Wsites + g.Ravg * dEdR;
   }
 }
 
