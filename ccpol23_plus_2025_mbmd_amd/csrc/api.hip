@@ -318,7 +318,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   const long nch3 = n3 / RED_CHUNK + nbox + 1, nch2 = ni2 / RED_CHUNK + nbox + 1;
   if (do3) hipLaunchKernelGGL(k_energy_chunks, dim3(nch3), dim3(256), 0, st, D, 3, nch3);
   if (do2) hipLaunchKernelGGL(k_energy_chunks, dim3(nch2), dim3(256), 0, st, D, 2, nch2);
-  hipLaunchKernelGGL(k_energy_boxes, dim3((nbox + 63) / 64), dim3(64), 0, st, D, do2, do3, energies);
+  hipLaunchKernelGGL(k_energy_boxes, dim3(nbox), dim3(64), 0, st, D, do2, do3, energies);
   LAUNCHCHK(c, "finish");
   HIPCHK(c, hipEventRecord(c->ev[4], st));
   HIPCHK(c, hipMemcpyAsync(c->h_totals + T_HIST, D.totals + T_HIST, 3 * sizeof(int), hipMemcpyDeviceToHost, st));

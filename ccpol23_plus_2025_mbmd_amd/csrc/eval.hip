@@ -156,44 +156,75 @@ __constant__ int c_nz[NTYPE] = {1, 2, 2, 4, 4, 4};
 //  pair stage:   e, e r and d ln e/dr of the 867 site pairs of a region -> LDS;
 //  triple stage: 3 passes x 9 x 9 tiles of 2x2 site pairs (243 threads), each
 //                accumulating G[x,y,s] = sum_tz sum_pq C M_tz over the type
-//                blocks of the third molecule's sites;
-//  chain rule:   dE/dr of each site pair -> site forces, site virial, COM forces.
-template <typename T>
+//                blocks of the third molecule's sites; the tile's owner keeps
+//                dE/dr of its site pairs in registers across regions and ends
+//                with h = (dE/dr)/r and the site virial;
+//  chain rule:   h of each site pair -> site forces; dE/dR_avg -> COM forces.
+// TR: the site-pair blocks are also stored transposed, so both rows of every
+// tile are read with unit stride (more LDS, two address registers per tile);
+// without TR the column-wise passes step with stride 17.
+template <typename T, bool TR>
 __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __restrict__ pairpar,
                                               const T* __restrict__ coef) {
   using T2 = typename Pair2<T>::type;
-  __shared__ T sC[2 * NCOEF];
+  // sR holds r and then h = (dE/dr)/r.  The mixed variant's r is FP32-exact
+  // (FP32 square root), so T storage loses nothing.
+  __shared__ T sC[NCOEF];
   __shared__ double sP[3][NS3][3];
-  // pair vectors (e, e r): blocks ij, ik, jk (first molecule's site major) and
-  // their transposes ji, ki, kj, so every pass reads its rows with unit stride
-  __shared__ T2 sV[2 * NPAIR3];
+  __shared__ T2 sV[(TR ? 2 : 1) * NPAIR3];
   __shared__ T sDer[NPAIR3];
-  __shared__ double sR[NPAIR3];
-  __shared__ T sG[NPAIR3];
+  __shared__ T sR[NPAIR3];
   __shared__ double sRed[4];
+  __shared__ TriGeo g;  // block-uniform geometry, read with lane-varying indices
 
   const long t = blockIdx.x;  // one block per triplet
   if (t >= D.totals[T_N3]) return;
   const int tid = threadIdx.x;
-  __shared__ TriGeo g;  // block-uniform geometry, read with lane-varying indices
   static_assert(sizeof(TriGeo) % 8 == 0 && sizeof(TriGeo) / 8 <= 256, "TriGeo copy");
   if (tid < (int)(sizeof(TriGeo) / 8))
     reinterpret_cast<double*>(&g)[tid] = reinterpret_cast<const double*>(D.geo3 + t)[tid];
   const int nreg = D.geo3[t].nreg, reg0 = D.geo3[t].reg0;
-  // coefficients of the regions this triplet needs
-  for (int q = tid; q < nreg * NCOEF; q += 256) sC[q] = coef[reg0 * NCOEF + q];
-  for (int p = tid; p < NPAIR3; p += 256) sG[p] = T(0);
   if (tid < 3 * NS3 * 3) {
     const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
     const int4 tr = D.tri[t];
     const int m = role == 0 ? tr.x : role == 1 ? tr.y : tr.z;
     (&sP[0][0][0])[tid] = D.site[(long)m * NS3 * 3 + rem];
   }
-  __syncthreads();
 
+  // tile of this thread (threads 0..242): pass P, 2x2 site tile (x0,x1) x (y0,y1)
+  const bool tile = tid < 243;
+  const int P = tid / 81, trem = tid - P * 81;
+  const int xb = trem / 9, yb = trem - xb * 9;
+  const int tsel = tile ? xb : 0, usel = tile ? yb : 0;
+  const int x0 = c_tile0[tsel], x1r = c_tile1[tsel], tx = c_tilet[tsel];
+  const int y0 = c_tile0[usel], y1r = c_tile1[usel], ty = c_tilet[usel];
+  // The second row of a tile is the next site (the O tile reads the first H
+  // row and discards it).  Rows of v_xz and v_yz:
+  //   P=0 (X=i,Y=j,Z=k): v_xz = ik[x][z], v_yz = jk[y][z]
+  //   P=1 (X=i,Y=k,Z=j): v_xz = ij[x][z], v_yz = jk[z][y]
+  //   P=2 (X=j,Y=k,Z=i): v_xz = ij[z][x], v_yz = ik[z][y]
+  int aoff, boff, ao2, bo2, as, bs;
+  if (TR) {  // transposed blocks ji, ki, kj at 3, 4, 5
+    const int xzblk = P == 0 ? 1 : P == 1 ? 0 : 3;
+    const int yzblk = P == 0 ? 2 : P == 1 ? 5 : 4;
+    aoff = xzblk * NPAIR + x0 * NS3; boff = yzblk * NPAIR + y0 * NS3;
+    ao2 = bo2 = NS3; as = bs = 1;
+  } else {
+    const bool acol = P == 2, bcol = P != 0;
+    const int xzblk = P == 0 ? 1 : 0, yzblk = P == 2 ? 1 : 2;
+    aoff = xzblk * NPAIR + (acol ? x0 : x0 * NS3);
+    boff = yzblk * NPAIR + (bcol ? y0 : y0 * NS3);
+    ao2 = acol ? 1 : NS3; as = acol ? NS3 : 1;
+    bo2 = bcol ? 1 : NS3; bs = bcol ? NS3 : 1;
+  }
+  T gacc[2][2] = {{T(0), T(0)}, {T(0), T(0)}};  // fd w dE/dr of the tile's site pairs
   double eacc0 = 0.0, eacc1 = 0.0;
   for (int rr = 0; rr < nreg; ++rr) {
     const int X = reg0 + rr;
+    // coefficients of this region (the previous region's readers have passed
+    // the barrier that ends the triple stage)
+    for (int q = tid; q < NCOEF; q += 256) sC[q] = coef[X * NCOEF + q];
+    if (rr == 0) __syncthreads();  // sP and g
     // ---------------- pair stage: e, e*r and d ln e / dr for the 867 site pairs
     for (int p = tid; p < NPAIR3; p += 256) {
       const int blk = p / NPAIR, rem = p - blk * NPAIR;
@@ -212,27 +243,16 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
       v.x = e;
       v.y = e * rt;
       sV[p] = v;
-      sV[(blk + 3) * NPAIR + y * NS3 + x] = v;
+      if (TR) sV[(blk + 3) * NPAIR + y * NS3 + x] = v;
       sDer[p] = -T(par.x) + T(par.y) * tg * damp;
-      if (rr == 0) sR[p] = r;
+      if (rr == 0) sR[p] = rt;
     }
     __syncthreads();
     // ---------------- site-triple stage
-    if (tid < 243) {
-      const int P = tid / 81, rem = tid - P * 81;
-      const int xb = rem / 9, yb = rem - xb * 9;
-      const int x0 = c_tile0[xb], x1r = c_tile1[xb], tx = c_tilet[xb];
-      const int y0 = c_tile0[yb], y1r = c_tile1[yb], ty = c_tilet[yb];
-      const int x1 = x1r < 0 ? x0 : x1r, y1 = y1r < 0 ? y0 : y1r;
-      // rows of v_xz and v_yz: P=0 (X=i,Y=j,Z=k): ik, jk; P=1 (X=i,Y=k,Z=j): ij, kj;
-      // P=2 (X=j,Y=k,Z=i): ji, ki.  The second row of a tile is always the next
-      // row in LDS (the O tile reads the first H row and discards it), so two
-      // pointers and immediate offsets address all four rows.
-      const int xzblk = P == 0 ? 1 : P == 1 ? 0 : 3;
-      const int yzblk = P == 0 ? 2 : P == 1 ? 5 : 4;
-      const T2* ap = sV + xzblk * NPAIR + x0 * NS3;
-      const T2* bp = sV + yzblk * NPAIR + y0 * NS3;
-      const T* cp = sC + rr * NCOEF + (tx * NTYPE + ty) * NTYPE * 8;
+    if (tile) {
+      const T2* ap = sV + aoff;
+      const T2* bp = sV + boff;
+      const T* cp = sC + (tx * NTYPE + ty) * NTYPE * 8;
       T G[2][2][2];
 #pragma unroll
       for (int xs = 0; xs < 2; ++xs)
@@ -245,7 +265,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
         T M[2][2][2][2];
         const int nz = c_nz[tz];
         {
-          const T2 a0 = ap[0], a1 = ap[NS3], b0 = bp[0], b1 = bp[NS3];
+          const T2 a0 = ap[0], a1 = ap[ao2], b0 = bp[0], b1 = bp[bo2];
           const T av[2][2] = {{a0.x, a0.y}, {a1.x, a1.y}};
           const T bv[2][2] = {{b0.x, b0.y}, {b1.x, b1.y}};
 #pragma unroll
@@ -256,10 +276,12 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
               for (int p = 0; p < 2; ++p)
 #pragma unroll
                 for (int q = 0; q < 2; ++q) M[xs][ys][p][q] = bv[ys][p] * av[xs][q];
+          ap += as;
+          bp += bs;
         }
 #pragma unroll 1
         for (int z = 1; z < nz; ++z) {
-          const T2 a0 = ap[z], a1 = ap[NS3 + z], b0 = bp[z], b1 = bp[NS3 + z];
+          const T2 a0 = ap[0], a1 = ap[ao2], b0 = bp[0], b1 = bp[bo2];
           const T av[2][2] = {{a0.x, a0.y}, {a1.x, a1.y}};
           const T bv[2][2] = {{b0.x, b0.y}, {b1.x, b1.y}};
 #pragma unroll
@@ -270,9 +292,9 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
               for (int p = 0; p < 2; ++p)
 #pragma unroll
                 for (int q = 0; q < 2; ++q) M[xs][ys][p][q] += bv[ys][p] * av[xs][q];
+          ap += as;
+          bp += bs;
         }
-        ap += nz;
-        bp += nz;
         T cc[8];
 #pragma unroll
         for (int a = 0; a < 8; ++a) cc[a] = cp[tz * 8 + a];
@@ -288,21 +310,17 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
                 for (int s = 0; s < 2; ++s) G[xs][ys][s] += cc[(p * 2 + q) * 2 + s] * M[xs][ys][p][q];
       }
       const T fdw = T(g.fd * g.w[rr]);
-      const int outb = P * NPAIR;
 #pragma unroll
       for (int xs = 0; xs < 2; ++xs) {
-        if (xs == 1 && x1r < 0) continue;
-        const int x = xs ? x1 : x0;
+        const int x = xs ? x1r : x0;
 #pragma unroll
         for (int ys = 0; ys < 2; ++ys) {
-          if (ys == 1 && y1r < 0) continue;
-          const int y = ys ? y1 : y0;
-          const int po = outb + x * NS3 + y;
+          const int y = ys ? y1r : y0;
+          if (x < 0 || y < 0) continue;
+          const int po = P * NPAIR + x * NS3 + y;
           const T2 v = sV[po];
-          const T der = sDer[po];
           const T ev = v.x * G[xs][ys][0] + v.y * G[xs][ys][1];
-          const T dEdr = der * ev + v.x * G[xs][ys][1];
-          sG[po] += fdw * dEdr;
+          gacc[xs][ys] += fdw * (sDer[po] * ev + v.x * G[xs][ys][1]);
           if (P == 0) {
             if (rr == 0) eacc0 += double(ev); else eacc1 += double(ev);
           }
@@ -311,21 +329,32 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
     }
     __syncthreads();
   }
-  // region energies (FP64 reduction)
+  // the tile owner turns dE/dr into h = (dE/dr)/r and the site virial sum r dE/dr
+  double wsum = 0.0;
+  if (tile) {
+#pragma unroll
+    for (int xs = 0; xs < 2; ++xs) {
+      const int x = xs ? x1r : x0;
+#pragma unroll
+      for (int ys = 0; ys < 2; ++ys) {
+        const int y = ys ? y1r : y0;
+        if (x < 0 || y < 0) continue;
+        const int po = P * NPAIR + x * NS3 + y;
+        const T r = sR[po];
+        wsum += double(gacc[xs][ys]) * double(r);
+        sR[po] = gacc[xs][ys] / r;
+      }
+    }
+  }
+  // region energies and the site virial (FP64, fixed reduction trees); the
+  // first reduction's barrier also publishes h
   const double EX0 = block_sum256(eacc0, sRed);
   const double EX1 = nreg == 2 ? block_sum256(eacc1, sRed) : 0.0;
+  const double Wsites = block_sum256(wsum, sRed);
   const double Eb = g.w[0] * EX0 + g.w[1] * EX1;
   const double Et = g.fd * Eb;
   const double dEdR = g.dfd * Eb + g.fd * (g.wp[0] * EX0 + g.wp[1] * EX1);  // dE/dR_avg
-  // ---------------- chain rule: h = (dE/dr)/r; site virial sum r dE/dr
-  double wsum = 0.0;
-  for (int p = tid; p < NPAIR3; p += 256) {
-    const double gp = double(sG[p]);
-    const double r = sR[p];
-    wsum += gp * r;
-    sR[p] = gp / r;
-  }
-  const double Wsites = block_sum256(wsum, sRed);  // includes the barrier before reuse of sR
+  // ---------------- chain rule: site forces from h
   if (tid < 3 * NS3 * 3) {
     const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
     const int s = rem / 3, c = rem - s * 3;
@@ -334,24 +363,24 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
     if (role == 0) {
 #pragma unroll 4
       for (int y = 0; y < NS3; ++y)
-        F -= sR[s * NS3 + y] * (aij + (sP[0][s][c] - sP[1][y][c]));
+        F -= double(sR[s * NS3 + y]) * (aij + (sP[0][s][c] - sP[1][y][c]));
 #pragma unroll 4
       for (int z = 0; z < NS3; ++z)
-        F -= sR[NPAIR + s * NS3 + z] * (aik + (sP[0][s][c] - sP[2][z][c]));
+        F -= double(sR[NPAIR + s * NS3 + z]) * (aik + (sP[0][s][c] - sP[2][z][c]));
     } else if (role == 1) {
 #pragma unroll 4
       for (int x = 0; x < NS3; ++x)
-        F += sR[x * NS3 + s] * (aij + (sP[0][x][c] - sP[1][s][c]));
+        F += double(sR[x * NS3 + s]) * (aij + (sP[0][x][c] - sP[1][s][c]));
 #pragma unroll 4
       for (int z = 0; z < NS3; ++z)
-        F -= sR[2 * NPAIR + s * NS3 + z] * (ajk + (sP[1][s][c] - sP[2][z][c]));
+        F -= double(sR[2 * NPAIR + s * NS3 + z]) * (ajk + (sP[1][s][c] - sP[2][z][c]));
     } else {
 #pragma unroll 4
       for (int x = 0; x < NS3; ++x)
-        F += sR[NPAIR + x * NS3 + s] * (aik + (sP[0][x][c] - sP[2][s][c]));
+        F += double(sR[NPAIR + x * NS3 + s]) * (aik + (sP[0][x][c] - sP[2][s][c]));
 #pragma unroll 4
       for (int y = 0; y < NS3; ++y)
-        F += sR[2 * NPAIR + y * NS3 + s] * (ajk + (sP[1][y][c] - sP[2][s][c]));
+        F += double(sR[2 * NPAIR + y * NS3 + s]) * (ajk + (sP[1][y][c] - sP[2][s][c]));
     }
     unsafeAtomicAdd(D.acc + (long)g.m[role] * ACC_W + rem, F);
   } else if (tid < 3 * NS3 * 3 + 9) {
@@ -658,37 +687,44 @@ __global__ __launch_bounds__(256) void k_energy_chunks(Dev D, int which, long nc
   }
 }
 
-// Level 2: per box, the chunk partial sums in chunk order ->
-// {qfs2, qfs3 (kcal/mol), vir increment (10 J/mol)}.
-__global__ void k_energy_boxes(Dev D, int do2, int do3, double* out) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// Level 2: one wavefront per box; lane l sums chunks l, l+64, ... of its box in
+// order, then a fixed butterfly -> {qfs2, qfs3 (kcal/mol), vir increment
+// (10 J/mol)}.  The order depends only on the chunk layout, not on timing.
+__global__ __launch_bounds__(64) void k_energy_boxes(Dev D, int do2, int do3, double* out) {
+  const int b = blockIdx.x;
   if (b >= D.nbox) return;
+  const int lane = threadIdx.x;
   double e2 = 0.0, w2 = 0.0, e3 = 0.0, w3 = 0.0;
   if (do2)
-    for (int q = D.box_c2[b]; q < D.box_c2[b + 1]; ++q) {
+    for (int q = D.box_c2[b] + lane; q < D.box_c2[b + 1]; q += 64) {
       e2 += D.red2[2 * q];
       w2 += D.red2[2 * q + 1];
     }
   if (do3)
-    for (int q = D.box_c3[b]; q < D.box_c3[b + 1]; ++q) {
+    for (int q = D.box_c3[b] + lane; q < D.box_c3[b + 1]; q += 64) {
       e3 += D.red3[2 * q];
       w3 += D.red3[2 * q + 1];
     }
-  out[3 * b + 0] = e2 * H2KCAL;
-  out[3 * b + 1] = H2KCAL * e3;
-  out[3 * b + 2] = w2 * H2KCAL * 418.4 + w3 * H2KCAL * 418.4;
+  e2 = wave_sum(e2);
+  w2 = wave_sum(w2);
+  e3 = wave_sum(e3);
+  w3 = wave_sum(w3);
+  if (lane == 0) {
+    out[3 * b + 0] = e2 * H2KCAL;
+    out[3 * b + 1] = H2KCAL * e3;
+    out[3 * b + 2] = w2 * H2KCAL * 418.4 + w3 * H2KCAL * 418.4;
+  }
 }
 
-// The FP64 variant is held to two 256-thread blocks per CU by its 63 KB of LDS;
-// the mixed variant (36 KB of LDS) is compiled for three.
-__global__ __launch_bounds__(256, 2) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
+// LDS per block: FP64 ~43 KB (3 blocks per CU), mixed ~29 KB (5 blocks per CU).
+__global__ __launch_bounds__(256, 3) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const double* __restrict__ coef) {
-  fs3_eval_body<double>(D, K, pairpar, coef);
+  fs3_eval_body<double, false>(D, K, pairpar, coef);
 }
 
-__global__ __launch_bounds__(256, 3) void k_fs3_eval_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
+__global__ __launch_bounds__(256, 5) void k_fs3_eval_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const float* __restrict__ coef) {
-  fs3_eval_body<float>(D, K, pairpar, coef);
+  fs3_eval_body<float, true>(D, K, pairpar, coef);
 }
 
 }  // namespace ccpol3b
