@@ -15,21 +15,20 @@
 #include "ccpol3b_internal.h"
 
 namespace ccpol3b {
-// kernels (lists.hip, eval.hip)
+// kernels (lists.hip, eval.hip, fs2.hip)
 __global__ void k_prep(int, int, const double*, const double*, const double*, const double*,
                        const double*, const double*, Dev);
 __global__ void k_neighbours(Dev, Consts);
 __global__ void k_fs3_list(Dev, Consts, int);
-__global__ void k_fs2_pairs(Dev);
 __global__ void k_fs2_items(Dev, Consts, int);
 __global__ void k_scan(Dev, int, int, int);
 __global__ void k_box_ranges(Dev, int, int);
 __global__ void k_fs3_geo(Dev, Consts);
 __global__ void k_fs3_eval_f64(Dev, Consts, const double4*, const double*);
 __global__ void k_fs3_eval_f32(Dev, Consts, const double4*, const float*);
-__global__ void k_fs2_items_eval(Dev, Consts);
-__global__ void k_fs2_pair_prep(Dev);
-__global__ void k_fs2_pair_sites(Dev, Consts);
+__global__ void k_fs2_pair_pass(Dev, Consts);
+__global__ void k_fs2_spectator_pass(Dev, Consts);
+size_t fs2_lds_bytes(int cap_nb);
 __global__ void k_finish(Dev, int, int, int, int, double*, double*, double*, double*, double*);
 __global__ void k_energy_chunks(Dev, int, long);
 __global__ void k_energy_boxes(Dev, int, int, double*);
@@ -109,7 +108,7 @@ struct ccpol3b_ctx {
   // evaluation buffers
   Dev D{};
   int cap_nm = 0, cap_box = 0;
-  int* h_totals = nullptr;  // pinned
+  int* h_totals = nullptr;  // pinned: [0, T_COUNT) mid-call copy, [T_COUNT, 2 T_COUNT) end of call
   // staging for the host entry point
   double* d_in = nullptr;   // x,y,z,xs,ys,zs
   double* d_out = nullptr;  // fxs,fys,fzs (ld=17), fcom2, fcom3, energies
@@ -159,8 +158,8 @@ void free_dev(ccpol3b_ctx* c) {
   dev_free(D.nb2_cnt); dev_free(D.nb2); dev_free(D.up2);
   dev_free(D.sl3_cnt); dev_free(D.sl3_off); dev_free(D.sl2_cnt); dev_free(D.sl2_off);
   dev_free(D.cnt3s); dev_free(D.off3s); dev_free(D.cnt2s); dev_free(D.off2s);
-  dev_free(D.tri); dev_free(D.geo3); dev_free(D.pcnt2); dev_free(D.poff2); dev_free(D.pairs2); dev_free(D.items2);
-  dev_free(D.wt2); dev_free(D.acc); dev_free(D.ew3); dev_free(D.ew2);
+  dev_free(D.tri); dev_free(D.geo3); dev_free(D.pcnt2); dev_free(D.poff2); dev_free(D.items2);
+  dev_free(D.acc); dev_free(D.ew3); dev_free(D.ew2);
   dev_free(D.box_t3); dev_free(D.box_t2); dev_free(D.totals);
   dev_free(D.box_c3); dev_free(D.box_c2); dev_free(D.red3); dev_free(D.red2); dev_free(D.s2pair);
   D.cap_red = 0;
@@ -184,8 +183,7 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
         (rc = dev_alloc(c, D.pcnt2, n)) || (rc = dev_alloc(c, D.poff2, n + 1)) ||
         (rc = dev_alloc(c, D.cnt3s, nn)) || (rc = dev_alloc(c, D.off3s, nn + 1)) ||
         (rc = dev_alloc(c, D.cnt2s, nn)) || (rc = dev_alloc(c, D.off2s, nn + 1)) ||
-        (rc = dev_alloc(c, D.pairs2, nn)) || (rc = dev_alloc(c, D.wt2, nn)) ||
-        (rc = dev_alloc(c, D.s2pair, 2 * nn)) ||
+        (rc = dev_alloc(c, D.s2pair, 2 * nn)) || (rc = dev_alloc(c, D.ew2, (size_t)n * FS2_PARTS * 2)) ||
         (rc = dev_alloc(c, D.acc, (size_t)n * ACC_W)))
       return rc;
     D.cap_sl3 = D.cap_sl2 = D.cap_pairs2 = (long)nn;
@@ -213,10 +211,10 @@ int ensure_lists(ccpol3b_ctx* c, long n3, long ni2) {
   }
   if (ni2 > D.cap_items2) {
     const long n = std::max<long>(ni2 + ni2 / 8, 1024);
-    if ((rc = dev_alloc(c, D.items2, n)) || (rc = dev_alloc(c, D.ew2, (size_t)n * 2))) return rc;
+    if ((rc = dev_alloc(c, D.items2, n))) return rc;
     D.cap_items2 = n;
   }
-  const long nred = std::max(D.cap3, D.cap_items2) / RED_CHUNK + D.nbox + 2;
+  const long nred = std::max<long>(D.cap3, (long)c->cap_nm * FS2_PARTS) / RED_CHUNK + D.nbox + 2;
   if (nred > D.cap_red) {
     if ((rc = dev_alloc(c, D.red3, (size_t)nred * 2)) || (rc = dev_alloc(c, D.red2, (size_t)nred * 2))) return rc;
     D.cap_red = nred;
@@ -257,7 +255,6 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     // sub-list sweeps, counting; their sizes are bounded by nm * cap_nb
     const long max_sl = std::min<long>((long)nm * D.cap_nb, (long)c->cu_count * 32);
     if (do3) hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 0);
-    if (do2) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 0);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 1, do3, do2);
     LAUNCHCHK(c, "list counts");
     HIPCHK(c, hipMemcpyAsync(c->h_totals, D.totals, T_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -280,23 +277,20 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     }
     break;
   }
-  const long n3 = do3 ? c->h_totals[T_N3] : 0, ni2 = do2 ? c->h_totals[T_NI2] : 0;
+  const long n3 = do3 ? c->h_totals[T_N3] : 0;
   const long np2 = do2 ? c->h_totals[T_NP2] : 0;
-  const long nsl3 = c->h_totals[T_SL3], nsl2 = c->h_totals[T_SL2];
-  if ((rc = ensure_lists(c, n3, ni2))) return rc;
+  const long nsl3 = c->h_totals[T_SL3];
+  if ((rc = ensure_lists(c, n3, 0))) return rc;
   if (do3 && nsl3 > 0) hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(nsl3)), dim3(256), 0, st, D, c->K, 1);
-  if (do2) {
-    hipLaunchKernelGGL(k_fs2_pairs, dim3((nm + 255) / 256), dim3(256), 0, st, D);
-    if (nsl2 > 0) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nsl2)), dim3(256), 0, st, D, c->K, 1);
-    if (np2 > 0) HIPCHK(c, hipMemsetAsync(D.wt2, 0, sizeof(double) * np2, st));
-  }
   hipLaunchKernelGGL(k_box_ranges, dim3(1), dim3(64), 0, st, D, do3, do2);
   LAUNCHCHK(c, "list fill");
   HIPCHK(c, hipEventRecord(c->ev[1], st));
-  if (do2 && ni2 > 0) {
-    hipLaunchKernelGGL(k_fs2_pair_prep, dim3((np2 * 16 + 255) / 256), dim3(256), 0, st, D);
-    hipLaunchKernelGGL(k_fs2_items_eval, dim3((ni2 + 255) / 256), dim3(256), 0, st, D, c->K);
-    hipLaunchKernelGGL(k_fs2_pair_sites, dim3((np2 * 9 + 255) / 256), dim3(256), 0, st, D, c->K);
+  if (do2) {
+    // pair pass (S_ij, pair-owned forces) before the spectator pass that reads S_ij
+    const size_t lds = fs2_lds_bytes(D.cap_nb);
+    if (lds > 65536) return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the spectator pass");
+    if (np2 > 0) hipLaunchKernelGGL(k_fs2_pair_pass, dim3(waves_grid(np2)), dim3(256), 0, st, D, c->K);
+    hipLaunchKernelGGL(k_fs2_spectator_pass, dim3(nm * FS2_PARTS), dim3(256), lds, st, D, c->K);
     LAUNCHCHK(c, "FS2");
   }
   HIPCHK(c, hipEventRecord(c->ev[2], st));
@@ -315,23 +309,24 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   const long nfin = (long)nm * ACC_W;
   hipLaunchKernelGGL(k_finish, dim3((nfin + 255) / 256), dim3(256), 0, st, D, ld, do3 ? NS3 : (do2 ? NS2 : 0),
                      do2, do3, fxs, fys, fzs, fcom2, fcom3);
-  const long nch3 = n3 / RED_CHUNK + nbox + 1, nch2 = ni2 / RED_CHUNK + nbox + 1;
+  const long nch3 = n3 / RED_CHUNK + nbox + 1, nch2 = (long)nm * FS2_PARTS / RED_CHUNK + nbox + 1;
   if (do3) hipLaunchKernelGGL(k_energy_chunks, dim3(nch3), dim3(256), 0, st, D, 3, nch3);
   if (do2) hipLaunchKernelGGL(k_energy_chunks, dim3(nch2), dim3(256), 0, st, D, 2, nch2);
   hipLaunchKernelGGL(k_energy_boxes, dim3(nbox), dim3(64), 0, st, D, do2, do3, energies);
   LAUNCHCHK(c, "finish");
   HIPCHK(c, hipEventRecord(c->ev[4], st));
-  HIPCHK(c, hipMemcpyAsync(c->h_totals + T_HIST, D.totals + T_HIST, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(c->h_totals + T_COUNT, D.totals, T_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
   c->stats.n_fs3 = n3;
   c->stats.n_fs2_pairs = np2;
-  c->stats.n_fs2_items = ni2;
   return CCPOL3B_OK;
 }
 
 void finish_stats(ccpol3b_ctx* c) {
-  c->stats.n_fs3_sr = c->h_totals[T_HIST];
-  c->stats.n_fs3_switch = c->h_totals[T_HIST + 1];
-  c->stats.n_fs3_lr = c->h_totals[T_HIST + 2];
+  const int* tot = c->h_totals + T_COUNT;
+  c->stats.n_fs3_sr = tot[T_HIST];
+  c->stats.n_fs3_switch = tot[T_HIST + 1];
+  c->stats.n_fs3_lr = tot[T_HIST + 2];
+  c->stats.n_fs2_items = c->s2 ? tot[T_NI2] : 0;
   c->stats.ms_lists = event_ms(c->ev[0], c->ev[1]);
   c->stats.ms_fs2 = event_ms(c->ev[1], c->ev[2]);
   c->stats.ms_fs3 = event_ms(c->ev[2], c->ev[3]);
@@ -493,7 +488,7 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
   if (hipMemcpy(c->d_pairpar, pairpar.data(), sizeof(double4) * 2 * NPAIR, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_coef64, c64.data(), sizeof(double) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_coef32, c32.data(), sizeof(float) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_totals, T_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_totals, 2 * T_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     ccpol3b_destroy(c);
     return CCPOL3B_EHIP;
   }
@@ -687,19 +682,35 @@ long ccpol3b_get_fs3_list(ccpol3b_ctx* c, int* ijk, long cap) {
 
 long ccpol3b_get_fs2_list(ccpol3b_ctx* c, int* kij, long cap) {
   if (!c || (cap > 0 && !kij)) return CCPOL3B_EINVAL;
-  const long n = c->stats.n_fs2_items;
-  const long m = std::min(n, cap);
+  if (!c->s2 || c->D.nm == 0) return 0;
+  // The evaluation does not materialise the FS2 items; build them from the
+  // neighbour lists of the last call, in the reference's order.
+  Dev& D = c->D;
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
+  const hipStream_t st = nullptr;
+  const long nsl2 = c->h_totals[T_SL2];
+  if (nsl2 > 0) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nsl2)), dim3(256), 0, st, D, c->K, 0);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 2, 0, 1);
+  LAUNCHCHK(c, "FS2 list count");
+  int ni = 0;
+  HIPCHK(c, hipMemcpy(&ni, D.totals + T_NI2L, sizeof(int), hipMemcpyDeviceToHost));
+  int rc;
+  if ((rc = ensure_lists(c, 0, ni))) return rc;
+  if (nsl2 > 0) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nsl2)), dim3(256), 0, st, D, c->K, 1);
+  LAUNCHCHK(c, "FS2 list fill");
+  const long m = std::min<long>(ni, cap);
   if (m > 0) {
     std::vector<int4> tmp(m);
-    if (hipMemcpy(tmp.data(), c->D.items2, sizeof(int4) * m, hipMemcpyDeviceToHost) != hipSuccess)
-      return fail(c, CCPOL3B_EHIP, "copy of the FS2 list");
+    HIPCHK(c, hipMemcpy(tmp.data(), D.items2, sizeof(int4) * m, hipMemcpyDeviceToHost));
     for (long t = 0; t < m; ++t) {
       kij[3 * t] = tmp[t].x;
       kij[3 * t + 1] = tmp[t].y;
       kij[3 * t + 2] = tmp[t].z;
     }
+  } else {
+    HIPCHK(c, hipDeviceSynchronize());
   }
-  return n;
+  return ni;
 }
 
 }  // extern "C"

@@ -24,7 +24,9 @@ constexpr int T_NBOVER = 3;  // largest neighbour count when a list overflowed
 constexpr int T_HIST = 4;    // [4..6] FS3 region histogram SR, switch, LR
 constexpr int T_SL3 = 7;     // FS3 sub-lists (i; j)
 constexpr int T_SL2 = 8;     // FS2 sub-lists (k; i)
+constexpr int T_NI2L = 9;    // FS2 items of an exported list
 constexpr int T_COUNT = 16;
+constexpr int FS2_PARTS = 4;     // spectator-pass blocks per molecule
 constexpr int RED_CHUNK = 2048;  // entries per block in the first level of the energy reduction
 
 constexpr double BOHR2A = 0.529177249;  // common_arrays.f90:34
@@ -79,16 +81,15 @@ struct Dev {
   // FS3 triplets (i<j<k, lexicographic)
   int4 *tri;                  // [cap3] (i,j,k,-)
   TriGeo *geo3;               // [cap3] geometry of each triplet
-  // FS2 pairs (i<j within rcut3) and k-major items (k,i,j,pair)
+  // FS2 pairs (i<j within the FS2 cutoff): pair index poff2[i] + q - up2[i] for
+  // j = nb2(i)[q]; k-major items (k,i,j,pair), built only for list export
   int *pcnt2, *poff2;         // [nm], [nm+1]
-  int2 *pairs2;               // [cap_pairs2]
   int4 *items2;               // [cap_items2]
-  double *wt2;                // [cap_pairs2] sum over spectators of f f T
   // results (hartree, bohr)
   double *acc;                // [nm][ACC_W] force accumulators
   double *ew3;                // [cap3][2] per-triplet energy, virial
-  double *ew2;                // [cap_items2/64+1][2] per-wavefront FS2 energy, virial
-  int *box_t3, *box_t2;       // [nbox+1] triplet / item ranges of each box
+  double *ew2;                // [nm][FS2_PARTS][2] FS2 energy, virial per spectator-pass block
+  int *box_t3, *box_t2;       // [nbox+1] triplet / FS2 energy-entry ranges of each box
   int *box_c3, *box_c2;       // [nbox+1] first reduction chunk of each box
   double *red3, *red2;        // [chunks][2] first-level energy/virial partial sums
   long cap_red;               // capacity of red3/red2 in chunks

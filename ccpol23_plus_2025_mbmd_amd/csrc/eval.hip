@@ -1,5 +1,5 @@
-// FS3 and FS2 evaluation kernels, deterministic per-molecule gathers and the
-// per-box energy/virial reductions.
+// FS3 evaluation kernels, the accumulator write-back and the per-box
+// energy/virial reductions.
 //
 // FS3 (threebody_potential.f90:676-2062).  For an accepted triplet (i,j,k) and
 // region X (SR or LR) the reference energy is
@@ -18,51 +18,9 @@
 // dv0/dr = e' and dv1/dr = e + r e', e' = e (-beta + gamma (1 - damp)).
 // The triplet energy is f_d(R_avg) * sum_X w_X(R_avg) E_X with the damping and
 // switch of :894-904 and :1420-1423.
-#include "ccpol3b_internal.h"
+#include "devcommon.h"
 
 namespace ccpol3b {
-
-template <typename T> struct Pair2;
-template <> struct Pair2<double> { using type = double2; };
-template <> struct Pair2<float> { using type = float2; };
-
-__device__ __forceinline__ double minimg_e(double d, double L) {
-  return L > 0.0 ? d - L * round(d / L) : d;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Deterministic block sum over 256 threads (4 wavefronts).
-__device__ __forceinline__ double block_sum256(double v, double* red) {
-  v = wave_sum(v);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[w] = v;
-  __syncthreads();
-  return (red[0] + red[1]) + (red[2] + red[3]);
-}
-
-template <typename T> __device__ __forceinline__ T t_exp(T x);
-template <> __device__ __forceinline__ double t_exp<double>(double x) { return exp(x); }
-template <> __device__ __forceinline__ float t_exp<float>(float x) { return __expf(x); }
-
-// x**delta for the damping exponent: exponentiation by squaring when delta is
-// a small whole number (10 in both shipped CONTROL files), pow otherwise.
-__device__ __forceinline__ double pow_delta(double x, const Consts& K) {
-  if (K.idelta > 0) {
-    double r = 1.0, b = x;
-    for (int e = K.idelta; e; e >>= 1) {
-      if (e & 1) r *= b;
-      b *= b;
-    }
-    return r;
-  }
-  return pow(x, K.delta33);
-}
 
 // 2x2 site tiles of one molecule's 17 sites, aligned to the type blocks
 // (O | H H | D1 D1 | N1 N1 | N1 N1 | N2 N2 | N2 N2 | N3 N3 | N3 N3); -1 pads O.
@@ -397,235 +355,6 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
   if (tid == 0) {
     D.ew3[2 * t] = Et;
     D.ew3[2 * t + 1] = Wsites + g.Ravg * dEdR;
-  }
-}
-
-// ====================================================================== FS2
-// FS2 (threebody_potential.f90:101-674).  For an accepted item (k; i<j):
-//   E = f(R_ki) f(R_kj) S_ij T_k,  S_ij = eta sum_{a,b in O,H,H} exp(-beta r_ab),
-//   T_k = sum_{ks=1..5} q_ks (1/|r_ks-R_i| + 1/|r_ks-R_j| - 1/|r_ks-R_i^+| - 1/|r_ks-R_j^+|)
-// with R_i^+ = R_i + g u_ij, R_j^+ = R_j - g u_ij and f the damping of :339-349.
-// The pair pass owns everything that belongs to i and j (S_ij and its site
-// gradient, COM forces of i and j); the spectator pass owns everything that
-// belongs to k.  Both recompute T_k, so neither needs cross-thread scatter.
-__constant__ double c_q2[NS2] = {0.258504607653328389, 0.564050362569283648, 0.564050362569283648,
-                                 -0.693302666395947953, -0.693302666395947953};
-constexpr double ETA3 = 0.48056977605478400;
-constexpr double BETA3 = 0.85836606579445000;
-constexpr double G3 = 0.400000000000000;
-
-struct Fs2Geo {
-  double xij[3], xki[3], xkj[3];
-  double Rij, Rki, Rkj;
-};
-
-__device__ __forceinline__ void fs2_damp(double R, const Consts& K, double& f, double& fp) {
-  const double pom = exp(K.alp3 * (R - K.r0dmp2));
-  const double pp = 1.0 / (1.0 + pom);
-  f = pow_delta(pp, K);
-  fp = -K.alp3 * K.delta33 * pom * pp * f;  // df/dR
-}
-
-// T_k and its derivatives.  sk: spectator site offsets [5][3].
-// dTdsk: dT/d(site ks), dTdRi, dTdRj: explicit COM derivatives, Tl: dT/dlambda
-// under uniform scaling of all vectors except the fixed shift g.
-template <bool WANT_SITES, bool WANT_COM>
-__device__ __forceinline__ double fs2_T(const Fs2Geo& G, const double (*sk)[3], double (*dTdsk)[3],
-                                        double* dTdRi, double* dTdRj, double& Tl) {
-  const double uij[3] = {G.xij[0] / G.Rij, G.xij[1] / G.Rij, G.xij[2] / G.Rij};
-  double T = 0.0;
-  Tl = 0.0;
-  if (WANT_COM)
-    for (int c = 0; c < 3; ++c) dTdRi[c] = dTdRj[c] = 0.0;
-  for (int ks = 0; ks < NS2; ++ks) {
-    const double q = c_q2[ks];
-    double dci[3], dcj[3], dqi[3], dqj[3];
-    for (int c = 0; c < 3; ++c) {
-      dci[c] = G.xki[c] + sk[ks][c];
-      dcj[c] = G.xkj[c] + sk[ks][c];
-      dqi[c] = dci[c] - G3 * uij[c];
-      dqj[c] = dcj[c] + G3 * uij[c];
-    }
-    const double ici = 1.0 / sqrt(dci[0] * dci[0] + dci[1] * dci[1] + dci[2] * dci[2]);
-    const double icj = 1.0 / sqrt(dcj[0] * dcj[0] + dcj[1] * dcj[1] + dcj[2] * dcj[2]);
-    const double iqi = 1.0 / sqrt(dqi[0] * dqi[0] + dqi[1] * dqi[1] + dqi[2] * dqi[2]);
-    const double iqj = 1.0 / sqrt(dqj[0] * dqj[0] + dqj[1] * dqj[1] + dqj[2] * dqj[2]);
-    const double ici3 = ici * ici * ici, icj3 = icj * icj * icj;
-    const double iqi3 = iqi * iqi * iqi, iqj3 = iqj * iqj * iqj;
-    const double term0 = ici + icj - iqi - iqj;
-    T += q * term0;
-    const double pqi = dqi[0] * uij[0] + dqi[1] * uij[1] + dqi[2] * uij[2];
-    const double pqj = dqj[0] * uij[0] + dqj[1] * uij[1] + dqj[2] * uij[2];
-    Tl += q * (-term0 + G3 * pqi * iqi3 - G3 * pqj * iqj3);
-    if (WANT_SITES)
-      for (int c = 0; c < 3; ++c)
-        dTdsk[ks][c] = q * (-dci[c] * ici3 - dcj[c] * icj3 + dqi[c] * iqi3 + dqj[c] * iqj3);
-    if (WANT_COM) {
-      // projector P = I - u u^T applied to dqi, dqj
-      for (int c = 0; c < 3; ++c) {
-        const double Pqi = dqi[c] - pqi * uij[c];
-        const double Pqj = dqj[c] - pqj * uij[c];
-        dTdRi[c] += q * (dci[c] * ici3 - (dqi[c] + G3 * Pqi / G.Rij) * iqi3 + G3 * Pqj / G.Rij * iqj3);
-        dTdRj[c] += q * (dcj[c] * icj3 + G3 * Pqi / G.Rij * iqi3 - (dqj[c] + G3 * Pqj / G.Rij) * iqj3);
-      }
-    }
-  }
-  return T;
-}
-
-__device__ __forceinline__ void fs2_geometry(const Dev& D, int k, int i, int j, Fs2Geo& G) {
-  const double L = D.boxL[D.mol_box[i]];
-  const double pi_[3] = {D.cx[i], D.cy[i], D.cz[i]};
-  const double pj_[3] = {D.cx[j], D.cy[j], D.cz[j]};
-  const double pk_[3] = {D.cx[k], D.cy[k], D.cz[k]};
-  for (int c = 0; c < 3; ++c) {
-    G.xij[c] = minimg_e(pi_[c] - pj_[c], L);
-    G.xki[c] = minimg_e(pk_[c] - pi_[c], L);
-    G.xkj[c] = G.xki[c] + G.xij[c];  // same periodic image (:325-328)
-  }
-  G.Rij = sqrt(G.xij[0] * G.xij[0] + G.xij[1] * G.xij[1] + G.xij[2] * G.xij[2]);
-  G.Rki = sqrt(G.xki[0] * G.xki[0] + G.xki[1] * G.xki[1] + G.xki[2] * G.xki[2]);
-  G.Rkj = sqrt(G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2]);
-}
-
-// Segmented inclusive sum over runs of equal keys within a wavefront (keys
-// form contiguous runs); the last lane of each run ends up with the run total.
-__device__ __forceinline__ double seg_sum(double v, long long key, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const double vu = __shfl_up(v, d, 64);
-    const long long ku = __shfl_up(key, d, 64);
-    if (lane >= d && ku == key) v += vu;
-  }
-  return v;
-}
-
-// One thread per FS2 item (k; i<j) of the k-major list.  Writes the spectator
-// site forces and all three COM forces through wavefront-segmented sums and
-// FP64 atomics, accumulates sum_k f f T per pair for the exchange-site forces,
-// and stores the item's energy and virial for the deterministic box sums.
-__global__ __launch_bounds__(256) void k_fs2_items_eval(Dev D, Consts K) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const long nit = D.totals[T_NI2];
-  const bool live = t < nit;
-  int k = -1, i = -1, pr = -1;
-  double Fk[NS2 * 3 + 3], Fi[3], Fj[3], WT = 0.0, E = 0.0, W = 0.0;
-#pragma unroll
-  for (int a = 0; a < NS2 * 3 + 3; ++a) Fk[a] = 0.0;
-  Fi[0] = Fi[1] = Fi[2] = Fj[0] = Fj[1] = Fj[2] = 0.0;
-  int j = -1;
-  if (live) {
-    const int4 it = D.items2[t];
-    k = it.x; i = it.y; j = it.z; pr = it.w;
-    Fs2Geo G;
-    fs2_geometry(D, k, i, j, G);
-    const double S = D.s2pair[2 * (long)pr], Sl = D.s2pair[2 * (long)pr + 1];
-    double sk[NS2][3];
-    for (int s2 = 0; s2 < NS2; ++s2)
-      for (int c = 0; c < 3; ++c) sk[s2][c] = D.site[((long)k * NS3 + s2) * 3 + c];
-    double dTdsk[NS2][3], dTdRi[3], dTdRj[3], Tl;
-    const double T = fs2_T<true, true>(G, sk, dTdsk, dTdRi, dTdRj, Tl);
-    double fki, fpki, fkj, fpkj;
-    fs2_damp(G.Rki, K, fki, fpki);
-    fs2_damp(G.Rkj, K, fkj, fpkj);
-    const double ff = fki * fkj;
-    for (int s2 = 0; s2 < NS2; ++s2)
-      for (int c = 0; c < 3; ++c) Fk[s2 * 3 + c] = -ff * S * dTdsk[s2][c];
-    for (int c = 0; c < 3; ++c) {
-      const double uki = G.xki[c] / G.Rki, ukj = G.xkj[c] / G.Rkj;
-      Fk[NS2 * 3 + c] = -S * T * (fpki * fkj * uki + fki * fpkj * ukj);
-      Fi[c] = -ff * S * dTdRi[c] + S * T * fpki * fkj * uki;
-      Fj[c] = -ff * S * dTdRj[c] + S * T * fki * fpkj * ukj;
-    }
-    WT = ff * T;
-    E = ff * S * T;
-    W = ff * (Sl * T + S * Tl) + S * T * (G.Rki * fpki * fkj + G.Rkj * fki * fpkj);
-  }
-  // spectator part: runs of equal k (the list is k-major)
-  const int knext = __shfl_down(k, 1, 64);
-  const bool ktail = live && (lane == 63 || knext != k);
-#pragma unroll
-  for (int a = 0; a < NS2 * 3 + 3; ++a) {
-    const double v = seg_sum(Fk[a], (long long)k, lane);
-    if (ktail) {
-      const int off = a < NS2 * 3 ? a : ACC_C2 + (a - NS2 * 3);
-      unsafeAtomicAdd(D.acc + (long)k * ACC_W + off, v);
-    }
-  }
-  // COM of i: runs of equal (k, i)
-  const long long ki = live ? (long long)k * D.nm + i : -1;
-  const long long kinext = __shfl_down(ki, 1, 64);
-  const bool itail = live && (lane == 63 || kinext != ki);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double v = seg_sum(Fi[c], ki, lane);
-    if (itail) unsafeAtomicAdd(D.acc + (long)i * ACC_W + ACC_C2 + c, v);
-  }
-  if (live) {
-    for (int c = 0; c < 3; ++c) unsafeAtomicAdd(D.acc + (long)j * ACC_W + ACC_C2 + c, Fj[c]);
-    unsafeAtomicAdd(D.wt2 + pr, WT);
-    D.ew2[2 * t] = E;
-    D.ew2[2 * t + 1] = W;
-  }
-}
-
-// Exchange sum of every FS2 pair (i<j): S_ij = eta sum_{is,js in O,H,H} exp(-beta r)
-// (:361-372) and its scaling derivative sum -beta r eta exp(-beta r); one
-// thread per (pair, site pair) and a 9-lane sum.
-__global__ __launch_bounds__(256) void k_fs2_pair_prep(Dev D) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long p = g / 16;
-  const int q = (int)(g - p * 16);
-  const long np = D.totals[T_NP2];
-  double e = 0.0, el = 0.0;
-  if (p < np && q < 9) {
-    const int2 pr = D.pairs2[p];
-    const int is = q / 3, js = q - is * 3;
-    const double L = D.boxL[D.mol_box[pr.x]];
-    const double* si = D.site + ((long)pr.x * NS3 + is) * 3;
-    const double* sj = D.site + ((long)pr.y * NS3 + js) * 3;
-    const double vx = minimg_e(D.cx[pr.x] - D.cx[pr.y], L) + (si[0] - sj[0]);
-    const double vy = minimg_e(D.cy[pr.x] - D.cy[pr.y], L) + (si[1] - sj[1]);
-    const double vz = minimg_e(D.cz[pr.x] - D.cz[pr.y], L) + (si[2] - sj[2]);
-    const double r = sqrt(vx * vx + vy * vy + vz * vz);
-    e = ETA3 * exp(-BETA3 * r);
-    el = -BETA3 * r * e;
-  }
-#pragma unroll
-  for (int o = 8; o > 0; o >>= 1) {
-    e += __shfl_xor(e, o, 16);
-    el += __shfl_xor(el, o, 16);
-  }
-  if (p < np && q == 0) {
-    D.s2pair[2 * p] = e;
-    D.s2pair[2 * p + 1] = el;
-  }
-}
-
-// Exchange-site forces of FS2 (:477-519 for is/js): F_s = -(sum_k f f T) dS_ij/ds.
-// One thread per (pair, site pair).
-__global__ __launch_bounds__(256) void k_fs2_pair_sites(Dev D, Consts K) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long p = g / 9;
-  const int q = (int)(g - p * 9);
-  if (p >= D.totals[T_NP2]) return;
-  const double WT = D.wt2[p];
-  if (WT == 0.0) return;
-  const int2 pr = D.pairs2[p];
-  const int i = pr.x, j = pr.y, is = q / 3, js = q - is * 3;
-  const double L = D.boxL[D.mol_box[i]];
-  const double* si = D.site + ((long)i * NS3 + is) * 3;
-  const double* sj = D.site + ((long)j * NS3 + js) * 3;
-  double v[3];
-  v[0] = minimg_e(D.cx[i] - D.cx[j], L) + (si[0] - sj[0]);
-  v[1] = minimg_e(D.cy[i] - D.cy[j], L) + (si[1] - sj[1]);
-  v[2] = minimg_e(D.cz[i] - D.cz[j], L) + (si[2] - sj[2]);
-  const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-  const double g1 = -WT * (-BETA3 * ETA3 * exp(-BETA3 * r) / r);  // -WT dS/dv = F on site is
-  for (int c = 0; c < 3; ++c) {
-    unsafeAtomicAdd(D.acc + (long)i * ACC_W + is * 3 + c, g1 * v[c]);
-    unsafeAtomicAdd(D.acc + (long)j * ACC_W + js * 3 + c, -g1 * v[c]);
   }
 }
 

@@ -12,7 +12,7 @@
 // FS2 (k,i,j) with i<j (:297-301).
 #pragma clang fp contract(off)
 
-#include "ccpol3b_internal.h"
+#include "devcommon.h"
 
 namespace ccpol3b {
 
@@ -35,16 +35,6 @@ __device__ __forceinline__ int upper_start(const int* list, int n, int v) {
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
     if (list[mid] > v) hi = mid; else lo = mid + 1;
-  }
-  return lo;
-}
-
-// owner of flat index s in an exclusive-offset array off[0..n]
-__device__ __forceinline__ int owner_of(const int* off, int n, long s) {
-  int lo = 0, hi = n;  // off[lo] <= s < off[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (off[mid] <= s) lo = mid; else hi = mid;
   }
   return lo;
 }
@@ -184,18 +174,7 @@ __global__ __launch_bounds__(256) void k_fs3_list(Dev D, Consts K, int mode) {
   }
 }
 
-// ------------------------------------------------------------------ FS2 pairs and items
-// Unordered pairs i<j within rcut3: the upper part of nb2(i).
-__global__ void k_fs2_pairs(Dev D) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= D.nm) return;
-  const int* nb = D.nb2 + (long)i * D.cap_nb;
-  const int n = D.nb2_cnt[i], u0 = D.up2[i];
-  const long o = D.poff2[i];
-  for (int q = u0; q < n; ++q)
-    if (o + q - u0 < D.cap_pairs2) D.pairs2[o + q - u0] = make_int2(i, nb[q]);
-}
-
+// ------------------------------------------------------------------ FS2 items
 // Ordered items (k; i<j): one wavefront per sub-list (k; i = nb2(k)[a]); the
 // candidates j are the upper entries of nb2(i) other than k, accepted when the
 // derived k-j vector x_kj = x_ki + x_ij is inside the cutoff (:325-336).
@@ -296,7 +275,8 @@ __device__ void block_exclusive_scan(const int* cnt, int* off, long n, int* tota
 }
 
 // phase 0: per-molecule counts -> sub-list and pair offsets
-// phase 1: per-sub-list candidate counts -> list offsets
+// phase 1: per-sub-list candidate counts -> FS3 triplet offsets
+// phase 2: per-sub-list candidate counts -> FS2 item offsets (list export only)
 __global__ __launch_bounds__(1024) void k_scan(Dev D, int phase, int do3, int do2) {
   if (phase == 0) {
     if (do3) block_exclusive_scan(D.sl3_cnt, D.sl3_off, D.nm, D.totals + T_SL3);
@@ -304,22 +284,23 @@ __global__ __launch_bounds__(1024) void k_scan(Dev D, int phase, int do3, int do
       block_exclusive_scan(D.sl2_cnt, D.sl2_off, D.nm, D.totals + T_SL2);
       block_exclusive_scan(D.pcnt2, D.poff2, D.nm, D.totals + T_NP2);
     }
-  } else {
+  } else if (phase == 1) {
     if (do3) block_exclusive_scan(D.cnt3s, D.off3s, D.totals[T_SL3], D.totals + T_N3);
-    if (do2) block_exclusive_scan(D.cnt2s, D.off2s, D.totals[T_SL2], D.totals + T_NI2);
+  } else {
+    if (do2) block_exclusive_scan(D.cnt2s, D.off2s, D.totals[T_SL2], D.totals + T_NI2L);
   }
 }
 
-// Triplet / item range of each box (lists are sorted by their first molecule
-// and boxes are contiguous in molecule index), and the box-aligned chunking of
-// those ranges used by the energy reduction.  One thread.
+// Triplet range of each box (the list is sorted by its first molecule, and
+// boxes are contiguous molecule ranges) and the first reduction chunk of each
+// box, for FS3 triplets and for the FS2 energies (FS2_PARTS entries per molecule).
 __global__ void k_box_ranges(Dev D, int do3, int do2) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   int c3 = 0, c2 = 0;
   for (int b = 0; b <= D.nbox; ++b) {
     const int m = D.box_start[b];
     const int t3 = do3 ? (m < D.nm ? D.off3s[D.sl3_off[m]] : D.totals[T_N3]) : 0;
-    const int t2 = do2 ? (m < D.nm ? D.off2s[D.sl2_off[m]] : D.totals[T_NI2]) : 0;
+    const int t2 = do2 ? m * FS2_PARTS : 0;
     D.box_t3[b] = t3;
     D.box_t2[b] = t2;
     D.box_c3[b] = c3;
@@ -327,7 +308,7 @@ __global__ void k_box_ranges(Dev D, int do3, int do2) {
     if (b < D.nbox) {
       const int m1 = D.box_start[b + 1];
       const int e3 = do3 ? (m1 < D.nm ? D.off3s[D.sl3_off[m1]] : D.totals[T_N3]) : 0;
-      const int e2 = do2 ? (m1 < D.nm ? D.off2s[D.sl2_off[m1]] : D.totals[T_NI2]) : 0;
+      const int e2 = do2 ? m1 * FS2_PARTS : 0;
       c3 += (e3 - t3 + RED_CHUNK - 1) / RED_CHUNK;
       c2 += (e2 - t2 + RED_CHUNK - 1) / RED_CHUNK;
     }
