@@ -7,6 +7,13 @@ inputs resident in HBM.  The metric is FS3 triplet evaluations per second of
 the mixed-precision path; the FP64 path is timed beside it, the force error of
 mixed vs FP64 is reported, and the CPU oracle is timed on the host cores.
 
+Roofline: the dominant kernel is the FS3 evaluation (k_fs3_eval_f32 in the
+mixed path), bound by the vector ALU.  Its algorithmic work is 221,479 ops per
+region evaluation (a triplet in the 4.5-5.5 A switch band is two); its duration
+comes from HIP events recorded on the evaluation stream around that kernel.
+HBM traffic per launch is read from the committed rocprofv3 counter summary
+when present (FETCH_SIZE + WRITE_SIZE, calibrated on a copy of known size).
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -14,6 +21,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -31,7 +39,8 @@ METRIC = "3-body triplet evals/s (E+F), 512 H2O: mixed vs FP64 at force RMS err<
 W3_OPS = 221_479
 PEAK_FP32_VECTOR_TFLOPS = 157.3   # MI355X datasheet, packed FP32 vector
 PEAK_FP64_VECTOR_TFLOPS = 78.6    # MI355X datasheet, FP64 vector
-PEAK_FP32_NONPACKED_TFLOPS = 78.6  # 256 CU x 64 lanes x 2 x 2.4 GHz, one FMA per lane per clock
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # 256 CU x 4 SIMD x 16 lanes x 2 flops x 2.4 GHz, one FMA per lane per clock
+TRAFFIC_FILE = ROOT / "profiles" / "k_fs3_eval_f32_512w_traffic.txt"
 
 
 def log(*a):
@@ -47,6 +56,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=2025)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the timed region, run one device copy of this many MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -59,28 +70,38 @@ def timed_steps(tb, com, site, box, steps, warmup):
     for _ in range(warmup):
         tb.eval_device(com, site, box, fs, c2, c3, en)
     torch.cuda.synchronize()
-    ms_fs3 = []
+    ms_eval, ms_fs3 = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         tb.eval_device(com, site, box, fs, c2, c3, en)
-        ms_fs3.append(tb.stats()["ms_fs3"])
+        st = tb.stats()  # waits for the call's events
+        ms_eval.append(st["ms_fs3_eval"])
+        ms_fs3.append(st["ms_fs3"])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    return (t1 - t0) * 1e3 / steps, float(np.mean(ms_fs3)), tb.stats()
+    return (t1 - t0) * 1e3 / steps, float(np.mean(ms_eval)), float(np.mean(ms_fs3)), tb.stats()
+
+
+def read_traffic():
+    """HBM bytes per launch of k_fs3_eval_f32 from the committed counter summary."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", TRAFFIC_FILE.read_text(), re.M)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(p, box, threads):
     from oracle import oracle as O
-    log(f"cpu baseline: oracle (faithful per-triplet table rebuild), {threads} threads, full 512-water call")
+    log(f"cpu baseline: oracle (faithful per-triplet table rebuild), {threads} threads, full {box.nm}-water call")
     t0 = time.perf_counter()
     r = O.threebody_term(p, box, nthreads=threads, rebuild=True)
     wall = time.perf_counter() - t0
     rate = r["n_fs3"] / wall
     return {"value": rate, "unit": "triplet evals/s", "cores": threads, "kind": "port",
-            "sample": (f"one full FS2+FS3 E+F call at 512 waters ({r['n_fs3']} FS3 triplets, {r['n_fs2']} FS2 "
-                       f"items) of the C FP64 restatement in the reference's loop order with the per-triplet "
-                       f"orbit-table rebuild, {wall:.2f} s wall"),
-            "ms_per_call": wall * 1e3}, r
+            "sample": (f"one full FS2+FS3 E+F call at {box.nm} waters ({r['n_fs3']} FS3 triplets, {r['n_fs2']} "
+                       f"FS2 items) of the C FP64 restatement in the reference's loop order with the "
+                       f"per-triplet orbit-table rebuild, {threads} OpenMP threads, {wall:.2f} s wall"),
+            "ms_per_call": wall * 1e3}
 
 
 def main():
@@ -105,49 +126,64 @@ def main():
     tb_mix = ThreeBody(p, "mixed")
     tb_64 = ThreeBody(p, "fp64")
     log(f"box: {box.nm} waters, L={box.box:.4f} A")
-    ms_mix, ms_fs3_mix, st_mix = timed_steps(tb_mix, com, site, box.box, args.steps, args.warmup)
-    log(f"mixed: {ms_mix:.3f} ms/step, FS3 kernel {ms_fs3_mix:.3f} ms, stats {st_mix}")
-    ms_64, ms_fs3_64, st_64 = timed_steps(tb_64, com, site, box.box, args.steps, args.warmup)
-    log(f"fp64:  {ms_64:.3f} ms/step, FS3 kernel {ms_fs3_64:.3f} ms")
+    ms_mix, ms_eval_mix, ms_fs3_mix, st_mix = timed_steps(tb_mix, com, site, box.box, args.steps, args.warmup)
+    log(f"mixed: {ms_mix:.3f} ms/step, FS3 eval kernel {ms_eval_mix:.3f} ms, stats {st_mix}")
+    ms_64, ms_eval_64, ms_fs3_64, st_64 = timed_steps(tb_64, com, site, box.box, args.steps, args.warmup)
+    log(f"fp64:  {ms_64:.3f} ms/step, FS3 eval kernel {ms_eval_64:.3f} ms")
 
-    # force error of mixed vs FP64 on this snapshot
+    if args.copy_mb > 0:
+        a = torch.empty(args.copy_mb << 20, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_mb} MiB")
+
+    # force error of mixed vs FP64 on this snapshot (host entry point)
     r64 = tb_64.evaluate(box)
     rmx = tb_mix.evaluate(box)
     f64 = np.stack([r64["fxs"][:, :17], r64["fys"][:, :17], r64["fzs"][:, :17]])
     fmx = np.stack([rmx["fxs"][:, :17], rmx["fys"][:, :17], rmx["fzs"][:, :17]])
     ferr = float(np.sqrt(((fmx - f64) ** 2).sum() / (f64 ** 2).sum()))
-    eerr = float(abs((rmx["qfs2"] + rmx["qfs3"]) - (r64["qfs2"] + r64["qfs3"])) / abs(r64["qfs2"] + r64["qfs3"]))
+    e64 = r64["qfs2"] + r64["qfs3"]
+    eerr = float(abs((rmx["qfs2"] + rmx["qfs3"]) - e64) / abs(e64))
 
     n3 = st_mix["n_fs3"]
     nreg = st_mix["n_fs3_sr"] + st_mix["n_fs3_lr"] + 2 * st_mix["n_fs3_switch"]
     value = n3 / (ms_mix * 1e-3)
     value_64 = n3 / (ms_64 * 1e-3)
     ops = nreg * W3_OPS
-    achieved = ops / (ms_fs3_mix * 1e-3) / 1e12
-    achieved_64 = ops / (ms_fs3_64 * 1e-3) / 1e12
+    achieved = ops / (ms_eval_mix * 1e-3) / 1e12
+    achieved_64 = ops / (ms_eval_64 * 1e-3) / 1e12
+    traffic = read_traffic()
 
     out = {
         "metric": METRIC, "value": value, "unit": "triplet evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_mix, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{box.nm} rigid H2O, L={box.box:.2f} A, jittered 8x8x8 lattice, random "
                                f"quaternions (seed {args.seed}), FS2+FS3 fs2fs3 parameters, mixed precision "
-                               f"(FP32 exp/polynomial, FP64 displacements, switch and reductions)",
+                               f"(FP32 exp/polynomial, FP64 displacements, FS2, switch and reductions)",
                    "n_molecules": box.nm, "n_fs3_triplets": n3, "n_fs2_items": st_mix["n_fs2_items"],
+                   "n_fs2_pairs": st_mix["n_fs2_pairs"],
                    "region_hist_sr_switch_lr": [st_mix["n_fs3_sr"], st_mix["n_fs3_switch"], st_mix["n_fs3_lr"]]},
-        "roofline": {"bound": "valu", "kernel": "k_fs3_eval<float>", "achieved": achieved,
+        "roofline": {"bound": "valu", "kernel": "k_fs3_eval_f32", "achieved": achieved,
                      "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VECTOR_TFLOPS,
-                     "traffic": None, "ops_per_launch": ops, "ms_per_launch": ms_fs3_mix,
-                     "peak_fp32_nonpacked": PEAK_FP32_NONPACKED_TFLOPS},
-        "fp64": {"value": value_64, "ms_per_step": ms_64, "fs3_kernel_ms": ms_fs3_64,
+                     "traffic": traffic, "ops_per_launch": ops, "ms_per_launch": ms_eval_mix,
+                     "peak_fp32_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
+                     "frac_of_nonpacked": achieved / PEAK_FP32_NONPACKED_TFLOPS},
+        "fp64": {"value": value_64, "ms_per_step": ms_64, "fs3_eval_kernel_ms": ms_eval_64,
                  "achieved_tflops": achieved_64, "frac_of_fp64_vector_peak": achieved_64 / PEAK_FP64_VECTOR_TFLOPS},
         "mixed_over_fp64": value / value_64,
         "mixed_vs_fp64_error": {"site_force_rel_rms": ferr, "energy_rel": eerr},
-        "stats_mixed_ms": {k: st_mix[k] for k in ("ms_lists", "ms_fs2", "ms_fs3", "ms_reduce", "ms_total")},
+        "stats_mixed_ms": {k: st_mix[k] for k in ("ms_lists", "ms_fs2", "ms_fs3", "ms_fs3_eval", "ms_reduce",
+                                                  "ms_total")},
+        "stats_fp64_ms": {k: st_64[k] for k in ("ms_lists", "ms_fs2", "ms_fs3", "ms_fs3_eval", "ms_reduce",
+                                                "ms_total")},
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cb, _ = cpu_baseline(p, box, threads)
+        cb = cpu_baseline(p, box, threads)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
     else:

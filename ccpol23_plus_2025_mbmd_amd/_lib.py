@@ -36,6 +36,7 @@ class Stats(C.Structure):
         ("n_fs2_items", C.c_long), ("n_fs2_pairs", C.c_long),
         ("ms_total", C.c_double), ("ms_lists", C.c_double), ("ms_fs2", C.c_double), ("ms_fs3", C.c_double),
         ("ms_reduce", C.c_double), ("ms_h2d", C.c_double), ("ms_d2h", C.c_double),
+        ("ms_fs3_eval", C.c_double),
     ]
 
     def as_dict(self):

@@ -115,7 +115,7 @@ struct ccpol3b_ctx {
   size_t cap_in = 0, cap_out = 0;
   std::vector<int> h_molbox, h_boxstart;
   std::vector<double> h_boxL;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[9] = {};
 };
 
 namespace {
@@ -296,6 +296,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   HIPCHK(c, hipEventRecord(c->ev[2], st));
   if (do3 && n3 > 0) {
     hipLaunchKernelGGL(k_fs3_geo, dim3((n3 + 255) / 256), dim3(256), 0, st, D, c->K);
+    HIPCHK(c, hipEventRecord(c->ev[8], st));
     const int grid = (int)n3;  // one block per triplet
     if (c->precision == CCPOL3B_MIXED)
       hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
@@ -332,6 +333,7 @@ void finish_stats(ccpol3b_ctx* c) {
   c->stats.ms_fs3 = event_ms(c->ev[2], c->ev[3]);
   c->stats.ms_reduce = event_ms(c->ev[3], c->ev[4]);
   c->stats.ms_total = event_ms(c->ev[0], c->ev[4]);
+  c->stats.ms_fs3_eval = c->stats.n_fs3 > 0 ? event_ms(c->ev[8], c->ev[3]) : 0.0;
 }
 
 void set_single_box(ccpol3b_ctx* c, int nm, double box) {

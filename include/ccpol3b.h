@@ -73,6 +73,7 @@ typedef struct {
   double ms_total;                   /* device time of the whole evaluation (HIP events) */
   double ms_lists, ms_fs2, ms_fs3, ms_reduce;
   double ms_h2d, ms_d2h;             /* host-pointer entry point only */
+  double ms_fs3_eval;                /* the FS3 evaluation kernel alone (inside ms_fs3) */
 } ccpol3b_stats;
 
 /* Create / destroy a context holding device copies of the parameters.
