@@ -80,6 +80,12 @@ def load():
     L.ccpol3b_eval_device.restype = C.c_int
     L.ccpol3b_eval_batch_device.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp, C.c_int] + [vp] * 12 + [vp]
     L.ccpol3b_eval_batch_device.restype = C.c_int
+    L.ccpol3b_eval_batch_grad_device.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp, C.c_int] + [vp] * 13 + [vp]
+    L.ccpol3b_eval_batch_grad_device.restype = C.c_int
+    L.ccpol3b_set_coefficients.argtypes = [vp, dp, dp]
+    L.ccpol3b_set_coefficients.restype = C.c_int
+    L.ccpol3b_gram_device.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp]
+    L.ccpol3b_gram_device.restype = C.c_int
     L.ccpol3b_last_stats.argtypes = [vp, C.POINTER(Stats)]
     L.ccpol3b_last_stats.restype = C.c_int
     L.ccpol3b_get_fs3_list.argtypes = [vp, C.POINTER(C.c_int), C.c_long]
@@ -102,7 +108,7 @@ EXPORTED = [
     "ccpol3b_create", "ccpol3b_destroy", "ccpol3b_set_precision", "ccpol3b_threebody_term",
     "ccpol3b_eval_device", "ccpol3b_eval_batch_device", "ccpol3b_last_stats", "ccpol3b_get_fs3_list",
     "ccpol3b_get_fs2_list", "ccpol3b_read_data3b", "ccpol3b_orbit_index", "ccpol3b_status_string",
-    "ccpol3b_last_error",
+    "ccpol3b_last_error", "ccpol3b_eval_batch_grad_device", "ccpol3b_set_coefficients", "ccpol3b_gram_device",
 ]
 
 

@@ -26,6 +26,13 @@ __global__ void k_box_ranges(Dev, int, int);
 __global__ void k_fs3_geo(Dev, Consts);
 __global__ void k_fs3_eval_f64(Dev, Consts, const double4*, const double*);
 __global__ void k_fs3_eval_f32(Dev, Consts, const double4*, const float*);
+__global__ void k_fs3_phi_f64(Dev, Consts, const double4*, const double*, double*);
+__global__ void k_fs3_phi_f32(Dev, Consts, const double4*, const float*, double*);
+__global__ void k_phi_boxes(Dev, const double*, double*);
+int set_orbit_tables(const short* off, const short* mem);
+__global__ void k_gram_exps(int, int, const double*, int, int*);
+__global__ void k_gram_digits(int, int, int, int, const double*, int, const int*, float*);
+__global__ void k_gram_mfma(int, int, int, const float*, const int*, double*, int);
 __global__ void k_fs2_pair_pass(Dev, Consts);
 __global__ void k_fs2_spectator_pass(Dev, Consts);
 size_t fs2_lds_bytes(int cap_nb);
@@ -85,6 +92,25 @@ int first_site_of_type(int t) {
   return -1;
 }
 
+// Orbit (0-based) of every entry of the dense table [tx][ty][tz][p][q][s]: all
+// sites of one type share their labels, so a representative site per type
+// gives the entry's orbit.
+const std::vector<int>& dense_orbit() {
+  static const std::vector<int> d = [] {
+    std::vector<int> v(NCOEF);
+    const OrbitMap& om = orbit_map();
+    for (int tx = 0; tx < NTYPE; ++tx)
+      for (int ty = 0; ty < NTYPE; ++ty)
+        for (int tz = 0; tz < NTYPE; ++tz)
+          for (int k = 0; k < 8; ++k)
+            v[((tx * NTYPE + ty) * NTYPE + tz) * 8 + k] =
+                om(first_site_of_type(tx), first_site_of_type(ty), first_site_of_type(tz), k >> 2, (k >> 1) & 1,
+                   k & 1) - 1;
+    return v;
+  }();
+  return d;
+}
+
 template <typename P>
 void dev_free(P*& p) {
   if (p) (void)hipFree(p);
@@ -104,6 +130,11 @@ struct ccpol3b_ctx {
   double4* d_pairpar = nullptr;  // [2][289]
   double* d_coef64 = nullptr;    // [2][NCOEF]
   float* d_coef32 = nullptr;
+  double* d_phitrip = nullptr;   // [cap_phi][728] per-triplet coefficient-gradient rows
+  long cap_phi = 0;
+  float* d_gdig = nullptr;       // Gram digits [6][rows][cols]
+  int* d_gex = nullptr;          // Gram column exponents
+  size_t cap_gdig = 0, cap_gex = 0;
   int cu_count = 256;
   // evaluation buffers
   Dev D{};
@@ -233,7 +264,7 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
 // The evaluation proper, on device arrays already in place.
 int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double* y, const double* z,
         const double* xs, const double* ys, const double* zs, double* fxs, double* fys, double* fzs,
-        double* fcom2, double* fcom3, double* energies, hipStream_t st) {
+        double* fcom2, double* fcom3, double* energies, hipStream_t st, double* phi = nullptr) {
   Dev& D = c->D;
   int rc;
   const int do3 = c->s3, do2 = c->s2;
@@ -298,13 +329,31 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     hipLaunchKernelGGL(k_fs3_geo, dim3((n3 + 255) / 256), dim3(256), 0, st, D, c->K);
     HIPCHK(c, hipEventRecord(c->ev[8], st));
     const int grid = (int)n3;  // one block per triplet
-    if (c->precision == CCPOL3B_MIXED)
+    if (phi) {
+      if (n3 > c->cap_phi) {
+        const long n = n3 + n3 / 8;
+        if ((rc = dev_alloc(c, c->d_phitrip, (size_t)n * 2 * NLIN3))) return rc;
+        c->cap_phi = n;
+      }
+      if (c->precision == CCPOL3B_MIXED)
+        hipLaunchKernelGGL(k_fs3_phi_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+                           (const float*)c->d_coef32, c->d_phitrip);
+      else
+        hipLaunchKernelGGL(k_fs3_phi_f64, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+                           (const double*)c->d_coef64, c->d_phitrip);
+    } else if (c->precision == CCPOL3B_MIXED) {
       hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
                          (const float*)c->d_coef32);
-    else
+    } else {
       hipLaunchKernelGGL(k_fs3_eval_f64, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
                          (const double*)c->d_coef64);
+    }
     LAUNCHCHK(c, "k_fs3_eval");
+  }
+  if (phi) {
+    hipLaunchKernelGGL(k_phi_boxes, dim3((2 * NLIN3 + 255) / 256, nbox), dim3(256), 0, st, D,
+                       (const double*)c->d_phitrip, phi);
+    LAUNCHCHK(c, "k_phi_boxes");
   }
   HIPCHK(c, hipEventRecord(c->ev[3], st));
   const long nfin = (long)nm * ACC_W;
@@ -340,6 +389,29 @@ void set_single_box(ccpol3b_ctx* c, int nm, double box) {
   c->h_molbox.assign(nm, 0);
   c->h_boxstart = {0, nm};
   c->h_boxL = {box > 0.0 ? box / BOHR2A : 0.0};
+}
+
+int eval_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld, const double* x,
+               const double* y, const double* z, const double* xs, const double* ys, const double* zs, double* fxs,
+               double* fys, double* fzs, double* fcom2, double* fcom3, double* energies, double* phi,
+               void* stream);
+
+// Dense coefficient tables C[X][tx][ty][tz][p][q][s] = cs_X[orbit], FP64 and FP32.
+int upload_coefficients(ccpol3b_ctx* c, const double* cs_sr, const double* cs_lr) {
+  std::vector<double> c64(2 * NCOEF);
+  std::vector<float> c32(2 * NCOEF);
+  const std::vector<int>& dob = dense_orbit();
+  for (int X = 0; X < 2; ++X) {
+    const double* cs = X ? cs_lr : cs_sr;
+    for (int d = 0; d < NCOEF; ++d) {
+      c64[(size_t)X * NCOEF + d] = cs[dob[d]];
+      c32[(size_t)X * NCOEF + d] = (float)cs[dob[d]];
+    }
+  }
+  if (hipMemcpy(c->d_coef64, c64.data(), sizeof(double) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_coef32, c32.data(), sizeof(float) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(c, CCPOL3B_EHIP, "coefficient upload");
+  return CCPOL3B_OK;
 }
 
 }  // namespace
@@ -462,34 +534,26 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
         pairpar[X * NPAIR + is * 17 + js] = make_double4(b[q], g[q], r[q], 0.0);
       }
   }
-  // dense coefficient tables C[X][tx][ty][tz][p][q][s]
-  std::vector<double> c64(2 * NCOEF);
-  std::vector<float> c32(2 * NCOEF);
-  const OrbitMap& om = orbit_map();
-  for (int X = 0; X < 2; ++X) {
-    const double* cs = X ? p->cs_lr : p->cs_sr;
-    for (int tx = 0; tx < NTYPE; ++tx)
-      for (int ty = 0; ty < NTYPE; ++ty)
-        for (int tz = 0; tz < NTYPE; ++tz)
-          for (int pp = 0; pp < 2; ++pp)
-            for (int qq = 0; qq < 2; ++qq)
-              for (int ss = 0; ss < 2; ++ss) {
-                const int n = om(first_site_of_type(tx), first_site_of_type(ty), first_site_of_type(tz),
-                                 pp, qq, ss);
-                const size_t q = (size_t)X * NCOEF + (((tx * NTYPE + ty) * NTYPE + tz) * 2 + pp) * 4 + qq * 2 + ss;
-                c64[q] = cs[n - 1];
-                c32[q] = (float)cs[n - 1];
-              }
+  // orbit -> dense entries (CSR) for the coefficient-gradient kernels
+  {
+    const std::vector<int>& dob = dense_orbit();
+    std::vector<short> off(NLIN3 + 1, 0), mem(NCOEF);
+    for (int d = 0; d < NCOEF; ++d) ++off[dob[d] + 1];
+    for (int n = 0; n < NLIN3; ++n) off[n + 1] += off[n];
+    std::vector<short> fill(off.begin(), off.end() - 1);
+    for (int d = 0; d < NCOEF; ++d) mem[fill[dob[d]]++] = (short)d;
+    if (set_orbit_tables(off.data(), mem.data()) != 0) {
+      delete c;
+      return CCPOL3B_EHIP;
+    }
   }
   int rc = CCPOL3B_OK;
   if ((rc = dev_alloc(c, c->d_pairpar, 2 * NPAIR)) || (rc = dev_alloc(c, c->d_coef64, 2 * NCOEF)) ||
-      (rc = dev_alloc(c, c->d_coef32, 2 * NCOEF))) {
+      (rc = dev_alloc(c, c->d_coef32, 2 * NCOEF)) || (rc = upload_coefficients(c, p->cs_sr, p->cs_lr))) {
     ccpol3b_destroy(c);
     return rc;
   }
   if (hipMemcpy(c->d_pairpar, pairpar.data(), sizeof(double4) * 2 * NPAIR, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_coef64, c64.data(), sizeof(double) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_coef32, c32.data(), sizeof(float) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
       hipHostMalloc((void**)&c->h_totals, 2 * T_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     ccpol3b_destroy(c);
     return CCPOL3B_EHIP;
@@ -511,6 +575,9 @@ void ccpol3b_destroy(ccpol3b_ctx* c) {
   dev_free(c->d_pairpar);
   dev_free(c->d_coef64);
   dev_free(c->d_coef32);
+  dev_free(c->d_phitrip);
+  dev_free(c->d_gdig);
+  dev_free(c->d_gex);
   dev_free(c->d_in);
   dev_free(c->d_out);
   if (c->h_totals) (void)hipHostFree(c->h_totals);
@@ -543,10 +610,65 @@ int ccpol3b_eval_device(ccpol3b_ctx* c, int nm, int ld, const double* x, const d
   return CCPOL3B_OK;
 }
 
+int ccpol3b_gram_device(ccpol3b_ctx* c, int nrow, int ncol, const double* phi, int ld_phi, double* gram,
+                        int ld_gram, void* stream) {
+  if (!c || nrow < 0 || ncol <= 0 || ld_phi < ncol || ld_gram < ncol || (nrow > 0 && !phi) || !gram)
+    return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  const int nrp = std::max(64, (nrow + 63) / 64 * 64), ncp = (ncol + 31) / 32 * 32;
+  const size_t ndig = (size_t)6 * nrp * ncp;
+  int rc;
+  if (ndig > c->cap_gdig) {
+    if ((rc = dev_alloc(c, c->d_gdig, ndig))) return rc;
+    c->cap_gdig = ndig;
+  }
+  if ((size_t)ncol > c->cap_gex) {
+    if ((rc = dev_alloc(c, c->d_gex, ncol))) return rc;
+    c->cap_gex = ncol;
+  }
+  hipLaunchKernelGGL(k_gram_exps, dim3((ncol + 255) / 256), dim3(256), 0, st, nrow, ncol, phi, ld_phi, c->d_gex);
+  const long nd = (long)nrp * ncp;
+  hipLaunchKernelGGL(k_gram_digits, dim3((nd + 255) / 256), dim3(256), 0, st, nrow, ncol, nrp, ncp, phi, ld_phi,
+                     (const int*)c->d_gex, c->d_gdig);
+  const int nt = ncp / 32;
+  hipLaunchKernelGGL(k_gram_mfma, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, ncol, nrp, ncp,
+                     (const float*)c->d_gdig, (const int*)c->d_gex, gram, ld_gram);
+  LAUNCHCHK(c, "Gram");
+  return CCPOL3B_OK;
+}
+
+int ccpol3b_set_coefficients(ccpol3b_ctx* c, const double* cs_sr, const double* cs_lr) {
+  if (!c || !cs_sr || !cs_lr) return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
+  // earlier evaluations on any stream still read the old tables
+  HIPCHK(c, hipDeviceSynchronize());
+  return upload_coefficients(c, cs_sr, cs_lr);
+}
+
+int ccpol3b_eval_batch_grad_device(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld,
+                                   const double* x, const double* y, const double* z, const double* xs,
+                                   const double* ys, const double* zs, double* fxs, double* fys, double* fzs,
+                                   double* fcom2, double* fcom3, double* energies, double* phi, void* stream) {
+  if (!phi) return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  return eval_batch(c, nbox, mol_start, box_len, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies,
+                    phi, stream);
+}
+
 int ccpol3b_eval_batch_device(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld,
                               const double* x, const double* y, const double* z, const double* xs,
                               const double* ys, const double* zs, double* fxs, double* fys, double* fzs,
                               double* fcom2, double* fcom3, double* energies, void* stream) {
+  return eval_batch(c, nbox, mol_start, box_len, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies,
+                    nullptr, stream);
+}
+}  // extern "C"
+
+namespace {
+int eval_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld, const double* x,
+               const double* y, const double* z, const double* xs, const double* ys, const double* zs, double* fxs,
+               double* fys, double* fzs, double* fcom2, double* fcom3, double* energies, double* phi,
+               void* stream) {
   if (!c || nbox <= 0 || !mol_start || !box_len || ld < 17) return fail(c, CCPOL3B_EINVAL, "invalid argument");
   const int nm = mol_start[nbox];
   if (mol_start[0] != 0 || nm < 0) return fail(c, CCPOL3B_EINVAL, "mol_start must start at 0");
@@ -562,12 +684,15 @@ int ccpol3b_eval_batch_device(ccpol3b_ctx* c, int nbox, const int* mol_start, co
     for (int m = mol_start[b]; m < mol_start[b + 1]; ++m) c->h_molbox[m] = b;
   }
   if (nm == 0) return CCPOL3B_OK;
-  int rc = run(c, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st);
+  int rc = run(c, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st, phi);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(st));
   finish_stats(c);
   return CCPOL3B_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int ccpol3b_threebody_term(ccpol3b_ctx* c, int nm, int ld, const double* x, const double* y, const double* z,
                            const double* xs, const double* ys, const double* zs, double box, double* qfs2,

@@ -12,6 +12,7 @@ constexpr int NTYPE = 6;       // FS3 site types (O, H, D1, N1, N2, N3)
 constexpr int NPAIR = NS3 * NS3;      // 289 site pairs per molecule pair
 constexpr int NPAIR3 = 3 * NPAIR;     // 867 site pairs per triplet
 constexpr int NCOEF = NTYPE * NTYPE * NTYPE * 8;  // dense coefficients per region
+constexpr int NLIN3 = 364;     // linear coefficients per region (orbits)
 constexpr int ACC_W = NS3 * 3 + 6;     // per-molecule accumulator: 17 site forces, COM(FS2), COM(FS3)
 constexpr int ACC_C2 = NS3 * 3;        // offset of the FS2 COM force
 constexpr int ACC_C3 = NS3 * 3 + 3;    // offset of the FS3 COM force

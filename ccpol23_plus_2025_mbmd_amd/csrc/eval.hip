@@ -107,6 +107,17 @@ template <typename T> __device__ __forceinline__ T t_rcp(T x);
 template <> __device__ __forceinline__ double t_rcp<double>(double x) { return 1.0 / x; }
 template <> __device__ __forceinline__ float t_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Orbit -> dense-table entries (CSR over the 364 linear coefficients), set by
+// set_orbit_tables() when a context is created.
+__constant__ short c_orb_off[NLIN3 + 1];
+__constant__ short c_orb_mem[NCOEF];
+
+int set_orbit_tables(const short* off, const short* mem) {
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_orb_off), off, sizeof(short) * (NLIN3 + 1)) != hipSuccess) return -1;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_orb_mem), mem, sizeof(short) * NCOEF) != hipSuccess) return -1;
+  return 0;
+}
+
 // sites of each type block (O, H, D1, N1, N2, N3)
 __constant__ int c_nz[NTYPE] = {1, 2, 2, 4, 4, 4};
 
@@ -121,9 +132,11 @@ __constant__ int c_nz[NTYPE] = {1, 2, 2, 4, 4, 4};
 // TR: the site-pair blocks are also stored transposed, so both rows of every
 // tile are read with unit stride (more LDS, two address registers per tile);
 // without TR the column-wise passes step with stride 17.
-template <typename T, bool TR>
+// PHI: also write the triplet's row of dE/dc (kcal/mol per unit coefficient;
+// 364 SR then 364 LR) to phi_trip[t][728].
+template <typename T, bool TR, bool PHI>
 __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __restrict__ pairpar,
-                                              const T* __restrict__ coef) {
+                                              const T* __restrict__ coef, double* __restrict__ phi_trip) {
   using T2 = typename Pair2<T>::type;
   // sR holds r and then h = (dE/dr)/r.  The mixed variant's r is FP32-exact
   // (FP32 square root), so T storage loses nothing.
@@ -134,6 +147,8 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
   __shared__ T sR[NPAIR3];
   __shared__ double sRed[4];
   __shared__ TriGeo g;  // block-uniform geometry, read with lane-varying indices
+  // PHI: dE_X/dC of the dense table [tx][ty][tz][p][q][s] of the current region
+  __shared__ T sDC[PHI ? NCOEF : 1];
 
   const long t = blockIdx.x;  // one block per triplet
   if (t >= D.totals[T_N3]) return;
@@ -182,6 +197,8 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
     // coefficients of this region (the previous region's readers have passed
     // the barrier that ends the triple stage)
     for (int q = tid; q < NCOEF; q += 256) sC[q] = coef[X * NCOEF + q];
+    if constexpr (PHI)
+      for (int q = tid; q < NCOEF; q += 256) sDC[q] = T(0);
     if (rr == 0) __syncthreads();  // sP and g
     // ---------------- pair stage: e, e*r and d ln e / dr for the 867 site pairs
     for (int p = tid; p < NPAIR3; p += 256) {
@@ -216,6 +233,21 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
       for (int xs = 0; xs < 2; ++xs)
 #pragma unroll
         for (int ys = 0; ys < 2; ++ys) G[xs][ys][0] = G[xs][ys][1] = T(0);
+      // PHI (pass 0 only): the pair vectors v_xy of the tile's i-j site pairs,
+      // zero for the padding row of the O tile
+      T vxy[2][2][2];
+      if constexpr (PHI) {
+#pragma unroll
+        for (int xs = 0; xs < 2; ++xs)
+#pragma unroll
+          for (int ys = 0; ys < 2; ++ys) {
+            const int x = xs ? x1r : x0, y = ys ? y1r : y0;
+            const bool ok = P == 0 && x >= 0 && y >= 0;
+            const T2 v = sV[ok ? x * NS3 + y : 0];
+            vxy[xs][ys][0] = ok ? v.x : T(0);
+            vxy[xs][ys][1] = ok ? v.y : T(0);
+          }
+      }
 #pragma unroll 1
       for (int tz = 0; tz < NTYPE; ++tz) {
         // M[xs][ys][p][q] = sum_{z in type tz} v_yz[p] v_xz[q]; every type block
@@ -253,6 +285,25 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
           ap += as;
           bp += bs;
         }
+        if constexpr (PHI) {
+          // dE_X/dC[tx][ty][tz][p][q][s] = sum over the tile's (x, y) of v_xy[s] M[x,y,p,q]
+          if (P == 0) {
+            T* dc = sDC + ((tx * NTYPE + ty) * NTYPE + tz) * 8;
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+              for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                  T d = T(0);
+#pragma unroll
+                  for (int xs = 0; xs < 2; ++xs)
+#pragma unroll
+                    for (int ys = 0; ys < 2; ++ys) d += vxy[xs][ys][s] * M[xs][ys][p][q];
+                  atomicAdd(dc + (p * 2 + q) * 2 + s, d);
+                }
+          }
+        }
         T cc[8];
 #pragma unroll
         for (int a = 0; a < 8; ++a) cc[a] = cp[tz * 8 + a];
@@ -286,6 +337,18 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
       }
     }
     __syncthreads();
+    if constexpr (PHI) {
+      // orbit sums of the dense gradient, scaled to the triplet's energy in kcal/mol
+      const double scale = H2KCAL * g.fd * g.w[rr];
+      double* row = phi_trip + t * (2 * NLIN3);
+      for (int n = tid; n < NLIN3; n += 256) {
+        T a = T(0);
+        for (int m = c_orb_off[n]; m < c_orb_off[n + 1]; ++m) a += sDC[c_orb_mem[m]];
+        row[X * NLIN3 + n] = scale * double(a);
+        if (nreg == 1) row[(1 - X) * NLIN3 + n] = 0.0;
+      }
+      __syncthreads();
+    }
   }
   // the tile owner turns dE/dr into h = (dE/dr)/r and the site virial sum r dE/dr
   double wsum = 0.0;
@@ -448,12 +511,34 @@ __global__ __launch_bounds__(64) void k_energy_boxes(Dev D, int do2, int do3, do
 // LDS per block: FP64 ~43 KB (3 blocks per CU), mixed ~29 KB (5 blocks per CU).
 __global__ __launch_bounds__(256, 3) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const double* __restrict__ coef) {
-  fs3_eval_body<double, false>(D, K, pairpar, coef);
+  fs3_eval_body<double, false, false>(D, K, pairpar, coef, nullptr);
 }
 
 __global__ __launch_bounds__(256, 5) void k_fs3_eval_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const float* __restrict__ coef) {
-  fs3_eval_body<float, true>(D, K, pairpar, coef);
+  fs3_eval_body<float, true, false>(D, K, pairpar, coef, nullptr);
+}
+
+// The same evaluation plus the per-triplet coefficient-gradient rows.
+__global__ __launch_bounds__(256, 2) void k_fs3_phi_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
+                                                        const double* __restrict__ coef, double* phi_trip) {
+  fs3_eval_body<double, false, true>(D, K, pairpar, coef, phi_trip);
+}
+
+__global__ __launch_bounds__(256, 4) void k_fs3_phi_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
+                                                        const float* __restrict__ coef, double* phi_trip) {
+  fs3_eval_body<float, true, true>(D, K, pairpar, coef, phi_trip);
+}
+
+// Coefficient gradient of each box: the sum, in triplet order, of its
+// triplets' rows (boxes own contiguous triplet ranges).  Thread per (box, coefficient).
+__global__ __launch_bounds__(256) void k_phi_boxes(Dev D, const double* __restrict__ phi_trip, double* phi) {
+  const int b = blockIdx.y;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= D.nbox || n >= 2 * NLIN3) return;
+  double a = 0.0;
+  for (int t = D.box_t3[b]; t < D.box_t3[b + 1]; ++t) a += phi_trip[(long)t * 2 * NLIN3 + n];
+  phi[(long)b * 2 * NLIN3 + n] = a;
 }
 
 }  // namespace ccpol3b

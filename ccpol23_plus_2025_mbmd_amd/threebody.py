@@ -16,6 +16,7 @@ from . import _lib
 from .params import ThreeBodyParams
 
 PRECISIONS = {"fp64": _lib.FP64, "mixed": _lib.MIXED}
+NLIN = 364  # linear coefficients per region (cs3_s, cs3_l)
 
 
 def _dp(a):
@@ -166,6 +167,56 @@ class ThreeBody:
             p(fsite, 0), p(fsite, 1), p(fsite, 2), p(fcom2), p(fcom3), p(energies), C.c_void_p(stream))
         _lib.check(rc, self._ctx, "ccpol3b_eval_batch_device")
         return fsite, fcom2, fcom3, energies
+
+    def eval_batch_grad_device(self, mol_start, box_len, com, site, phi=None, stream=None):
+        """eval_batch_device plus phi [nbox, 728] = d(qfs3 of the box)/d(cs3_s, cs3_l)
+        (kcal/mol per unit coefficient).  Returns (fsite, fcom2, fcom3, energies, phi)."""
+        import torch
+        nbox = len(box_len)
+        nm, ld = com.shape[1], site.shape[2]
+        dev = com.device
+        ms = np.ascontiguousarray(mol_start, dtype=np.int32)
+        bl = np.ascontiguousarray(box_len, dtype=np.float64)
+        assert ms[0] == 0 and ms[-1] == nm and len(ms) == nbox + 1
+        fsite = torch.zeros((3, nm, ld), dtype=torch.float64, device=dev)
+        fcom2 = torch.zeros((3, nm), dtype=torch.float64, device=dev)
+        fcom3 = torch.zeros((3, nm), dtype=torch.float64, device=dev)
+        energies = torch.zeros((nbox, 3), dtype=torch.float64, device=dev)
+        if phi is None:
+            phi = torch.empty((nbox, 2 * NLIN), dtype=torch.float64, device=dev)
+        assert phi.is_contiguous() and phi.shape == (nbox, 2 * NLIN) and phi.dtype == torch.float64
+        if stream is None:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        p = lambda t, *i: C.c_void_p(t[i].data_ptr()) if i else C.c_void_p(t.data_ptr())
+        rc = _lib.load().ccpol3b_eval_batch_grad_device(
+            self._ctx, nbox, ms.ctypes.data_as(C.POINTER(C.c_int)), _dp(bl), ld,
+            p(com, 0), p(com, 1), p(com, 2), p(site, 0), p(site, 1), p(site, 2),
+            p(fsite, 0), p(fsite, 1), p(fsite, 2), p(fcom2), p(fcom3), p(energies), p(phi), C.c_void_p(stream))
+        _lib.check(rc, self._ctx, "ccpol3b_eval_batch_grad_device")
+        return fsite, fcom2, fcom3, energies, phi
+
+    def set_coefficients(self, cs_sr, cs_lr):
+        """Replace the 364 SR and 364 LR linear coefficients of this context."""
+        a = np.ascontiguousarray(cs_sr, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(cs_lr, dtype=np.float64).reshape(-1)
+        assert a.size == NLIN and b.size == NLIN
+        _lib.check(_lib.load().ccpol3b_set_coefficients(self._ctx, _dp(a), _dp(b)), self._ctx,
+                   "ccpol3b_set_coefficients")
+
+    def gram(self, phi, stream=None):
+        """G = phi^T phi (float64 [ncol, ncol]) of a float64 cuda matrix phi [nrow, ncol]
+        on the matrix cores (FP32 digits, FP64-accurate)."""
+        import torch
+        assert phi.is_cuda and phi.dtype == torch.float64 and phi.dim() == 2
+        phi = phi.contiguous()
+        nrow, ncol = phi.shape
+        g = torch.empty((ncol, ncol), dtype=torch.float64, device=phi.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(phi.device).cuda_stream
+        rc = _lib.load().ccpol3b_gram_device(self._ctx, nrow, ncol, C.c_void_p(phi.data_ptr()), ncol,
+                                             C.c_void_p(g.data_ptr()), ncol, C.c_void_p(stream))
+        _lib.check(rc, self._ctx, "ccpol3b_gram_device")
+        return g
 
     # ------------------------------------------------------------------ introspection
     def stats(self) -> dict:

@@ -122,6 +122,32 @@ int ccpol3b_eval_batch_device(ccpol3b_ctx *ctx, int nbox, const int *mol_start,
                               double *d_fcom2, double *d_fcom3,
                               double *d_energies, void *stream);
 
+/* Gradient path (no counterpart in the reference, which has no derivative with
+ * respect to its fitted parameters): the batched evaluation above plus, per
+ * box, dE/dc = d(qfs3)/d(cs3_s(1:364), cs3_l(1:364)) in kcal/mol per unit
+ * coefficient, written to d_phi[box][728] (SR then LR).  FS2 has no fitted
+ * linear coefficients.  E is linear in c, so sum_n d_phi[b][n] c_n = qfs3(b). */
+int ccpol3b_eval_batch_grad_device(ccpol3b_ctx *ctx, int nbox, const int *mol_start,
+                                   const double *box_len, int ld_site,
+                                   const double *d_x, const double *d_y, const double *d_z,
+                                   const double *d_xs, const double *d_ys, const double *d_zs,
+                                   double *d_fxs, double *d_fys, double *d_fzs,
+                                   double *d_fcom2, double *d_fcom3,
+                                   double *d_energies, double *d_phi, void *stream);
+
+/* Gram matrix gram[a][b] = sum_r phi[r][a] phi[r][b] (row-major, FP64) of a
+ * design matrix of coefficient gradients (rows = configurations), on the FP32
+ * matrix cores with an exact digit decomposition that gives FP64 accuracy.
+ * No counterpart in the reference (its refit code is commented out,
+ * threebody_potential.f90:1071,1304).  Enqueued on `stream`; the context owns
+ * the scratch. */
+int ccpol3b_gram_device(ccpol3b_ctx *ctx, int nrow, int ncol, const double *d_phi, int ld_phi,
+                        double *d_gram, int ld_gram, void *stream);
+
+/* Replace the linear SR/LR coefficients cs3_s, cs3_l (364 each) of a context,
+ * e.g. during a refit (initialization.f90:660-683 read them once). */
+int ccpol3b_set_coefficients(ccpol3b_ctx *ctx, const double *cs_sr, const double *cs_lr);
+
 int ccpol3b_last_stats(const ccpol3b_ctx *ctx, ccpol3b_stats *stats);
 
 /* Copy the last FS3 triplet list (0-based i<j<k, lexicographic order, the

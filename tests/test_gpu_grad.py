@@ -1,0 +1,189 @@
+"""GPU tests of the gradient path: per-box coefficient gradients, the
+torch.autograd.Function, and the matrix-core Gram matrix.
+
+Pinned by: linearity of E in the coefficients (phi . c = qfs3); the oracle's
+energy with one-hot coefficient vectors (phi_n = E(c = e_n)); central finite
+differences of the GPU energy in COM positions (h = 1e-5 A), orientations
+(h = 1e-5 rad) and coefficients; and an extended-precision Gram reference."""
+import numpy as np
+import pytest
+import torch
+
+from ccpol23_plus_2025_mbmd_amd.geometry import principal_coords, trimer_set
+
+pytestmark = pytest.mark.gpu
+
+NLIN = 364
+
+
+def pack(boxes):
+    com = np.concatenate([b.com for b in boxes])
+    site = np.concatenate([b.site for b in boxes])
+    ms = np.cumsum([0] + [b.nm for b in boxes]).astype(np.int32)
+    bl = np.array([b.box for b in boxes], float)
+    return com, site, ms, bl
+
+
+def coef_vector(p):
+    return np.concatenate([p.sr.cs, p.lr.cs])
+
+
+def away_from_cutoffs(b, h=0.01):
+    d = np.linalg.norm(b.com[:, None] - b.com[None], axis=-1)[np.triu_indices(3, 1)]
+    return np.all(np.abs(d - 6.3) > h) and np.all(np.abs(d - 7.5) > h)
+
+
+@pytest.fixture(scope="module")
+def trimers():
+    ts = [b for b in trimer_set(96, seed=44) if away_from_cutoffs(b)]
+    return ts[:64]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_phi_linear_in_coefficients(params_fs2fs3, trimers, precision):
+    """sum_n phi[b, n] c_n equals the FS3 energy of every box."""
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    print(f"phi linearity: {len(trimers)} trimers, {precision}", flush=True)
+    tb = ThreeBody(params_fs2fs3, precision)
+    com, site, ms, bl = pack(trimers)
+    dev = torch.device("cuda:0")
+    com_t = torch.tensor(com.T.copy(), device=dev)
+    site_t = torch.tensor(site.transpose(2, 0, 1).copy(), device=dev)
+    _, _, _, en, phi = tb.eval_batch_grad_device(ms, bl, com_t, site_t)
+    c = torch.tensor(coef_vector(params_fs2fs3), device=dev)
+    e3 = en[:, 1]
+    lin = phi @ c
+    tol = 1e-10 if precision == "fp64" else 1e-4
+    scale = (phi.abs() @ c.abs())
+    assert torch.all((lin - e3).abs() <= tol * scale + 1e-12), (lin - e3).abs().max().item()
+    # the plain batch path gives the same energies
+    _, _, _, en2 = tb.eval_batch_device(ms, bl, com_t, site_t)
+    assert torch.allclose(en2, en, rtol=1e-12, atol=1e-13)
+
+
+def test_phi_matches_oracle_one_hot(params_fs2fs3, oracle):
+    """phi[b, n] = qfs3 of the oracle with the coefficient vector e_n (SR and LR)."""
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    from helpers import System, run_oracle
+    ts = [b for b in trimer_set(400, seed=7) if away_from_cutoffs(b)]
+    # one trimer per region: SR, switch band, LR (R_avg from the COM triangle)
+    def ravg(b):
+        return np.linalg.norm(b.com[:, None] - b.com[None], axis=-1)[np.triu_indices(3, 1)].mean()
+    def dmax(b):
+        return np.linalg.norm(b.com[:, None] - b.com[None], axis=-1).max()
+    pick = [next(b for b in ts if ravg(b) < 4.4), next(b for b in ts if 4.6 < ravg(b) < 5.4),
+            next(b for b in ts if 5.6 < ravg(b) < 6.0 and dmax(b) < 6.25)]
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    com, site, ms, bl = pack(pick)
+    dev = torch.device("cuda:0")
+    _, _, _, en, phi = tb.eval_batch_grad_device(ms, bl, torch.tensor(com.T.copy(), device=dev),
+                                                 torch.tensor(site.transpose(2, 0, 1).copy(), device=dev))
+    phi = phi.cpu().numpy()
+    p = params_fs2fs3.replace(s2_term=False)
+    for bi, b in enumerate(pick):
+        s = System(b.com, b.site, 0.0, ld=28)
+        for col in (0, 5, 130, 363, NLIN + 0, NLIN + 77, NLIN + 363):
+            onehot = np.zeros(2 * NLIN)
+            onehot[col] = 1.0
+            q = p.replace(sr=p.sr.__class__(p.sr.beta, p.sr.gamma, p.sr.r0, onehot[:NLIN]),
+                          lr=p.lr.__class__(p.lr.beta, p.lr.gamma, p.lr.r0, onehot[NLIN:]))
+            ref = run_oracle(oracle, q, s)["qfs3"]
+            assert abs(phi[bi, col] - ref) <= 1e-10 * abs(ref) + 1e-15, (bi, col, phi[bi, col], ref)
+
+
+def rotate_quat(q, axis, ang):
+    """q' such that R(q') = R(axis, ang) R(q) (lab-frame rotation)."""
+    axis = axis / np.linalg.norm(axis)
+    r = np.concatenate([[np.cos(ang / 2)], np.sin(ang / 2) * axis])
+    w1, v1 = r[0], r[1:]
+    w2, v2 = q[0], q[1:]
+    return np.concatenate([[w1 * w2 - v1 @ v2], w1 * v2 + w2 * v1 + np.cross(v1, v2)])
+
+
+def test_autograd_matches_finite_differences(params_fs2fs3, trimers):
+    """dE/d(COM), dE/d(orientation) and dE/dc through torch.autograd against
+    central differences of the GPU FP64 energy."""
+    from ccpol23_plus_2025_mbmd_amd.autograd import threebody_energy
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    dev = torch.device("cuda:0")
+    sel = trimers[:12]
+    com0 = np.concatenate([b.com for b in sel])
+    q0 = np.concatenate([b.quats for b in sel])
+    ms = np.cumsum([0] + [3] * len(sel)).astype(np.int32)
+    bl = np.zeros(len(sel))
+    pc = torch.tensor(principal_coords()[1], device=dev)
+    c0 = torch.tensor(coef_vector(params_fs2fs3), device=dev)
+    print(f"autograd FD: {len(sel)} trimers", flush=True)
+
+    def E(com, quat, coef):
+        return threebody_energy(tb, com, quat, coef, pc, ms, bl)
+
+    com = torch.tensor(com0, device=dev, requires_grad=True)
+    quat = torch.tensor(q0, device=dev, requires_grad=True)
+    coef = c0.clone().requires_grad_(True)
+    e = E(com, quat, coef)
+    w = torch.linspace(0.5, 1.5, len(sel), device=dev, dtype=torch.float64)
+    (w * e).sum().backward()
+    g_com, g_quat, g_coef = com.grad.cpu().numpy(), quat.grad.cpu().numpy(), coef.grad.cpu().numpy()
+
+    def L(comn, quatn, coefn=c0):
+        with torch.no_grad():
+            return float((w * E(torch.tensor(comn, device=dev), torch.tensor(quatn, device=dev), coefn)).sum())
+
+    h = 1e-5
+    scale = np.abs(g_com).max()
+    for m in (0, 4, 11, 20, 33):
+        for a in range(3):
+            cp, cm = com0.copy(), com0.copy()
+            cp[m, a] += h
+            cm[m, a] -= h
+            fd = (L(cp, q0) - L(cm, q0)) / (2 * h)
+            assert abs(fd - g_com[m, a]) <= 1e-5 * scale + 1e-8, (m, a, fd, g_com[m, a])
+    # orientation: rotation by +-h about a lab axis; dE/dtheta = sum over sites of (e x s) . dE/ds
+    for m in (1, 7, 30):
+        for a in range(3):
+            e_ax = np.eye(3)[a]
+            qp, qm = q0.copy(), q0.copy()
+            qp[m] = rotate_quat(q0[m], e_ax, h)
+            qm[m] = rotate_quat(q0[m], e_ax, -h)
+            fd = (L(com0, qp) - L(com0, qm)) / (2 * h)
+            # chain rule through the quaternion: d q / d theta for the lab rotation
+            dq = (rotate_quat(q0[m], e_ax, 1e-7) - rotate_quat(q0[m], e_ax, -1e-7)) / 2e-7
+            an = float(g_quat[m] @ dq)
+            assert abs(fd - an) <= 1e-5 * max(abs(fd), 1e-3) + 1e-8, (m, a, fd, an)
+    # coefficients: E is linear in c, so central differences are exact up to round-off
+    for col in (3, 200, NLIN + 10, NLIN + 300):
+        cp, cm = c0.clone(), c0.clone()
+        cp[col] += 1.0
+        cm[col] -= 1.0
+        fd = (L(com0, q0, cp) - L(com0, q0, cm)) / 2.0
+        assert abs(fd - g_coef[col]) <= 1e-9 * abs(fd) + 1e-13 * max(1.0, abs(L(com0, q0))), (col, fd, g_coef[col])
+
+
+def test_gram_matches_extended_precision(params_fs2fs3):
+    """Matrix-core Gram against a long-double reference (small) and FP64 BLAS (config size)."""
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(3)
+    # columns of very different scales and signs, like SR/LR coefficient gradients
+    a = rng.normal(size=(300, 70)) * np.exp(rng.uniform(-20, 5, size=70))
+    a[:, 7] = 0.0
+    g = tb.gram(torch.tensor(a, device=dev)).cpu().numpy()
+    al = a.astype(np.longdouble)
+    ref = (al.T @ al).astype(np.float64)
+    # the digit decomposition is exact to 2^-53 of the column maxima per row
+    m = np.abs(a).max(0)
+    bound = 16 * 2.0 ** -53 * a.shape[0] * np.outer(m, m)
+    assert np.all(np.abs(g - ref) <= bound), np.max(np.abs(g - ref) / (bound + 1e-300))
+    assert np.array_equal(g, g.T)
+    # the config-4 shape: 4096 x 728
+    b = rng.normal(size=(4096, 728)) * np.exp(rng.uniform(-10, 3, size=728))
+    bt = torch.tensor(b, device=dev)
+    g = tb.gram(bt)
+    ref = bt.T @ bt
+    m = bt.abs().max(0).values
+    bound = 1e-12 * (bt.abs().T @ bt.abs()) + 16 * 2.0 ** -53 * 4096 * torch.outer(m, m)
+    err = ((g - ref).abs() / bound).max().item()
+    assert err <= 1.0, err
