@@ -261,7 +261,12 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-// The evaluation proper, on device arrays already in place.
+// The evaluation proper, on device arrays already in place.  Enqueues the whole
+// call on `st` without waiting: grids are sized by the context's capacities
+// and every kernel reads the actual list sizes from the device.  A list that
+// outgrows its capacity is flagged on the device; the kernels that write the
+// caller's outputs then write nothing, and run_checked() grows the capacity and
+// evaluates again.  Ends with an asynchronous copy of the counters to the host.
 int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double* y, const double* z,
         const double* xs, const double* ys, const double* zs, double* fxs, double* fys, double* fzs,
         double* fcom2, double* fcom3, double* energies, hipStream_t st, double* phi = nullptr) {
@@ -269,6 +274,12 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   int rc;
   const int do3 = c->s3, do2 = c->s2;
   if ((rc = ensure_molecules(c, nm, nbox))) return rc;
+  // triplet capacity: the last call's, or a bulk-water estimate
+  if ((rc = ensure_lists(c, std::max<long>(D.cap3, (long)nm * 8), 0))) return rc;
+  if (phi && D.cap3 > c->cap_phi) {
+    if ((rc = dev_alloc(c, c->d_phitrip, (size_t)D.cap3 * 2 * NLIN3))) return rc;
+    c->cap_phi = D.cap3;
+  }
   D.nm = nm;
   D.nbox = nbox;
   HIPCHK(c, hipMemcpyAsync(D.mol_box, c->h_molbox.data(), sizeof(int) * nm, hipMemcpyHostToDevice, st));
@@ -276,65 +287,37 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   HIPCHK(c, hipMemcpyAsync(D.box_start, c->h_boxstart.data(), sizeof(int) * (nbox + 1),
                            hipMemcpyHostToDevice, st));
   HIPCHK(c, hipEventRecord(c->ev[0], st));
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    HIPCHK(c, hipMemsetAsync(D.totals, 0, T_COUNT * sizeof(int), st));
-    const long nprep = (long)nm * ACC_W;
-    hipLaunchKernelGGL(k_prep, dim3((nprep + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
-    hipLaunchKernelGGL(k_neighbours, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
-    LAUNCHCHK(c, "neighbour lists");
-    // sub-list sweeps, counting; their sizes are bounded by nm * cap_nb
-    const long max_sl = std::min<long>((long)nm * D.cap_nb, (long)c->cu_count * 32);
-    if (do3) hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 0);
+  const long nprep = std::max<long>((long)nm * ACC_W, T_COUNT);
+  hipLaunchKernelGGL(k_prep, dim3((nprep + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
+  hipLaunchKernelGGL(k_neighbours, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
+  LAUNCHCHK(c, "neighbour lists");
+  // sub-list sweeps (grid-stride over the sub-lists, count then fill)
+  const long max_sl = std::min<long>((long)nm * D.cap_nb, (long)c->cu_count * 32);
+  if (do3) {
+    hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 0);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 1, do3, do2);
-    LAUNCHCHK(c, "list counts");
-    HIPCHK(c, hipMemcpyAsync(c->h_totals, D.totals, T_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    if (c->h_totals[T_NBOVER] > 0) {
-      // a neighbour list overflowed: grow its capacity and rebuild once
-      if (attempt == 1 || c->h_totals[T_NBOVER] > 65536)
-        return fail(c, CCPOL3B_ECAPACITY, "neighbour list capacity exceeded");
-      D.cap_nb = ((c->h_totals[T_NBOVER] + 63) / 64) * 64 + 64;
-      c->cap_nm = 0;  // force reallocation with the new capacity
-      free_dev(c);
-      c->D.cap3 = c->D.cap_items2 = 0;
-      c->cap_box = 0;
-      if ((rc = ensure_molecules(c, nm, nbox))) return rc;
-      HIPCHK(c, hipMemcpyAsync(D.mol_box, c->h_molbox.data(), sizeof(int) * nm, hipMemcpyHostToDevice, st));
-      HIPCHK(c, hipMemcpyAsync(D.boxL, c->h_boxL.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
-      HIPCHK(c, hipMemcpyAsync(D.box_start, c->h_boxstart.data(), sizeof(int) * (nbox + 1),
-                               hipMemcpyHostToDevice, st));
-      continue;
-    }
-    break;
+    hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 1);
   }
-  const long n3 = do3 ? c->h_totals[T_N3] : 0;
-  const long np2 = do2 ? c->h_totals[T_NP2] : 0;
-  const long nsl3 = c->h_totals[T_SL3];
-  if ((rc = ensure_lists(c, n3, 0))) return rc;
-  if (do3 && nsl3 > 0) hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(nsl3)), dim3(256), 0, st, D, c->K, 1);
   hipLaunchKernelGGL(k_box_ranges, dim3(1), dim3(64), 0, st, D, do3, do2);
-  LAUNCHCHK(c, "list fill");
+  LAUNCHCHK(c, "lists");
   HIPCHK(c, hipEventRecord(c->ev[1], st));
   if (do2) {
     // pair pass (S_ij, pair-owned forces) before the spectator pass that reads S_ij
     const size_t lds = fs2_lds_bytes(D.cap_nb);
     if (lds > 65536) return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the spectator pass");
-    if (np2 > 0) hipLaunchKernelGGL(k_fs2_pair_pass, dim3(waves_grid(np2)), dim3(256), 0, st, D, c->K);
+    const long pair_waves = std::min<long>((long)nm * D.cap_nb / 2 + 1, (long)c->cu_count * 64);
+    hipLaunchKernelGGL(k_fs2_pair_pass, dim3(waves_grid(pair_waves)), dim3(256), 0, st, D, c->K);
     hipLaunchKernelGGL(k_fs2_spectator_pass, dim3(nm * FS2_PARTS), dim3(256), lds, st, D, c->K);
     LAUNCHCHK(c, "FS2");
   }
   HIPCHK(c, hipEventRecord(c->ev[2], st));
-  if (do3 && n3 > 0) {
-    hipLaunchKernelGGL(k_fs3_geo, dim3((n3 + 255) / 256), dim3(256), 0, st, D, c->K);
+  const long cap3 = D.cap3;
+  if (do3) {
+    hipLaunchKernelGGL(k_fs3_geo, dim3((cap3 + 255) / 256), dim3(256), 0, st, D, c->K);
     HIPCHK(c, hipEventRecord(c->ev[8], st));
-    const int grid = (int)n3;  // one block per triplet
+    const int grid = (int)cap3;  // one block per triplet; blocks past the list exit at once
     if (phi) {
-      if (n3 > c->cap_phi) {
-        const long n = n3 + n3 / 8;
-        if ((rc = dev_alloc(c, c->d_phitrip, (size_t)n * 2 * NLIN3))) return rc;
-        c->cap_phi = n;
-      }
       if (c->precision == CCPOL3B_MIXED)
         hipLaunchKernelGGL(k_fs3_phi_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
                            (const float*)c->d_coef32, c->d_phitrip);
@@ -349,6 +332,8 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
                          (const double*)c->d_coef64);
     }
     LAUNCHCHK(c, "k_fs3_eval");
+  } else {
+    HIPCHK(c, hipEventRecord(c->ev[8], st));
   }
   if (phi) {
     hipLaunchKernelGGL(k_phi_boxes, dim3((2 * NLIN3 + 255) / 256, nbox), dim3(256), 0, st, D,
@@ -359,20 +344,51 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   const long nfin = (long)nm * ACC_W;
   hipLaunchKernelGGL(k_finish, dim3((nfin + 255) / 256), dim3(256), 0, st, D, ld, do3 ? NS3 : (do2 ? NS2 : 0),
                      do2, do3, fxs, fys, fzs, fcom2, fcom3);
-  const long nch3 = n3 / RED_CHUNK + nbox + 1, nch2 = (long)nm * FS2_PARTS / RED_CHUNK + nbox + 1;
+  const long nch3 = cap3 / RED_CHUNK + nbox + 1, nch2 = (long)nm * FS2_PARTS / RED_CHUNK + nbox + 1;
   if (do3) hipLaunchKernelGGL(k_energy_chunks, dim3(nch3), dim3(256), 0, st, D, 3, nch3);
   if (do2) hipLaunchKernelGGL(k_energy_chunks, dim3(nch2), dim3(256), 0, st, D, 2, nch2);
   hipLaunchKernelGGL(k_energy_boxes, dim3(nbox), dim3(64), 0, st, D, do2, do3, energies);
   LAUNCHCHK(c, "finish");
   HIPCHK(c, hipEventRecord(c->ev[4], st));
   HIPCHK(c, hipMemcpyAsync(c->h_totals + T_COUNT, D.totals, T_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
-  c->stats.n_fs3 = n3;
-  c->stats.n_fs2_pairs = np2;
   return CCPOL3B_OK;
+}
+
+// run() until no list outgrew its capacity (at most twice), waiting for the
+// stream at the end of each attempt.
+int run_checked(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double* y, const double* z,
+                const double* xs, const double* ys, const double* zs, double* fxs, double* fys, double* fzs,
+                double* fcom2, double* fcom3, double* energies, hipStream_t st, double* phi = nullptr) {
+  Dev& D = c->D;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    int rc = run(c, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st, phi);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(st));
+    const int* tot = c->h_totals + T_COUNT;
+    if (tot[T_NBOVER] > 0) {
+      // a neighbour list overflowed: grow its capacity (all per-molecule arrays)
+      if (tot[T_NBOVER] > 65536) return fail(c, CCPOL3B_ECAPACITY, "neighbour list capacity exceeded");
+      D.cap_nb = ((tot[T_NBOVER] + 63) / 64) * 64 + 64;
+      c->cap_nm = 0;
+      c->cap_box = 0;
+      free_dev(c);
+      D.cap3 = D.cap_items2 = 0;
+      continue;
+    }
+    if (tot[T_N3] > D.cap3) {
+      int rc2 = ensure_lists(c, tot[T_N3], 0);
+      if (rc2) return rc2;
+      continue;
+    }
+    return CCPOL3B_OK;
+  }
+  return fail(c, CCPOL3B_ECAPACITY, "list capacity could not be established");
 }
 
 void finish_stats(ccpol3b_ctx* c) {
   const int* tot = c->h_totals + T_COUNT;
+  c->stats.n_fs3 = c->s3 ? tot[T_N3] : 0;
+  c->stats.n_fs2_pairs = c->s2 ? tot[T_NP2] : 0;
   c->stats.n_fs3_sr = tot[T_HIST];
   c->stats.n_fs3_switch = tot[T_HIST + 1];
   c->stats.n_fs3_lr = tot[T_HIST + 2];
@@ -603,9 +619,8 @@ int ccpol3b_eval_device(ccpol3b_ctx* c, int nm, int ld, const double* x, const d
   hipStream_t st = (hipStream_t)stream;
   set_single_box(c, nm, box);
   if (nm == 0) return CCPOL3B_OK;
-  int rc = run(c, nm, 1, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st);
+  int rc = run_checked(c, nm, 1, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st);
   if (rc) return rc;
-  HIPCHK(c, hipStreamSynchronize(st));
   finish_stats(c);
   return CCPOL3B_OK;
 }
@@ -684,9 +699,8 @@ int eval_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box
     for (int m = mol_start[b]; m < mol_start[b + 1]; ++m) c->h_molbox[m] = b;
   }
   if (nm == 0) return CCPOL3B_OK;
-  int rc = run(c, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st, phi);
+  int rc = run_checked(c, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st, phi);
   if (rc) return rc;
-  HIPCHK(c, hipStreamSynchronize(st));
   finish_stats(c);
   return CCPOL3B_OK;
 }
@@ -735,7 +749,7 @@ int ccpol3b_threebody_term(ccpol3b_ctx* c, int nm, int ld, const double* x, cons
   double* din = c->d_in;
   double* dout = c->d_out;
   set_single_box(c, nm, box);
-  rc = run(c, nm, 1, ldd, din, din + nm, din + 2 * nm, din + 3 * nm, din + 3 * nm + (size_t)ldd * nm,
+  rc = run_checked(c, nm, 1, ldd, din, din + nm, din + 2 * nm, din + 3 * nm, din + 3 * nm + (size_t)ldd * nm,
            din + 3 * nm + 2 * (size_t)ldd * nm, dout, dout + (size_t)ldd * nm, dout + 2 * (size_t)ldd * nm,
            dout + 3 * (size_t)ldd * nm, dout + 3 * (size_t)ldd * nm + 3 * nm, dout + 3 * (size_t)ldd * nm + 6 * nm,
            st);
@@ -815,7 +829,7 @@ long ccpol3b_get_fs2_list(ccpol3b_ctx* c, int* kij, long cap) {
   Dev& D = c->D;
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
   const hipStream_t st = nullptr;
-  const long nsl2 = c->h_totals[T_SL2];
+  const long nsl2 = c->h_totals[T_COUNT + T_SL2];
   if (nsl2 > 0) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nsl2)), dim3(256), 0, st, D, c->K, 0);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 2, 0, 1);
   LAUNCHCHK(c, "FS2 list count");

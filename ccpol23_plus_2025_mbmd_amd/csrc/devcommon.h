@@ -26,6 +26,12 @@ __device__ __forceinline__ double sq3_nc(double x, double y, double z) {
   return x * x + y * y + z * z;
 }
 
+// A list outgrew its capacity in this call: the caller's outputs are not
+// written and the call is repeated with larger capacities.
+__device__ __forceinline__ bool lists_overflowed(const Dev& D) {
+  return D.totals[T_NBOVER] > 0 || D.totals[T_N3] > D.cap3;
+}
+
 // owner of flat index s in an exclusive-offset array off[0..n]
 __device__ __forceinline__ int owner_of(const int* off, int n, long s) {
   int lo = 0, hi = n;  // off[lo] <= s < off[hi]

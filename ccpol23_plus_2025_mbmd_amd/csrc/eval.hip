@@ -134,13 +134,15 @@ __constant__ int c_nz[NTYPE] = {1, 2, 2, 4, 4, 4};
 // without TR the column-wise passes step with stride 17.
 // PHI: also write the triplet's row of dE/dc (kcal/mol per unit coefficient;
 // 364 SR then 364 LR) to phi_trip[t][728].
-template <typename T, bool TR, bool PHI>
+// CSM: stage the region's coefficient table in LDS; otherwise the tile threads
+// read it through the vector cache (less LDS per block, more blocks per CU).
+template <typename T, bool TR, bool PHI, bool CSM>
 __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __restrict__ pairpar,
                                               const T* __restrict__ coef, double* __restrict__ phi_trip) {
   using T2 = typename Pair2<T>::type;
   // sR holds r and then h = (dE/dr)/r.  The mixed variant's r is FP32-exact
   // (FP32 square root), so T storage loses nothing.
-  __shared__ T sC[NCOEF];
+  __shared__ T sC[CSM ? NCOEF : 1];
   __shared__ double sP[3][NS3][3];
   __shared__ T2 sV[(TR ? 2 : 1) * NPAIR3];
   __shared__ T sDer[NPAIR3];
@@ -196,7 +198,8 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
     const int X = reg0 + rr;
     // coefficients of this region (the previous region's readers have passed
     // the barrier that ends the triple stage)
-    for (int q = tid; q < NCOEF; q += 256) sC[q] = coef[X * NCOEF + q];
+    if constexpr (CSM)
+      for (int q = tid; q < NCOEF; q += 256) sC[q] = coef[X * NCOEF + q];
     if constexpr (PHI)
       for (int q = tid; q < NCOEF; q += 256) sDC[q] = T(0);
     if (rr == 0) __syncthreads();  // sP and g
@@ -227,7 +230,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
     if (tile) {
       const T2* ap = sV + aoff;
       const T2* bp = sV + boff;
-      const T* cp = sC + (tx * NTYPE + ty) * NTYPE * 8;
+      const T* cp = (CSM ? sC : coef + X * NCOEF) + (tx * NTYPE + ty) * NTYPE * 8;
       T G[2][2][2];
 #pragma unroll
       for (int xs = 0; xs < 2; ++xs)
@@ -426,7 +429,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
 __global__ void k_finish(Dev D, int ld, int nsite, int do2, int do3, double* fxs, double* fys,
                          double* fzs, double* fcom2, double* fcom3) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)D.nm * ACC_W) return;
+  if (t >= (long)D.nm * ACC_W || lists_overflowed(D)) return;
   const int m = (int)(t / ACC_W), a = (int)(t - (long)m * ACC_W);
   const double conv = H2KCAL * 418.4 / BOHR2A;
   const double v = D.acc[t] * conv;
@@ -448,7 +451,7 @@ __global__ void k_finish(Dev D, int ld, int nsite, int do2, int do3, double* fxs
 __global__ __launch_bounds__(256) void k_energy_chunks(Dev D, int which, long nchunks) {
   __shared__ double red[2][4];
   const long q = blockIdx.x;
-  if (q >= nchunks) return;
+  if (q >= nchunks || lists_overflowed(D) || q >= (which == 3 ? D.box_c3 : D.box_c2)[D.nbox]) return;
   const int* bt = which == 3 ? D.box_t3 : D.box_t2;
   const int* bc = which == 3 ? D.box_c3 : D.box_c2;
   const double* ew = which == 3 ? D.ew3 : D.ew2;
@@ -484,7 +487,7 @@ __global__ __launch_bounds__(256) void k_energy_chunks(Dev D, int which, long nc
 // (10 J/mol)}.  The order depends only on the chunk layout, not on timing.
 __global__ __launch_bounds__(64) void k_energy_boxes(Dev D, int do2, int do3, double* out) {
   const int b = blockIdx.x;
-  if (b >= D.nbox) return;
+  if (b >= D.nbox || lists_overflowed(D)) return;
   const int lane = threadIdx.x;
   double e2 = 0.0, w2 = 0.0, e3 = 0.0, w3 = 0.0;
   if (do2)
@@ -508,26 +511,28 @@ __global__ __launch_bounds__(64) void k_energy_boxes(Dev D, int do2, int do3, do
   }
 }
 
-// LDS per block: FP64 ~43 KB (3 blocks per CU), mixed ~29 KB (5 blocks per CU).
-__global__ __launch_bounds__(256, 3) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
+// Coefficients come through the vector cache; LDS per block is ~29 KB (FP64,
+// 5 blocks per CU, at the cost of a few spilled registers) and ~15 KB (mixed,
+// 7 blocks per CU by registers).
+__global__ __launch_bounds__(256, 5) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const double* __restrict__ coef) {
-  fs3_eval_body<double, false, false>(D, K, pairpar, coef, nullptr);
+  fs3_eval_body<double, false, false, false>(D, K, pairpar, coef, nullptr);
 }
 
-__global__ __launch_bounds__(256, 5) void k_fs3_eval_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
+__global__ __launch_bounds__(256, 7) void k_fs3_eval_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const float* __restrict__ coef) {
-  fs3_eval_body<float, true, false>(D, K, pairpar, coef, nullptr);
+  fs3_eval_body<float, false, false, false>(D, K, pairpar, coef, nullptr);
 }
 
 // The same evaluation plus the per-triplet coefficient-gradient rows.
 __global__ __launch_bounds__(256, 2) void k_fs3_phi_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                         const double* __restrict__ coef, double* phi_trip) {
-  fs3_eval_body<double, false, true>(D, K, pairpar, coef, phi_trip);
+  fs3_eval_body<double, false, true, true>(D, K, pairpar, coef, phi_trip);
 }
 
 __global__ __launch_bounds__(256, 4) void k_fs3_phi_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                         const float* __restrict__ coef, double* phi_trip) {
-  fs3_eval_body<float, true, true>(D, K, pairpar, coef, phi_trip);
+  fs3_eval_body<float, false, true, true>(D, K, pairpar, coef, phi_trip);
 }
 
 // Coefficient gradient of each box: the sum, in triplet order, of its
@@ -535,7 +540,7 @@ __global__ __launch_bounds__(256, 4) void k_fs3_phi_f32(Dev D, Consts K, const d
 __global__ __launch_bounds__(256) void k_phi_boxes(Dev D, const double* __restrict__ phi_trip, double* phi) {
   const int b = blockIdx.y;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= D.nbox || n >= 2 * NLIN3) return;
+  if (b >= D.nbox || n >= 2 * NLIN3 || lists_overflowed(D)) return;
   double a = 0.0;
   for (int t = D.box_t3[b]; t < D.box_t3[b + 1]; ++t) a += phi_trip[(long)t * 2 * NLIN3 + n];
   phi[(long)b * 2 * NLIN3 + n] = a;

@@ -98,14 +98,22 @@ __device__ __forceinline__ double fs2_T(const Fs2Geo& G, const double* __restric
 size_t fs2_lds_bytes(int cap_nb) { return (size_t)cap_nb * (6 * sizeof(double) + 4 * sizeof(int)) + 16; }
 
 // ---------------------------------------------------------------- pair pass
-// One wavefront per pair p = (i, j = nb2(i)[q]), q >= up2(i); lanes run over
+__device__ void fs2_pair(const Dev& D, const Consts& K, long p, int lane);
+
+// Wavefronts loop over the pairs p = (i, j = nb2(i)[q]), q >= up2(i); lanes run over
 // the spectator candidates k = nb2(i)[a], a != q.  Lanes 0..8 also own the
 // nine O,H,H site pairs of S_ij (:361-372) and of the exchange-site forces
 // F = -(sum_k f f T) dS/ds (:477-519).
 __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
   const int lane = threadIdx.x & 63;
-  const long p = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (p >= D.totals[T_NP2]) return;
+  const long np2 = D.totals[T_NP2];
+  const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+  for (long p = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; p < np2; p += nwaves)
+    fs2_pair(D, K, p, lane);
+}
+
+// The pair p: everything it owns (see k_fs2_pair_pass).
+__device__ void fs2_pair(const Dev& D, const Consts& K, long p, int lane) {
   const int i = owner_of(D.poff2, D.nm, p);
   const int n = D.nb2_cnt[i];
   const int jq = D.up2[i] + (int)(p - D.poff2[i]);

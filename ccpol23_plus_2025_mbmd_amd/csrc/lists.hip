@@ -40,7 +40,8 @@ __device__ __forceinline__ int upper_start(const int* list, int n, int v) {
 }
 
 // ------------------------------------------------------------------ prep
-// COM (A) -> bohr and site offsets (A, Fortran order xs(is,i)) -> [m][s][c] bohr.
+// COM (A) -> bohr and site offsets (A, Fortran order xs(is,i)) -> [m][s][c] bohr;
+// zero the force accumulators and the counters of the call.
 __global__ void k_prep(int nm, int ld, const double* __restrict__ x, const double* __restrict__ y,
                        const double* __restrict__ z, const double* __restrict__ xs,
                        const double* __restrict__ ys, const double* __restrict__ zs, Dev D) {
@@ -59,6 +60,7 @@ __global__ void k_prep(int nm, int ld, const double* __restrict__ x, const doubl
     dst[2] = zs[src] / BOHR2A;
   }
   if (t < nm * ACC_W) D.acc[t] = 0.0;
+  if (t < T_COUNT) D.totals[t] = 0;
 }
 
 // ------------------------------------------------------------------ neighbour lists
