@@ -43,6 +43,14 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("fs3_lds_coef", C.c_int), ("fs2_parts", C.c_int), ("fs2_pair_waves_per_cu", C.c_int),
+                ("list_waves_per_cu", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class CcpolError(RuntimeError):
     pass
 
@@ -86,6 +94,22 @@ def load():
     L.ccpol3b_set_coefficients.restype = C.c_int
     L.ccpol3b_gram_device.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp]
     L.ccpol3b_gram_device.restype = C.c_int
+    L.ccpol3b_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.ccpol3b_get_tuning.restype = C.c_int
+    L.ccpol3b_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.ccpol3b_set_tuning.restype = C.c_int
+    L.ccpol3b_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp, C.c_int] + [vp] * 6 + \
+        [C.c_int, vp, C.POINTER(Tuning), dp]
+    L.ccpol3b_autotune.restype = C.c_int
+    L.ccpol3b_graph_create.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp, C.c_int] + [vp] * 13 + \
+        [C.c_double, vp, C.POINTER(C.c_void_p)]
+    L.ccpol3b_graph_create.restype = C.c_int
+    L.ccpol3b_graph_launch.argtypes = [vp, vp]
+    L.ccpol3b_graph_launch.restype = C.c_int
+    L.ccpol3b_graph_status.argtypes = [vp]
+    L.ccpol3b_graph_status.restype = C.c_int
+    L.ccpol3b_graph_destroy.argtypes = [vp]
+    L.ccpol3b_graph_destroy.restype = None
     L.ccpol3b_last_stats.argtypes = [vp, C.POINTER(Stats)]
     L.ccpol3b_last_stats.restype = C.c_int
     L.ccpol3b_get_fs3_list.argtypes = [vp, C.POINTER(C.c_int), C.c_long]
@@ -109,6 +133,8 @@ EXPORTED = [
     "ccpol3b_eval_device", "ccpol3b_eval_batch_device", "ccpol3b_last_stats", "ccpol3b_get_fs3_list",
     "ccpol3b_get_fs2_list", "ccpol3b_read_data3b", "ccpol3b_orbit_index", "ccpol3b_status_string",
     "ccpol3b_last_error", "ccpol3b_eval_batch_grad_device", "ccpol3b_set_coefficients", "ccpol3b_gram_device",
+    "ccpol3b_get_tuning", "ccpol3b_set_tuning", "ccpol3b_autotune", "ccpol3b_graph_create", "ccpol3b_graph_launch",
+    "ccpol3b_graph_status", "ccpol3b_graph_destroy",
 ]
 
 

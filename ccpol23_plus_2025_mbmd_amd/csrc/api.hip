@@ -26,6 +26,8 @@ __global__ void k_box_ranges(Dev, int, int);
 __global__ void k_fs3_geo(Dev, Consts);
 __global__ void k_fs3_eval_f64(Dev, Consts, const double4*, const double*);
 __global__ void k_fs3_eval_f32(Dev, Consts, const double4*, const float*);
+__global__ void k_fs3_eval_f64_lds(Dev, Consts, const double4*, const double*);
+__global__ void k_fs3_eval_f32_lds(Dev, Consts, const double4*, const float*);
 __global__ void k_fs3_phi_f64(Dev, Consts, const double4*, const double*, double*);
 __global__ void k_fs3_phi_f32(Dev, Consts, const double4*, const float*, double*);
 __global__ void k_phi_boxes(Dev, const double*, double*);
@@ -132,6 +134,11 @@ struct ccpol3b_ctx {
   float* d_coef32 = nullptr;
   double* d_phitrip = nullptr;   // [cap_phi][728] per-triplet coefficient-gradient rows
   long cap_phi = 0;
+  ccpol3b_tuning tune{0, 4, 64, 32};
+  // creation parameters (Fortran-ordered 17x17 tables, current coefficients),
+  // kept so a graph can own a private copy of the context
+  std::vector<double> h_tab[6], h_cs[2];
+  double alpha3 = 0.0, delta33 = 0.0, rcut3 = 0.0;
   float* d_gdig = nullptr;       // Gram digits [6][rows][cols]
   int* d_gex = nullptr;          // Gram column exponents
   size_t cap_gdig = 0, cap_gex = 0;
@@ -214,7 +221,7 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
         (rc = dev_alloc(c, D.pcnt2, n)) || (rc = dev_alloc(c, D.poff2, n + 1)) ||
         (rc = dev_alloc(c, D.cnt3s, nn)) || (rc = dev_alloc(c, D.off3s, nn + 1)) ||
         (rc = dev_alloc(c, D.cnt2s, nn)) || (rc = dev_alloc(c, D.off2s, nn + 1)) ||
-        (rc = dev_alloc(c, D.s2pair, 2 * nn)) || (rc = dev_alloc(c, D.ew2, (size_t)n * FS2_PARTS * 2)) ||
+        (rc = dev_alloc(c, D.s2pair, 2 * nn)) || (rc = dev_alloc(c, D.ew2, (size_t)n * FS2_PARTS_MAX * 2)) ||
         (rc = dev_alloc(c, D.acc, (size_t)n * ACC_W)))
       return rc;
     D.cap_sl3 = D.cap_sl2 = D.cap_pairs2 = (long)nn;
@@ -245,7 +252,7 @@ int ensure_lists(ccpol3b_ctx* c, long n3, long ni2) {
     if ((rc = dev_alloc(c, D.items2, n))) return rc;
     D.cap_items2 = n;
   }
-  const long nred = std::max<long>(D.cap3, (long)c->cap_nm * FS2_PARTS) / RED_CHUNK + D.nbox + 2;
+  const long nred = std::max<long>(D.cap3, (long)c->cap_nm * FS2_PARTS_MAX) / RED_CHUNK + D.nbox + 2;
   if (nred > D.cap_red) {
     if ((rc = dev_alloc(c, D.red3, (size_t)nred * 2)) || (rc = dev_alloc(c, D.red2, (size_t)nred * 2))) return rc;
     D.cap_red = nred;
@@ -269,53 +276,62 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
 // evaluates again.  Ends with an asynchronous copy of the counters to the host.
 int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double* y, const double* z,
         const double* xs, const double* ys, const double* zs, double* fxs, double* fys, double* fzs,
-        double* fcom2, double* fcom3, double* energies, hipStream_t st, double* phi = nullptr) {
+        double* fcom2, double* fcom3, double* energies, hipStream_t st, double* phi = nullptr,
+        bool capture = false) {
   Dev& D = c->D;
   int rc;
   const int do3 = c->s3, do2 = c->s2;
+  // during graph capture nothing may allocate, copy from pageable memory or record timing events
+#define EVREC(i) \
+  if (!capture) HIPCHK(c, hipEventRecord(c->ev[i], st))
   if ((rc = ensure_molecules(c, nm, nbox))) return rc;
+  D.nm = nm;
+  D.nbox = nbox;  // before ensure_lists: the reduction arrays are sized by it
   // triplet capacity: the last call's, or a bulk-water estimate
   if ((rc = ensure_lists(c, std::max<long>(D.cap3, (long)nm * 8), 0))) return rc;
   if (phi && D.cap3 > c->cap_phi) {
     if ((rc = dev_alloc(c, c->d_phitrip, (size_t)D.cap3 * 2 * NLIN3))) return rc;
     c->cap_phi = D.cap3;
   }
-  D.nm = nm;
-  D.nbox = nbox;
-  HIPCHK(c, hipMemcpyAsync(D.mol_box, c->h_molbox.data(), sizeof(int) * nm, hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemcpyAsync(D.boxL, c->h_boxL.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemcpyAsync(D.box_start, c->h_boxstart.data(), sizeof(int) * (nbox + 1),
-                           hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipEventRecord(c->ev[0], st));
+  D.fs2_parts = c->tune.fs2_parts;
+  if (!capture) {
+    HIPCHK(c, hipMemcpyAsync(D.mol_box, c->h_molbox.data(), sizeof(int) * nm, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(D.boxL, c->h_boxL.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(D.box_start, c->h_boxstart.data(), sizeof(int) * (nbox + 1),
+                             hipMemcpyHostToDevice, st));
+  }
+  EVREC(0);
   const long nprep = std::max<long>((long)nm * ACC_W, T_COUNT);
   hipLaunchKernelGGL(k_prep, dim3((nprep + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
   hipLaunchKernelGGL(k_neighbours, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
   LAUNCHCHK(c, "neighbour lists");
   // sub-list sweeps (grid-stride over the sub-lists, count then fill)
-  const long max_sl = std::min<long>((long)nm * D.cap_nb, (long)c->cu_count * 32);
+  const long max_sl = std::min<long>((long)nm * D.cap_nb, (long)c->cu_count * c->tune.list_waves_per_cu);
   if (do3) {
     hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 0);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 1, do3, do2);
     hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 1);
   }
-  hipLaunchKernelGGL(k_box_ranges, dim3(1), dim3(64), 0, st, D, do3, do2);
+  hipLaunchKernelGGL(k_box_ranges, dim3((nbox + 256) / 256), dim3(256), 0, st, D, do3, do2);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 3, do3, do2);
   LAUNCHCHK(c, "lists");
-  HIPCHK(c, hipEventRecord(c->ev[1], st));
+  EVREC(1);
   if (do2) {
     // pair pass (S_ij, pair-owned forces) before the spectator pass that reads S_ij
     const size_t lds = fs2_lds_bytes(D.cap_nb);
     if (lds > 65536) return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the spectator pass");
-    const long pair_waves = std::min<long>((long)nm * D.cap_nb / 2 + 1, (long)c->cu_count * 64);
+    const long pair_waves =
+        std::min<long>((long)nm * D.cap_nb / 2 + 1, (long)c->cu_count * c->tune.fs2_pair_waves_per_cu);
     hipLaunchKernelGGL(k_fs2_pair_pass, dim3(waves_grid(pair_waves)), dim3(256), 0, st, D, c->K);
-    hipLaunchKernelGGL(k_fs2_spectator_pass, dim3(nm * FS2_PARTS), dim3(256), lds, st, D, c->K);
+    hipLaunchKernelGGL(k_fs2_spectator_pass, dim3(nm * D.fs2_parts), dim3(256), lds, st, D, c->K);
     LAUNCHCHK(c, "FS2");
   }
-  HIPCHK(c, hipEventRecord(c->ev[2], st));
+  EVREC(2);
   const long cap3 = D.cap3;
   if (do3) {
     hipLaunchKernelGGL(k_fs3_geo, dim3((cap3 + 255) / 256), dim3(256), 0, st, D, c->K);
-    HIPCHK(c, hipEventRecord(c->ev[8], st));
+    EVREC(8);
     const int grid = (int)cap3;  // one block per triplet; blocks past the list exit at once
     if (phi) {
       if (c->precision == CCPOL3B_MIXED)
@@ -325,33 +341,42 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
         hipLaunchKernelGGL(k_fs3_phi_f64, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
                            (const double*)c->d_coef64, c->d_phitrip);
     } else if (c->precision == CCPOL3B_MIXED) {
-      hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
-                         (const float*)c->d_coef32);
+      if (c->tune.fs3_lds_coef)
+        hipLaunchKernelGGL(k_fs3_eval_f32_lds, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+                           (const float*)c->d_coef32);
+      else
+        hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+                           (const float*)c->d_coef32);
     } else {
-      hipLaunchKernelGGL(k_fs3_eval_f64, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
-                         (const double*)c->d_coef64);
+      if (c->tune.fs3_lds_coef)
+        hipLaunchKernelGGL(k_fs3_eval_f64_lds, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+                           (const double*)c->d_coef64);
+      else
+        hipLaunchKernelGGL(k_fs3_eval_f64, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+                           (const double*)c->d_coef64);
     }
     LAUNCHCHK(c, "k_fs3_eval");
   } else {
-    HIPCHK(c, hipEventRecord(c->ev[8], st));
+    EVREC(8);
   }
   if (phi) {
     hipLaunchKernelGGL(k_phi_boxes, dim3((2 * NLIN3 + 255) / 256, nbox), dim3(256), 0, st, D,
                        (const double*)c->d_phitrip, phi);
     LAUNCHCHK(c, "k_phi_boxes");
   }
-  HIPCHK(c, hipEventRecord(c->ev[3], st));
+  EVREC(3);
   const long nfin = (long)nm * ACC_W;
   hipLaunchKernelGGL(k_finish, dim3((nfin + 255) / 256), dim3(256), 0, st, D, ld, do3 ? NS3 : (do2 ? NS2 : 0),
                      do2, do3, fxs, fys, fzs, fcom2, fcom3);
-  const long nch3 = cap3 / RED_CHUNK + nbox + 1, nch2 = (long)nm * FS2_PARTS / RED_CHUNK + nbox + 1;
+  const long nch3 = cap3 / RED_CHUNK + nbox + 1, nch2 = (long)nm * D.fs2_parts / RED_CHUNK + nbox + 1;
   if (do3) hipLaunchKernelGGL(k_energy_chunks, dim3(nch3), dim3(256), 0, st, D, 3, nch3);
   if (do2) hipLaunchKernelGGL(k_energy_chunks, dim3(nch2), dim3(256), 0, st, D, 2, nch2);
   hipLaunchKernelGGL(k_energy_boxes, dim3(nbox), dim3(64), 0, st, D, do2, do3, energies);
   LAUNCHCHK(c, "finish");
-  HIPCHK(c, hipEventRecord(c->ev[4], st));
+  EVREC(4);
   HIPCHK(c, hipMemcpyAsync(c->h_totals + T_COUNT, D.totals, T_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
   return CCPOL3B_OK;
+#undef EVREC
 }
 
 // run() until no list outgrew its capacity (at most twice), waiting for the
@@ -411,6 +436,9 @@ int eval_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box
                const double* y, const double* z, const double* xs, const double* ys, const double* zs, double* fxs,
                double* fys, double* fzs, double* fcom2, double* fcom3, double* energies, double* phi,
                void* stream);
+int check_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld);
+void set_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len);
+int clone_ctx(const ccpol3b_ctx* c, ccpol3b_ctx** out);
 
 // Dense coefficient tables C[X][tx][ty][tz][p][q][s] = cs_X[orbit], FP64 and FP32.
 int upload_coefficients(ccpol3b_ctx* c, const double* cs_sr, const double* cs_lr) {
@@ -525,6 +553,19 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
   c->s2 = p->s2_term ? 1 : 0;
   c->s3 = p->s3_term ? 1 : 0;
   c->cu_count = prop.multiProcessorCount;
+  {
+    const double* tabs[6] = {p->beta_sr, p->gamma_sr, p->r0_sr, p->beta_lr, p->gamma_lr, p->r0_lr};
+    for (int k = 0; k < 6; ++k) {
+      c->h_tab[k].resize(NPAIR);
+      for (int is = 0; is < NS3; ++is)
+        for (int js = 0; js < NS3; ++js) c->h_tab[k][is + NS3 * js] = tabs[k][is + (size_t)p->ld_pair * js];
+    }
+    c->h_cs[0].assign(p->cs_sr, p->cs_sr + NLIN3);
+    c->h_cs[1].assign(p->cs_lr, p->cs_lr + NLIN3);
+    c->alpha3 = p->alpha3;
+    c->delta33 = p->delta33;
+    c->rcut3 = p->rcut3;
+  }
   c->D.cap_nb = 256;
   // constants exactly as the reference forms them
   Consts& K = c->K;
@@ -658,7 +699,72 @@ int ccpol3b_set_coefficients(ccpol3b_ctx* c, const double* cs_sr, const double* 
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
   // earlier evaluations on any stream still read the old tables
   HIPCHK(c, hipDeviceSynchronize());
+  c->h_cs[0].assign(cs_sr, cs_sr + NLIN3);
+  c->h_cs[1].assign(cs_lr, cs_lr + NLIN3);
   return upload_coefficients(c, cs_sr, cs_lr);
+}
+
+int ccpol3b_get_tuning(const ccpol3b_ctx* c, ccpol3b_tuning* t) {
+  if (!c || !t) return CCPOL3B_EINVAL;
+  *t = c->tune;
+  return CCPOL3B_OK;
+}
+
+int ccpol3b_set_tuning(ccpol3b_ctx* c, const ccpol3b_tuning* t) {
+  if (!c || !t || t->fs2_parts < 1 || t->fs2_parts > FS2_PARTS_MAX || t->fs2_pair_waves_per_cu < 1 ||
+      t->fs2_pair_waves_per_cu > 1024 || t->list_waves_per_cu < 1 || t->list_waves_per_cu > 1024)
+    return fail(c, CCPOL3B_EINVAL, "invalid tuning");
+  c->tune = *t;
+  c->tune.fs3_lds_coef = t->fs3_lds_coef ? 1 : 0;
+  return CCPOL3B_OK;
+}
+
+int ccpol3b_autotune(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld,
+                     const double* x, const double* y, const double* z, const double* xs, const double* ys,
+                     const double* zs, int reps, void* stream, ccpol3b_tuning* best, double* best_ms) {
+  int rc = check_batch(c, nbox, mol_start, box_len, ld);
+  if (rc) return rc;
+  if (reps < 1 || !x || !y || !z || !xs || !ys || !zs) return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  const int nm = mol_start[nbox];
+  set_batch(c, nbox, mol_start, box_len);
+  double *fs = nullptr, *c2 = nullptr, *c3 = nullptr, *en = nullptr;
+  if ((rc = dev_alloc(c, fs, (size_t)3 * ld * nm)) || (rc = dev_alloc(c, c2, (size_t)3 * nm)) ||
+      (rc = dev_alloc(c, c3, (size_t)3 * nm)) || (rc = dev_alloc(c, en, (size_t)3 * nbox))) {
+    dev_free(fs); dev_free(c2); dev_free(c3); dev_free(en);
+    return rc;
+  }
+  const ccpol3b_tuning keep = c->tune;
+  ccpol3b_tuning win = keep;
+  double win_ms = 1e300;
+  const int parts[] = {2, 4, 8}, pwaves[] = {16, 64, 256}, lwaves[] = {16, 64};
+  for (int lds = 0; lds < 2 && rc == CCPOL3B_OK; ++lds)
+    for (int pp : parts)
+      for (int pw : pwaves)
+        for (int lw : lwaves) {
+          if (rc) break;
+          c->tune = ccpol3b_tuning{lds, pp, pw, lw};
+          double tot = 0.0;
+          for (int r = 0; r <= reps && rc == CCPOL3B_OK; ++r) {
+            rc = run_checked(c, nm, nbox, ld, x, y, z, xs, ys, zs, fs, fs + (size_t)ld * nm,
+                             fs + 2 * (size_t)ld * nm, c2, c3, en, st);
+            if (rc == CCPOL3B_OK && r > 0) tot += event_ms(c->ev[0], c->ev[4]);
+          }
+          if (rc == CCPOL3B_OK && tot / reps < win_ms) {
+            win_ms = tot / reps;
+            win = c->tune;
+          }
+        }
+  dev_free(fs); dev_free(c2); dev_free(c3); dev_free(en);
+  if (rc) {
+    c->tune = keep;
+    return rc;
+  }
+  c->tune = win;
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
+  return CCPOL3B_OK;
 }
 
 int ccpol3b_eval_batch_grad_device(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld,
@@ -684,13 +790,30 @@ int eval_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box
                const double* y, const double* z, const double* xs, const double* ys, const double* zs, double* fxs,
                double* fys, double* fzs, double* fcom2, double* fcom3, double* energies, double* phi,
                void* stream) {
-  if (!c || nbox <= 0 || !mol_start || !box_len || ld < 17) return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  int rc0 = check_batch(c, nbox, mol_start, box_len, ld);
+  if (rc0) return rc0;
   const int nm = mol_start[nbox];
-  if (mol_start[0] != 0 || nm < 0) return fail(c, CCPOL3B_EINVAL, "mol_start must start at 0");
-  for (int b = 0; b < nbox; ++b)
-    if (mol_start[b + 1] < mol_start[b]) return fail(c, CCPOL3B_EINVAL, "mol_start not ascending");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
   hipStream_t st = (hipStream_t)stream;
+  set_batch(c, nbox, mol_start, box_len);
+  if (nm == 0) return CCPOL3B_OK;
+  int rc = run_checked(c, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st, phi);
+  if (rc) return rc;
+  finish_stats(c);
+  return CCPOL3B_OK;
+}
+
+// Validated batch description -> host copies of the box map and lengths.
+int check_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld) {
+  if (!c || nbox <= 0 || !mol_start || !box_len || ld < 17) return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  if (mol_start[0] != 0 || mol_start[nbox] < 0) return fail(c, CCPOL3B_EINVAL, "mol_start must start at 0");
+  for (int b = 0; b < nbox; ++b)
+    if (mol_start[b + 1] < mol_start[b]) return fail(c, CCPOL3B_EINVAL, "mol_start not ascending");
+  return CCPOL3B_OK;
+}
+
+void set_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len) {
+  const int nm = mol_start[nbox];
   c->h_molbox.resize(nm);
   c->h_boxstart.assign(mol_start, mol_start + nbox + 1);
   c->h_boxL.resize(nbox);
@@ -698,11 +821,31 @@ int eval_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box
     c->h_boxL[b] = box_len[b] > 0.0 ? box_len[b] / BOHR2A : 0.0;
     for (int m = mol_start[b]; m < mol_start[b + 1]; ++m) c->h_molbox[m] = b;
   }
-  if (nm == 0) return CCPOL3B_OK;
-  int rc = run_checked(c, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st, phi);
-  if (rc) return rc;
-  finish_stats(c);
-  return CCPOL3B_OK;
+}
+
+// A new context with the same parameters, precision, switches and tuning.
+int clone_ctx(const ccpol3b_ctx* c, ccpol3b_ctx** out) {
+  ccpol3b_params p{};
+  p.beta_sr = c->h_tab[0].data();
+  p.gamma_sr = c->h_tab[1].data();
+  p.r0_sr = c->h_tab[2].data();
+  p.beta_lr = c->h_tab[3].data();
+  p.gamma_lr = c->h_tab[4].data();
+  p.r0_lr = c->h_tab[5].data();
+  p.ld_pair = 17;
+  p.cs_sr = c->h_cs[0].data();
+  p.cs_lr = c->h_cs[1].data();
+  p.nlin = CCPOL3B_NLIN;
+  p.alpha3 = c->alpha3;
+  p.delta33 = c->delta33;
+  p.rcut3 = c->rcut3;
+  p.s2_term = c->s2;
+  p.s3_term = c->s3;
+  p.precision = c->precision;
+  p.device = c->device;
+  int rc = ccpol3b_create(&p, out);
+  if (rc == CCPOL3B_OK) (*out)->tune = c->tune;
+  return rc;
 }
 }  // namespace
 
@@ -852,6 +995,93 @@ long ccpol3b_get_fs2_list(ccpol3b_ctx* c, int* kij, long cap) {
     HIPCHK(c, hipDeviceSynchronize());
   }
   return ni;
+}
+
+struct ccpol3b_graph {
+  ccpol3b_ctx* c = nullptr;  // private context: the recorded kernels hold its device pointers
+  hipStream_t st = nullptr;  // capture stream
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+};
+
+void ccpol3b_graph_destroy(ccpol3b_graph* g) {
+  if (!g) return;
+  if (g->c) (void)hipSetDevice(g->c->device);
+  if (g->x) (void)hipGraphExecDestroy(g->x);
+  if (g->g) (void)hipGraphDestroy(g->g);
+  if (g->st) (void)hipStreamDestroy(g->st);
+  if (g->c) ccpol3b_destroy(g->c);
+  delete g;
+}
+
+int ccpol3b_graph_create(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld,
+                         const double* x, const double* y, const double* z, const double* xs, const double* ys,
+                         const double* zs, double* fxs, double* fys, double* fzs, double* fcom2, double* fcom3,
+                         double* energies, double* phi, double headroom, void* stream, ccpol3b_graph** out) {
+  if (!out) return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  *out = nullptr;
+  int rc = check_batch(c, nbox, mol_start, box_len, ld);
+  if (rc) return rc;
+  const int nm = mol_start[nbox];
+  if (nm <= 0 || !x || !y || !z || !xs || !ys || !zs || !fxs || !fys || !fzs || !fcom2 || !fcom3 || !energies ||
+      !(headroom >= 0.0 && headroom <= 16.0))
+    return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
+  ccpol3b_graph* g = new ccpol3b_graph();
+  if ((rc = clone_ctx(c, &g->c))) {
+    delete g;
+    return fail(c, rc, "graph context");
+  }
+  ccpol3b_ctx* gc = g->c;
+  hipStream_t st = (hipStream_t)stream;
+  // one checked evaluation sizes every list; its outputs are valid
+  set_batch(gc, nbox, mol_start, box_len);
+  rc = run_checked(gc, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st, phi);
+  if (rc == CCPOL3B_OK) {
+    finish_stats(gc);
+    c->stats = gc->stats;
+    const long want = (long)std::ceil((double)std::max<long>(gc->D.cap3, gc->stats.n_fs3) * (1.0 + headroom));
+    rc = ensure_lists(gc, want, 0);
+    if (rc == CCPOL3B_OK && phi && gc->D.cap3 > gc->cap_phi) {
+      rc = dev_alloc(gc, gc->d_phitrip, (size_t)gc->D.cap3 * 2 * NLIN3);
+      if (rc == CCPOL3B_OK) gc->cap_phi = gc->D.cap3;
+    }
+  }
+  if (rc == CCPOL3B_OK && hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess) rc = CCPOL3B_EHIP;
+  if (rc == CCPOL3B_OK && hipStreamSynchronize(st) != hipSuccess) rc = CCPOL3B_EHIP;
+  if (rc == CCPOL3B_OK) {
+    if (hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      rc = CCPOL3B_EHIP;
+    } else {
+      rc = run(gc, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, g->st, phi, true);
+      hipGraph_t gr = nullptr;
+      if (hipStreamEndCapture(g->st, &gr) != hipSuccess && rc == CCPOL3B_OK) rc = CCPOL3B_EHIP;
+      g->g = gr;
+      if (rc == CCPOL3B_OK && (!gr || hipGraphInstantiate(&g->x, gr, nullptr, nullptr, 0) != hipSuccess))
+        rc = CCPOL3B_EHIP;
+    }
+  }
+  if (rc) {
+    std::string msg = gc->err.empty() ? "graph capture failed" : gc->err;
+    ccpol3b_graph_destroy(g);
+    return fail(c, rc, msg);
+  }
+  *out = g;
+  return CCPOL3B_OK;
+}
+
+int ccpol3b_graph_launch(ccpol3b_graph* g, void* stream) {
+  if (!g || !g->x) return CCPOL3B_EINVAL;
+  if (hipSetDevice(g->c->device) != hipSuccess) return CCPOL3B_EHIP;
+  if (hipGraphLaunch(g->x, (hipStream_t)stream) != hipSuccess) return fail(g->c, CCPOL3B_EHIP, "hipGraphLaunch");
+  return CCPOL3B_OK;
+}
+
+int ccpol3b_graph_status(const ccpol3b_graph* g) {
+  if (!g || !g->c) return CCPOL3B_EINVAL;
+  const int* tot = g->c->h_totals + T_COUNT;
+  if (tot[T_NBOVER] > 0 || tot[T_N3] > g->c->D.cap3) return CCPOL3B_ECAPACITY;
+  return CCPOL3B_OK;
 }
 
 }  // extern "C"

@@ -26,8 +26,10 @@ constexpr int T_HIST = 4;    // [4..6] FS3 region histogram SR, switch, LR
 constexpr int T_SL3 = 7;     // FS3 sub-lists (i; j)
 constexpr int T_SL2 = 8;     // FS2 sub-lists (k; i)
 constexpr int T_NI2L = 9;    // FS2 items of an exported list
+constexpr int T_NCH3 = 10;   // FS3 / FS2 energy reduction chunks
+constexpr int T_NCH2 = 11;
 constexpr int T_COUNT = 16;
-constexpr int FS2_PARTS = 4;     // spectator-pass blocks per molecule
+constexpr int FS2_PARTS_MAX = 8; // spectator-pass blocks per molecule, at most
 constexpr int RED_CHUNK = 2048;  // entries per block in the first level of the energy reduction
 
 constexpr double BOHR2A = 0.529177249;  // common_arrays.f90:34
@@ -64,6 +66,7 @@ struct TriGeo {
 // Device arrays of one evaluation (sized by the context; see api.hip).
 struct Dev {
   int nm, nbox, cap_nb;
+  int fs2_parts;              // spectator-pass blocks per molecule (1..FS2_PARTS_MAX)
   long cap3, cap_pairs2, cap_items2, cap_sl3, cap_sl2;
   // geometry in bohr
   double *cx, *cy, *cz;   // [nm]
@@ -89,7 +92,7 @@ struct Dev {
   // results (hartree, bohr)
   double *acc;                // [nm][ACC_W] force accumulators
   double *ew3;                // [cap3][2] per-triplet energy, virial
-  double *ew2;                // [nm][FS2_PARTS][2] FS2 energy, virial per spectator-pass block
+  double *ew2;                // [nm][fs2_parts][2] FS2 energy, virial per spectator-pass block
   int *box_t3, *box_t2;       // [nbox+1] triplet / FS2 energy-entry ranges of each box
   int *box_c3, *box_c2;       // [nbox+1] first reduction chunk of each box
   double *red3, *red2;        // [chunks][2] first-level energy/virial partial sums

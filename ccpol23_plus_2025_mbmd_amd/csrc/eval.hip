@@ -80,7 +80,8 @@ __device__ void fs3_geometry(const Dev& D, const Consts& K, long t, TriGeo& g) {
 
 __global__ __launch_bounds__(256) void k_fs3_geo(Dev D, Consts K) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long n3 = D.totals[T_N3];
+  // the list holds min(n3, cap3) triplets (an overflowing call is repeated)
+  const long n3 = min((long)D.totals[T_N3], D.cap3);
   if (t - (threadIdx.x & 63) >= n3) return;  // whole waves past the end
   int hist = -1;
   if (t < n3) {
@@ -153,7 +154,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
   __shared__ T sDC[PHI ? NCOEF : 1];
 
   const long t = blockIdx.x;  // one block per triplet
-  if (t >= D.totals[T_N3]) return;
+  if (t >= D.totals[T_N3] || t >= D.cap3) return;
   const int tid = threadIdx.x;
   static_assert(sizeof(TriGeo) % 8 == 0 && sizeof(TriGeo) / 8 <= 256, "TriGeo copy");
   if (tid < (int)(sizeof(TriGeo) / 8))
@@ -522,6 +523,17 @@ __global__ __launch_bounds__(256, 5) void k_fs3_eval_f64(Dev D, Consts K, const 
 __global__ __launch_bounds__(256, 7) void k_fs3_eval_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const float* __restrict__ coef) {
   fs3_eval_body<float, false, false, false>(D, K, pairpar, coef, nullptr);
+}
+
+// The same with the region's coefficient table staged in LDS (a tuning choice).
+__global__ __launch_bounds__(256, 3) void k_fs3_eval_f64_lds(Dev D, Consts K, const double4* __restrict__ pairpar,
+                                                             const double* __restrict__ coef) {
+  fs3_eval_body<double, false, false, true>(D, K, pairpar, coef, nullptr);
+}
+
+__global__ __launch_bounds__(256, 7) void k_fs3_eval_f32_lds(Dev D, Consts K, const double4* __restrict__ pairpar,
+                                                             const float* __restrict__ coef) {
+  fs3_eval_body<float, false, false, true>(D, K, pairpar, coef, nullptr);
 }
 
 // The same evaluation plus the per-triplet coefficient-gradient rows.

@@ -195,14 +195,15 @@ __device__ void fs2_pair(const Dev& D, const Consts& K, long p, int lane) {
 }
 
 // ---------------------------------------------------------------- spectator pass
-// FS2_PARTS blocks per molecule k, each taking an equal share of k's items
+// D.fs2_parts blocks per molecule k, each taking an equal share of k's items
 // (k; i = nb2(k)[a], j = nb2(i)[q]) with q >= up2(i), flattened in the
 // reference's order (a, then q).
 __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   extern __shared__ double dsm[];
   __shared__ int wtot[4];
   __shared__ int carry_s;
-  const int k = blockIdx.x / FS2_PARTS, part = blockIdx.x - k * FS2_PARTS;
+  const int parts = D.fs2_parts;
+  const int k = blockIdx.x / parts, part = blockIdx.x - k * parts;
   if (k >= D.nm) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int n = D.nb2_cnt[k];
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   if (tid == 0) off[n] = carry_s;
   __syncthreads();
   const int total = off[n];
-  const int share = (total + FS2_PARTS - 1) / FS2_PARTS;
+  const int share = (total + parts - 1) / parts;
   const int f0 = part * share, f1 = min(total, f0 + share);
   const double* skp = D.site + (long)k * NS3 * 3;
   double Fk[NS2 * 3 + 3];
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   if (tid < NV) {
     const double v = (wred[0][tid] + wred[1][tid]) + (wred[2][tid] + wred[3][tid]);
     double* acck = D.acc + (long)k * ACC_W;
-    const long e = (long)k * FS2_PARTS + part;
+    const long e = (long)k * parts + part;
     if (tid < NS2 * 3) unsafeAtomicAdd(acck + tid, v);
     else if (tid < NS2 * 3 + 3) unsafeAtomicAdd(acck + ACC_C2 + (tid - NS2 * 3), v);
     else if (tid == NS2 * 3 + 3) D.ew2[2 * e] = v;

@@ -279,6 +279,7 @@ __device__ void block_exclusive_scan(const int* cnt, int* off, long n, int* tota
 // phase 0: per-molecule counts -> sub-list and pair offsets
 // phase 1: per-sub-list candidate counts -> FS3 triplet offsets
 // phase 2: per-sub-list candidate counts -> FS2 item offsets (list export only)
+// phase 3: per-box reduction chunk counts -> chunk offsets
 __global__ __launch_bounds__(1024) void k_scan(Dev D, int phase, int do3, int do2) {
   if (phase == 0) {
     if (do3) block_exclusive_scan(D.sl3_cnt, D.sl3_off, D.nm, D.totals + T_SL3);
@@ -288,33 +289,39 @@ __global__ __launch_bounds__(1024) void k_scan(Dev D, int phase, int do3, int do
     }
   } else if (phase == 1) {
     if (do3) block_exclusive_scan(D.cnt3s, D.off3s, D.totals[T_SL3], D.totals + T_N3);
-  } else {
+  } else if (phase == 2) {
     if (do2) block_exclusive_scan(D.cnt2s, D.off2s, D.totals[T_SL2], D.totals + T_NI2L);
+  } else {
+    // reduction chunk counts per box -> first chunk of each box (in place)
+    block_exclusive_scan(D.box_c3, D.box_c3, D.nbox, D.totals + T_NCH3);
+    block_exclusive_scan(D.box_c2, D.box_c2, D.nbox, D.totals + T_NCH2);
   }
 }
 
 // Triplet range of each box (the list is sorted by its first molecule, and
-// boxes are contiguous molecule ranges) and the first reduction chunk of each
-// box, for FS3 triplets and for the FS2 energies (FS2_PARTS entries per molecule).
+// boxes are contiguous molecule ranges) and the number of reduction chunks of
+// each box, for FS3 triplets and for the FS2 energies (fs2_parts entries per
+// molecule).
+// Thread per box b (and b = nbox for the ends): range starts, and the chunk
+// counts of b in box_c3/box_c2, which k_scan phase 3 turns into offsets.
 __global__ void k_box_ranges(Dev D, int do3, int do2) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > D.nbox) return;
+  const int m = D.box_start[b];
+  const int t3 = do3 ? (m < D.nm ? D.off3s[D.sl3_off[m]] : D.totals[T_N3]) : 0;
+  const int t2 = do2 ? m * D.fs2_parts : 0;
+  D.box_t3[b] = t3;
+  D.box_t2[b] = t2;
   int c3 = 0, c2 = 0;
-  for (int b = 0; b <= D.nbox; ++b) {
-    const int m = D.box_start[b];
-    const int t3 = do3 ? (m < D.nm ? D.off3s[D.sl3_off[m]] : D.totals[T_N3]) : 0;
-    const int t2 = do2 ? m * FS2_PARTS : 0;
-    D.box_t3[b] = t3;
-    D.box_t2[b] = t2;
-    D.box_c3[b] = c3;
-    D.box_c2[b] = c2;
-    if (b < D.nbox) {
-      const int m1 = D.box_start[b + 1];
-      const int e3 = do3 ? (m1 < D.nm ? D.off3s[D.sl3_off[m1]] : D.totals[T_N3]) : 0;
-      const int e2 = do2 ? m1 * FS2_PARTS : 0;
-      c3 += (e3 - t3 + RED_CHUNK - 1) / RED_CHUNK;
-      c2 += (e2 - t2 + RED_CHUNK - 1) / RED_CHUNK;
-    }
+  if (b < D.nbox) {
+    const int m1 = D.box_start[b + 1];
+    const int e3 = do3 ? (m1 < D.nm ? D.off3s[D.sl3_off[m1]] : D.totals[T_N3]) : 0;
+    const int e2 = do2 ? m1 * D.fs2_parts : 0;
+    c3 = (e3 - t3 + RED_CHUNK - 1) / RED_CHUNK;
+    c2 = (e2 - t2 + RED_CHUNK - 1) / RED_CHUNK;
   }
+  D.box_c3[b] = c3;
+  D.box_c2[b] = c2;
 }
 
 }  // namespace ccpol3b

@@ -218,6 +218,38 @@ class ThreeBody:
         _lib.check(rc, self._ctx, "ccpol3b_gram_device")
         return g
 
+    # ------------------------------------------------------------------ launch shapes
+    def get_tuning(self) -> dict:
+        t = _lib.Tuning()
+        _lib.check(_lib.load().ccpol3b_get_tuning(self._ctx, C.byref(t)), self._ctx)
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = _lib.Tuning(**{**self.get_tuning(), **kw})
+        _lib.check(_lib.load().ccpol3b_set_tuning(self._ctx, C.byref(t)), self._ctx, "ccpol3b_set_tuning")
+
+    def autotune(self, mol_start, box_len, com, site, reps=3, stream=None):
+        """Time the candidate launch shapes on this batch and keep the fastest.
+        Returns (tuning dict, ms per evaluation)."""
+        import torch
+        ms = np.ascontiguousarray(mol_start, dtype=np.int32)
+        bl = np.ascontiguousarray(box_len, dtype=np.float64)
+        nbox, ld = len(bl), site.shape[2]
+        if stream is None:
+            stream = torch.cuda.current_stream(com.device).cuda_stream
+        p = lambda t, i: C.c_void_p(t[i].data_ptr())
+        t = _lib.Tuning()
+        best = C.c_double(0.0)
+        rc = _lib.load().ccpol3b_autotune(self._ctx, nbox, ms.ctypes.data_as(C.POINTER(C.c_int)), _dp(bl), ld,
+                                          p(com, 0), p(com, 1), p(com, 2), p(site, 0), p(site, 1), p(site, 2),
+                                          int(reps), C.c_void_p(stream), C.byref(t), C.byref(best))
+        _lib.check(rc, self._ctx, "ccpol3b_autotune")
+        return t.as_dict(), best.value
+
+    def graph(self, mol_start, box_len, com, site, phi=False, headroom=0.25, stream=None):
+        """Record the batched evaluation of these tensors into a HIP graph (BatchGraph)."""
+        return BatchGraph(self, mol_start, box_len, com, site, phi, headroom, stream)
+
     # ------------------------------------------------------------------ introspection
     def stats(self) -> dict:
         s = _lib.Stats()
@@ -237,6 +269,60 @@ class ThreeBody:
         out = np.zeros((max(n, 1), 3), dtype=np.int32)
         L.ccpol3b_get_fs2_list(self._ctx, out.ctypes.data_as(C.POINTER(C.c_int)), n)
         return out[:n]
+
+
+class BatchGraph:
+    """A batched evaluation (ThreeBody.eval_batch_device semantics) recorded into a
+    HIP graph for fixed tensors; ``launch()`` replays it with one launch and no
+    host synchronisation.  Owns its output tensors: fsite [3, nm, ld] (site
+    forces, accumulated by every replay -- zero it between replays if needed),
+    fcom2, fcom3 [3, nm], energies [nbox, 3] and, with phi=True, phi [nbox, 728]."""
+
+    def __init__(self, tb, mol_start, box_len, com, site, phi, headroom, stream):
+        import torch
+        self.ms = np.ascontiguousarray(mol_start, dtype=np.int32)
+        self.bl = np.ascontiguousarray(box_len, dtype=np.float64)
+        nbox, nm, ld = len(self.bl), com.shape[1], site.shape[2]
+        dev = com.device
+        self.com, self.site = com, site  # inputs read by every replay
+        self.fsite = torch.zeros((3, nm, ld), dtype=torch.float64, device=dev)
+        self.fcom2 = torch.zeros((3, nm), dtype=torch.float64, device=dev)
+        self.fcom3 = torch.zeros((3, nm), dtype=torch.float64, device=dev)
+        self.energies = torch.zeros((nbox, 3), dtype=torch.float64, device=dev)
+        self.phi = torch.zeros((nbox, 2 * NLIN), dtype=torch.float64, device=dev) if phi else None
+        if stream is None:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        p = lambda t, *i: C.c_void_p(t[i].data_ptr()) if i else C.c_void_p(t.data_ptr())
+        self._g = C.c_void_p()
+        self._L = _lib.load()
+        rc = self._L.ccpol3b_graph_create(
+            tb._ctx, nbox, self.ms.ctypes.data_as(C.POINTER(C.c_int)), _dp(self.bl), ld,
+            p(com, 0), p(com, 1), p(com, 2), p(site, 0), p(site, 1), p(site, 2),
+            p(self.fsite, 0), p(self.fsite, 1), p(self.fsite, 2), p(self.fcom2), p(self.fcom3), p(self.energies),
+            p(self.phi) if phi else None, float(headroom), C.c_void_p(stream), C.byref(self._g))
+        _lib.check(rc, tb._ctx, "ccpol3b_graph_create")
+
+    def launch(self, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(self.com.device).cuda_stream
+        _lib.check(self._L.ccpol3b_graph_launch(self._g, C.c_void_p(stream)), None, "ccpol3b_graph_launch")
+
+    def status(self) -> int:
+        """0, or ECAPACITY when the last replay outgrew the recorded capacities
+        (read after waiting for the stream)."""
+        return self._L.ccpol3b_graph_status(self._g)
+
+    def close(self):
+        if self._g:
+            self._L.ccpol3b_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def orbit_index(is_, js, ks, ip, jp, kp) -> int:

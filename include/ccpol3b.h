@@ -148,6 +148,48 @@ int ccpol3b_gram_device(ccpol3b_ctx *ctx, int nrow, int ncol, const double *d_ph
  * e.g. during a refit (initialization.f90:660-683 read them once). */
 int ccpol3b_set_coefficients(ccpol3b_ctx *ctx, const double *cs_sr, const double *cs_lr);
 
+/* Launch shapes of a context.  No counterpart in the reference (a CPU loop
+ * nest); chosen per workload by ccpol3b_autotune or set by hand. */
+typedef struct {
+  int fs3_lds_coef;           /* 1: FS3 blocks stage the region's coefficient table in LDS;
+                                 0: they read it through the vector cache (default) */
+  int fs2_parts;              /* FS2 spectator-pass blocks per molecule, 1..8 (default 4) */
+  int fs2_pair_waves_per_cu;  /* FS2 pair-pass wavefronts per CU, 1..1024 (default 64) */
+  int list_waves_per_cu;      /* FS3 sub-list sweep wavefronts per CU, 1..1024 (default 32) */
+} ccpol3b_tuning;
+
+int ccpol3b_get_tuning(const ccpol3b_ctx *ctx, ccpol3b_tuning *tuning);
+int ccpol3b_set_tuning(ccpol3b_ctx *ctx, const ccpol3b_tuning *tuning);
+
+/* Time every candidate launch shape on the given batch (ccpol3b_eval_batch_device
+ * arguments, `reps` timed evaluations each after one warm-up, HIP events on
+ * `stream`), keep the fastest in the context and return it.  The context
+ * allocates its own output scratch for the trials; the inputs are not modified. */
+int ccpol3b_autotune(ccpol3b_ctx *ctx, int nbox, const int *mol_start, const double *box_len, int ld_site,
+                     const double *d_x, const double *d_y, const double *d_z,
+                     const double *d_xs, const double *d_ys, const double *d_zs,
+                     int reps, void *stream, ccpol3b_tuning *best, double *best_ms);
+
+/* A batched evaluation recorded once into a HIP graph and replayed with one
+ * launch, for fixed shapes and device pointers (no counterpart in the
+ * reference).  Creation runs one checked evaluation on a private copy of the
+ * context (so its outputs are valid), enlarges the list capacities by
+ * `headroom` (e.g. 0.25 = 25 %) and records the launch sequence.  A replay
+ * never waits for the host; after waiting for the stream the caller reads
+ * ccpol3b_graph_status, which is CCPOL3B_ECAPACITY when the geometry outgrew
+ * the recorded capacities (that replay wrote none of the outputs).  Outputs
+ * follow ccpol3b_eval_batch_device (site forces accumulated); d_phi may be NULL. */
+typedef struct ccpol3b_graph ccpol3b_graph;
+int ccpol3b_graph_create(ccpol3b_ctx *ctx, int nbox, const int *mol_start, const double *box_len, int ld_site,
+                         const double *d_x, const double *d_y, const double *d_z,
+                         const double *d_xs, const double *d_ys, const double *d_zs,
+                         double *d_fxs, double *d_fys, double *d_fzs,
+                         double *d_fcom2, double *d_fcom3, double *d_energies, double *d_phi,
+                         double headroom, void *stream, ccpol3b_graph **graph);
+int ccpol3b_graph_launch(ccpol3b_graph *graph, void *stream);
+int ccpol3b_graph_status(const ccpol3b_graph *graph);
+void ccpol3b_graph_destroy(ccpol3b_graph *graph);
+
 int ccpol3b_last_stats(const ccpol3b_ctx *ctx, ccpol3b_stats *stats);
 
 /* Copy the last FS3 triplet list (0-based i<j<k, lexicographic order, the
