@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=2025)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-boxes and gradient-path measurements")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, run one device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -80,6 +82,80 @@ def timed_steps(tb, com, site, box, steps, warmup):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     return (t1 - t0) * 1e3 / steps, float(np.mean(ms_eval)), float(np.mean(ms_fs3)), tb.stats()
+
+
+def pack_boxes(boxes, dev):
+    import torch
+    com = np.concatenate([b.com for b in boxes])
+    site = np.concatenate([b.site for b in boxes])
+    ms = np.cumsum([0] + [b.nm for b in boxes]).astype(np.int32)
+    bl = np.array([b.box for b in boxes], float)
+    return (ms, bl, torch.tensor(com.T.copy(), device=dev),
+            torch.tensor(site.transpose(2, 0, 1).copy(), device=dev))
+
+
+def wall_ms(fn, reps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def bench_batched(p, seed, reps=10):
+    """96 independent periodic boxes, N log-uniform in [64, 1024], mixed FS2+FS3:
+    per-box sequential calls, one batched call, and autotuned graph replays."""
+    import torch
+    from ccpol23_plus_2025_mbmd_amd.geometry import uneven_boxes
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    dev = torch.device("cuda:0")
+    boxes = uneven_boxes(96, seed=seed)
+    ms, bl, com, site = pack_boxes(boxes, dev)
+    per_box = [(torch.tensor(b.com.T.copy(), device=dev),
+                torch.tensor(b.site.transpose(2, 0, 1).copy(), device=dev), b.box) for b in boxes]
+    out = {"n_boxes": len(boxes), "n_molecules": int(ms[-1]),
+           "box_sizes": [int(b.nm) for b in boxes]}
+    for prec in ("mixed", "fp64"):
+        tb = ThreeBody(p, prec)
+        tb.eval_batch_device(ms, bl, com, site)
+        n3 = tb.stats()["n_fs3"]
+        seq = wall_ms(lambda: [tb.eval_device(c, s, L) for c, s, L in per_box], max(2, reps // 4))
+        batch = wall_ms(lambda: tb.eval_batch_device(ms, bl, com, site), reps)
+        tune, tune_ms = tb.autotune(ms, bl, com, site, reps=2)
+        g = tb.graph(ms, bl, com, site)
+        graph = wall_ms(g.launch, reps)
+        assert g.status() == 0
+        g.close()
+        out[prec] = {"n_fs3_triplets": n3, "ms_per_box_sequential": seq, "ms_batched_call": batch,
+                     "ms_graph_replay": graph, "tuning": tune,
+                     "evals_per_s_graph": n3 / (graph * 1e-3), "evals_per_s_sequential": n3 / (seq * 1e-3),
+                     "graph_over_sequential": seq / graph}
+        log(f"batched {prec}: sequential {seq:.3f} ms, batched {batch:.3f} ms, graph {graph:.3f} ms, tuning {tune}")
+    return out
+
+
+def bench_gradient(p, seed, reps=10):
+    """4096 isolated trimers: energies, forces and dE/dc (728 per trimer), and the
+    728 x 728 Gram matrix of the coefficient gradients on the matrix cores."""
+    import torch
+    from ccpol23_plus_2025_mbmd_amd.geometry import trimer_set
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    dev = torch.device("cuda:0")
+    tri = trimer_set(4096, seed=seed)
+    ms, bl, com, site = pack_boxes(tri, dev)
+    out = {"n_trimers": len(tri)}
+    for prec in ("mixed", "fp64"):
+        tb = ThreeBody(p, prec)
+        phi = tb.eval_batch_grad_device(ms, bl, com, site)[4]
+        t_grad = wall_ms(lambda: tb.eval_batch_grad_device(ms, bl, com, site, phi=phi), reps)
+        t_gram = wall_ms(lambda: tb.gram(phi), reps)
+        out[prec] = {"n_fs3_triplets": tb.stats()["n_fs3"], "ms_energy_forces_coef_grad": t_grad,
+                     "ms_gram_4096x728": t_gram,
+                     "gram_tflops_equivalent_fp64": 2.0 * 4096 * 728 * 728 / 2 / (t_gram * 1e-3) / 1e12}
+        log(f"gradient {prec}: E+F+dE/dc {t_grad:.3f} ms, Gram {t_gram:.3f} ms")
+    return out
 
 
 def read_traffic():
@@ -181,6 +257,9 @@ def main():
         "stats_fp64_ms": {k: st_64[k] for k in ("ms_lists", "ms_fs2", "ms_fs3", "ms_fs3_eval", "ms_reduce",
                                                 "ms_total")},
     }
+    if not args.no_extras:
+        out["batched_96_boxes"] = bench_batched(p, args.seed)
+        out["gradient_4096_trimers"] = bench_gradient(p, args.seed)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cb = cpu_baseline(p, box, threads)

@@ -146,6 +146,7 @@ def trimer_set(n: int, seed: int, ravg_lo: float = 2.5, ravg_hi: float = 9.0,
     COM-COM separation >= dmin A.  The triangle shape is drawn by rejection:
     two side ratios ~ U(0.7, 1.3) around R_avg with the triangle inequality."""
     rng = np.random.default_rng(seed)
+    pcoord = principal_coords()[1]
     out = []
     while len(out) < n:
         ravg = rng.uniform(ravg_lo, ravg_hi)
@@ -164,7 +165,7 @@ def trimer_set(n: int, seed: int, ravg_lo: float = 2.5, ravg_hi: float = 9.0,
         rot = quat_to_rot(random_quaternions(1, rng))[0]
         pts = pts @ rot.T
         q = random_quaternions(3, rng)
-        out.append(WaterBox(pts, q, 0.0))
+        out.append(WaterBox(pts, q, 0.0, pcoord=pcoord))
     return out
 
 
