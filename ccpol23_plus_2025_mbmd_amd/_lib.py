@@ -110,6 +110,16 @@ def load():
     L.ccpol3b_graph_status.restype = C.c_int
     L.ccpol3b_graph_destroy.argtypes = [vp]
     L.ccpol3b_graph_destroy.restype = None
+    L.ccpol3b_md_create.argtypes = [vp, C.c_int, C.c_double, C.c_int, dp, C.c_double, dp, C.POINTER(C.c_void_p)]
+    L.ccpol3b_md_create.restype = C.c_int
+    L.ccpol3b_md_set_state.argtypes = [vp, dp, dp, dp, dp]
+    L.ccpol3b_md_set_state.restype = C.c_int
+    L.ccpol3b_md_get_state.argtypes = [vp, dp, dp, dp, dp]
+    L.ccpol3b_md_get_state.restype = C.c_int
+    L.ccpol3b_md_run.argtypes = [vp, C.c_int, C.c_double, dp]
+    L.ccpol3b_md_run.restype = C.c_int
+    L.ccpol3b_md_destroy.argtypes = [vp]
+    L.ccpol3b_md_destroy.restype = None
     L.ccpol3b_last_stats.argtypes = [vp, C.POINTER(Stats)]
     L.ccpol3b_last_stats.restype = C.c_int
     L.ccpol3b_get_fs3_list.argtypes = [vp, C.POINTER(C.c_int), C.c_long]
@@ -134,7 +144,8 @@ EXPORTED = [
     "ccpol3b_get_fs2_list", "ccpol3b_read_data3b", "ccpol3b_orbit_index", "ccpol3b_status_string",
     "ccpol3b_last_error", "ccpol3b_eval_batch_grad_device", "ccpol3b_set_coefficients", "ccpol3b_gram_device",
     "ccpol3b_get_tuning", "ccpol3b_set_tuning", "ccpol3b_autotune", "ccpol3b_graph_create", "ccpol3b_graph_launch",
-    "ccpol3b_graph_status", "ccpol3b_graph_destroy",
+    "ccpol3b_graph_status", "ccpol3b_graph_destroy", "ccpol3b_md_create", "ccpol3b_md_set_state",
+    "ccpol3b_md_get_state", "ccpol3b_md_run", "ccpol3b_md_destroy",
 ]
 
 

@@ -190,6 +190,26 @@ int ccpol3b_graph_launch(ccpol3b_graph *graph, void *stream);
 int ccpol3b_graph_status(const ccpol3b_graph *graph);
 void ccpol3b_graph_destroy(ccpol3b_graph *graph);
 
+/* Rigid-water NVE driver on the device (replaces the rigid-body steps of
+ * nve_1, evolve.f90:621-1062, with NO_SQUISH rotation, :8107-8204, for the
+ * three-body forces of this library; the caller's two-body, Ewald and
+ * induction terms are not part of it).  pcoord: [ns][3] principal-frame site
+ * coordinates (A, initialization.f90:728-845), ns >= 17; mass (amu) and the
+ * principal moments pmi[3] (amu A^2); box (A, <= 0 open).  State arrays on
+ * the host: com, vel (A/ps) and body-frame angular momentum jbody
+ * (amu A^2/ps) [nm][3], quaternions [nm][4].  ccpol3b_md_run advances nsteps
+ * steps of dt (ps) and writes, per step, {E3 = qfs2 + qfs3, translational and
+ * rotational kinetic energy, their sum} in kcal/mol to records[nsteps][4]
+ * (may be NULL).  The driver borrows ctx (precision, parameters). */
+typedef struct ccpol3b_md ccpol3b_md;
+int ccpol3b_md_create(ccpol3b_ctx *ctx, int nm, double box, int ns, const double *pcoord, double mass,
+                      const double *pmi, ccpol3b_md **md);
+int ccpol3b_md_set_state(ccpol3b_md *md, const double *com, const double *quat, const double *vel,
+                         const double *jbody);
+int ccpol3b_md_get_state(ccpol3b_md *md, double *com, double *quat, double *vel, double *jbody);
+int ccpol3b_md_run(ccpol3b_md *md, int nsteps, double dt, double *records);
+void ccpol3b_md_destroy(ccpol3b_md *md);
+
 int ccpol3b_last_stats(const ccpol3b_ctx *ctx, ccpol3b_stats *stats);
 
 /* Copy the last FS3 triplet list (0-based i<j<k, lexicographic order, the
