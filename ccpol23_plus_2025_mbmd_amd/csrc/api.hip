@@ -38,6 +38,7 @@ __global__ void k_gram_mfma(int, int, int, const float*, const int*, double*, in
 __global__ void k_fs2_pair_pass(Dev, Consts);
 __global__ void k_fs2_spectator_pass(Dev, Consts);
 size_t fs2_lds_bytes(int cap_nb);
+size_t fs2_pair_lds_bytes(int cap_nb);
 __global__ void k_finish(Dev, int, int, int, int, double*, double*, double*, double*, double*);
 __global__ void k_energy_chunks(Dev, int, long);
 __global__ void k_energy_boxes(Dev, int, int, double*);
@@ -134,7 +135,7 @@ struct ccpol3b_ctx {
   float* d_coef32 = nullptr;
   double* d_phitrip = nullptr;   // [cap_phi][728] per-triplet coefficient-gradient rows
   long cap_phi = 0;
-  ccpol3b_tuning tune{0, 4, 64, 32};
+  ccpol3b_tuning tune{0, 4, 2, 32};
   // creation parameters (Fortran-ordered 17x17 tables, current coefficients),
   // kept so a graph can own a private copy of the context
   std::vector<double> h_tab[6], h_cs[2];
@@ -193,7 +194,7 @@ void free_dev(ccpol3b_ctx* c) {
   dev_free(D.cx); dev_free(D.cy); dev_free(D.cz); dev_free(D.site);
   dev_free(D.mol_box); dev_free(D.boxL); dev_free(D.box_start);
   dev_free(D.nb3_cnt); dev_free(D.nb3); dev_free(D.up3);
-  dev_free(D.nb2_cnt); dev_free(D.nb2); dev_free(D.up2);
+  dev_free(D.nb2_cnt); dev_free(D.nb2); dev_free(D.up2); dev_free(D.nbv2);
   dev_free(D.sl3_cnt); dev_free(D.sl3_off); dev_free(D.sl2_cnt); dev_free(D.sl2_off);
   dev_free(D.cnt3s); dev_free(D.off3s); dev_free(D.cnt2s); dev_free(D.off2s);
   dev_free(D.tri); dev_free(D.geo3); dev_free(D.pcnt2); dev_free(D.poff2); dev_free(D.items2);
@@ -215,7 +216,7 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
         (rc = dev_alloc(c, D.site, (size_t)n * NS3 * 3)) || (rc = dev_alloc(c, D.mol_box, n)) ||
         (rc = dev_alloc(c, D.nb3_cnt, n)) || (rc = dev_alloc(c, D.nb2_cnt, n)) ||
         (rc = dev_alloc(c, D.up3, n)) || (rc = dev_alloc(c, D.up2, n)) ||
-        (rc = dev_alloc(c, D.nb3, nn)) || (rc = dev_alloc(c, D.nb2, nn)) ||
+        (rc = dev_alloc(c, D.nb3, nn)) || (rc = dev_alloc(c, D.nb2, nn)) || (rc = dev_alloc(c, D.nbv2, 3 * nn)) ||
         (rc = dev_alloc(c, D.sl3_cnt, n)) || (rc = dev_alloc(c, D.sl3_off, n + 1)) ||
         (rc = dev_alloc(c, D.sl2_cnt, n)) || (rc = dev_alloc(c, D.sl2_off, n + 1)) ||
         (rc = dev_alloc(c, D.pcnt2, n)) || (rc = dev_alloc(c, D.poff2, n + 1)) ||
@@ -294,6 +295,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     c->cap_phi = D.cap3;
   }
   D.fs2_parts = c->tune.fs2_parts;
+  D.fs2_pair_parts = c->tune.fs2_pair_parts;
   if (!capture) {
     HIPCHK(c, hipMemcpyAsync(D.mol_box, c->h_molbox.data(), sizeof(int) * nm, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(D.boxL, c->h_boxL.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
@@ -319,11 +321,10 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   EVREC(1);
   if (do2) {
     // pair pass (S_ij, pair-owned forces) before the spectator pass that reads S_ij
-    const size_t lds = fs2_lds_bytes(D.cap_nb);
-    if (lds > 65536) return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the spectator pass");
-    const long pair_waves =
-        std::min<long>((long)nm * D.cap_nb / 2 + 1, (long)c->cu_count * c->tune.fs2_pair_waves_per_cu);
-    hipLaunchKernelGGL(k_fs2_pair_pass, dim3(waves_grid(pair_waves)), dim3(256), 0, st, D, c->K);
+    const size_t lds = fs2_lds_bytes(D.cap_nb), lds_p = fs2_pair_lds_bytes(D.cap_nb);
+    if (std::max(lds, lds_p) > 65536)
+      return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the FS2 passes");
+    hipLaunchKernelGGL(k_fs2_pair_pass, dim3(nm * D.fs2_pair_parts), dim3(256), lds_p, st, D, c->K);
     hipLaunchKernelGGL(k_fs2_spectator_pass, dim3(nm * D.fs2_parts), dim3(256), lds, st, D, c->K);
     LAUNCHCHK(c, "FS2");
   }
@@ -711,8 +712,8 @@ int ccpol3b_get_tuning(const ccpol3b_ctx* c, ccpol3b_tuning* t) {
 }
 
 int ccpol3b_set_tuning(ccpol3b_ctx* c, const ccpol3b_tuning* t) {
-  if (!c || !t || t->fs2_parts < 1 || t->fs2_parts > FS2_PARTS_MAX || t->fs2_pair_waves_per_cu < 1 ||
-      t->fs2_pair_waves_per_cu > 1024 || t->list_waves_per_cu < 1 || t->list_waves_per_cu > 1024)
+  if (!c || !t || t->fs2_parts < 1 || t->fs2_parts > FS2_PARTS_MAX || t->fs2_pair_parts < 1 ||
+      t->fs2_pair_parts > FS2_PARTS_MAX || t->list_waves_per_cu < 1 || t->list_waves_per_cu > 1024)
     return fail(c, CCPOL3B_EINVAL, "invalid tuning");
   c->tune = *t;
   c->tune.fs3_lds_coef = t->fs3_lds_coef ? 1 : 0;
@@ -738,10 +739,10 @@ int ccpol3b_autotune(ccpol3b_ctx* c, int nbox, const int* mol_start, const doubl
   const ccpol3b_tuning keep = c->tune;
   ccpol3b_tuning win = keep;
   double win_ms = 1e300;
-  const int parts[] = {2, 4, 8}, pwaves[] = {16, 64, 256}, lwaves[] = {16, 64};
+  const int parts[] = {2, 4, 8}, pparts[] = {1, 2, 4}, lwaves[] = {16, 64};
   for (int lds = 0; lds < 2 && rc == CCPOL3B_OK; ++lds)
     for (int pp : parts)
-      for (int pw : pwaves)
+      for (int pw : pparts)
         for (int lw : lwaves) {
           if (rc) break;
           c->tune = ccpol3b_tuning{lds, pp, pw, lw};

@@ -67,6 +67,7 @@ struct TriGeo {
 struct Dev {
   int nm, nbox, cap_nb;
   int fs2_parts;              // spectator-pass blocks per molecule (1..FS2_PARTS_MAX)
+  int fs2_pair_parts;         // pair-pass blocks per molecule (1..FS2_PARTS_MAX)
   long cap3, cap_pairs2, cap_items2, cap_sl3, cap_sl2;
   // geometry in bohr
   double *cx, *cy, *cz;   // [nm]
@@ -77,6 +78,7 @@ struct Dev {
   // neighbour lists (ascending molecule index) and their first upper entries
   int *nb3_cnt, *nb3, *up3;   // FS3 cutoff: [nm], [nm][cap_nb], [nm]
   int *nb2_cnt, *nb2, *up2;   // FS2 cutoff
+  double *nbv2;               // [nm][3][cap_nb] x_i - x_nb (min image, bohr) of each FS2 entry
   // sub-list counts/offsets: FS3 (i; j) and FS2 (k; i)
   int *sl3_cnt, *sl3_off;     // [nm], [nm+1]
   int *sl2_cnt, *sl2_off;

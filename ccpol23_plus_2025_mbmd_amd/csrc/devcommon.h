@@ -42,6 +42,28 @@ __device__ __forceinline__ int owner_of(const int* off, int n, long s) {
   return lo;
 }
 
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int lane = threadIdx.x & 63;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Appends the values of the flagged threads of a 256-thread block to q[base..]
+// in thread order and returns how many there were.  Every thread of the block
+// calls it; wcnt is a 4-int LDS scratch.  Ends with a barrier, so q is visible
+// to the whole block on return.
+__device__ __forceinline__ int block_append(bool flag, int val, int* q, int base, int* wcnt) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t m = __ballot(flag);
+  if (lane == 0) wcnt[w] = __popcll(m);
+  __syncthreads();
+  int off = base;
+  for (int u = 0; u < w; ++u) off += wcnt[u];
+  if (flag) q[off + __popcll(m & lanemask_lt())] = val;
+  const int tot = (wcnt[0] + wcnt[1]) + (wcnt[2] + wcnt[3]);
+  __syncthreads();
+  return tot;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

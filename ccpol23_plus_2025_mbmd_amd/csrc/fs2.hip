@@ -8,17 +8,21 @@
 // FS2 neighbour of i other than k, accepted when the derived vector
 // x_kj = x_ki + x_ij is inside the cutoff.
 //
-// Two owner-computes passes, one block per molecule each, so no force is
+// Two owner-computes passes, a few blocks per molecule each, so no force is
 // scattered per item:
-//   pair pass (block i):      the pairs (i, j) and all their spectators k;
-//                             S_ij, the COM forces of i and j and the
-//                             exchange-site forces of both O,H,H triples;
-//   spectator pass (block k): all items with spectator k; the forces on k's
-//                             five sites and COM, and the item energies and
-//                             virials (summed per k in a fixed order).
+//   pair pass (blocks of i):      the pairs (i, j > i) and all their spectators
+//                                 k; S_ij, the COM forces of i and j and the
+//                                 exchange-site forces of both O,H,H triples;
+//   spectator pass (blocks of k): all items with spectator k; the forces on k's
+//                                 five sites and COM, and the item energies and
+//                                 virials (summed per k in a fixed order).
 // Both passes evaluate T_k, each with the derivatives it owns.  The acceptance
-// test is the list kernels' arithmetic (no FMA contraction), so both passes see
-// exactly the reference's items.
+// test is the list kernels' arithmetic (no FMA contraction) on the minimum-image
+// vectors the neighbour kernel stored with the FS2 lists, so both passes see
+// exactly the reference's items.  Each block first tests its candidates and
+// appends the accepted ones to an LDS queue in candidate order; the expensive T
+// evaluation then runs over the queue with every lane busy (about 45 % of the
+// candidates of a bulk-water box fail the derived-vector test).
 #include "devcommon.h"
 
 namespace ccpol3b {
@@ -94,110 +98,192 @@ __device__ __forceinline__ double fs2_T(const Fs2Geo& G, const double* __restric
   return T;
 }
 
-// Dynamic LDS of the spectator pass: 6 doubles and 4 ints per neighbour slot.
-size_t fs2_lds_bytes(int cap_nb) { return (size_t)cap_nb * (6 * sizeof(double) + 4 * sizeof(int)) + 16; }
+// Accepted candidates queued per block before they are evaluated.
+constexpr int FS2_QCAP = 1024;
 
-// ---------------------------------------------------------------- pair pass
-__device__ void fs2_pair(const Dev& D, const Consts& K, long p, int lane);
-
-// Wavefronts loop over the pairs p = (i, j = nb2(i)[q]), q >= up2(i); lanes run over
-// the spectator candidates k = nb2(i)[a], a != q.  Lanes 0..8 also own the
-// nine O,H,H site pairs of S_ij (:361-372) and of the exchange-site forces
-// F = -(sum_k f f T) dS/ds (:477-519).
-__global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
-  const int lane = threadIdx.x & 63;
-  const long np2 = D.totals[T_NP2];
-  const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
-  for (long p = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; p < np2; p += nwaves)
-    fs2_pair(D, K, p, lane);
+// Dynamic LDS of the spectator pass: 6 doubles and 4 ints per neighbour slot, the queue.
+size_t fs2_lds_bytes(int cap_nb) {
+  return (size_t)cap_nb * (6 * sizeof(double) + 4 * sizeof(int)) + FS2_QCAP * sizeof(int) + 16;
+}
+// Dynamic LDS of the pair pass: 6 doubles and 1 int per neighbour slot, 5 doubles
+// per pair slot, the queue.
+size_t fs2_pair_lds_bytes(int cap_nb) {
+  return (size_t)cap_nb * (11 * sizeof(double) + sizeof(int)) + FS2_QCAP * sizeof(int) + 16;
 }
 
-// The pair p: everything it owns (see k_fs2_pair_pass).
-__device__ void fs2_pair(const Dev& D, const Consts& K, long p, int lane) {
-  const int i = owner_of(D.poff2, D.nm, p);
-  const int n = D.nb2_cnt[i];
-  const int jq = D.up2[i] + (int)(p - D.poff2[i]);
-  const int* nb = D.nb2 + (long)i * D.cap_nb;
-  const int j = nb[jq];
-  const double L = D.boxL[D.mol_box[i]];
-  const double xi = D.cx[i], yi = D.cy[i], zi = D.cz[i];
-  const double xij[3] = {minimg_nc(xi - D.cx[j], L), minimg_nc(yi - D.cy[j], L), minimg_nc(zi - D.cz[j], L)};
-  const double Rij = sqrt(xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2]);
-  // S_ij and its scaling derivative
+// ---------------------------------------------------------------- pair pass
+// D.fs2_pair_parts blocks per molecule i, each taking an equal share of i's
+// upper pairs (i, j = nb2(i)[q]), q >= up2(i).  The candidates of pair q are the
+// other entries k = nb2(i)[a], a != q; item (k; i, j) is accepted when the
+// derived vector x_kj = x_ki + x_ij is inside the cutoff (:317-328).  Per pair
+// the block keeps S_ij (:361-372), sum_k f f T (for the exchange-site forces
+// F = -(sum_k f f T) dS/ds, :477-519) and the COM force on j in LDS; i's COM
+// force stays in registers until the end.
+__global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
+  extern __shared__ double dsm[];
+  __shared__ int wcnt[4];
+  __shared__ double sFsi[9];
+  __shared__ double wred[4][3];
+  const int parts = D.fs2_pair_parts;
+  const int i = blockIdx.x / parts, part = blockIdx.x - i * parts;
+  if (i >= D.nm) return;
+  const int n = D.nb2_cnt[i], up = D.up2[i];
+  const int share = (n - up + parts - 1) / parts;
+  const int q0 = up + part * share, q1 = min(n, q0 + share);
+  if (q0 >= q1) return;  // the whole block: no pair of i in this share
+  const int npb = q1 - q0;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int cap = D.cap_nb;
+  double* sx = dsm;  // x_ki = x_k - x_i (min image) of neighbour slot a
+  double* sy = sx + cap;
+  double* sz = sy + cap;
+  double* sR = sz + cap;   // R_ki
+  double* sF = sR + cap;   // f(R_ki)
+  double* sFp = sF + cap;  // f'(R_ki)
+  double* sS = sFp + cap;  // S_ij of pair slot ql = q - q0
+  double* sWT = sS + cap;  // sum over accepted k of f(R_ki) f(R_kj) T_k
+  double* sFj = sWT + cap; // [cap][3] COM force on j
+  int* snb = reinterpret_cast<int*>(sFj + 3 * cap);
+  int* queue = snb + cap;
+  const int* nb = D.nb2 + (long)i * cap;
+  const double* vx = D.nbv2 + (long)i * 3 * cap;
+  const double* vy = vx + cap;
+  const double* vz = vy + cap;
   const double* si = D.site + (long)i * NS3 * 3;
-  const double* sj = D.site + (long)j * NS3 * 3;
-  const int is = lane / 3, js = lane - 3 * (lane / 3);
-  double v9[3] = {0.0, 0.0, 0.0}, r9 = 1.0, e9 = 0.0, el9 = 0.0;
-  if (lane < 9) {
-    for (int c = 0; c < 3; ++c) v9[c] = xij[c] + (si[is * 3 + c] - sj[js * 3 + c]);
-    r9 = sqrt(v9[0] * v9[0] + v9[1] * v9[1] + v9[2] * v9[2]);
-    e9 = ETA3 * exp(-BETA3 * r9);
-    el9 = -BETA3 * r9 * e9;
+  for (int a = tid; a < n; a += 256) {
+    snb[a] = nb[a];
+    const double x = -vx[a], y = -vy[a], z = -vz[a];  // exact: minimg(-d) = -minimg(d)
+    sx[a] = x;
+    sy[a] = y;
+    sz[a] = z;
+    const double R = sqrt(x * x + y * y + z * z);
+    sR[a] = R;
+    fs2_damp(R, K, sF[a], sFp[a]);
   }
-  const double S = wave_sum(e9), Sl = wave_sum(el9);
-  if (lane == 0) {
+  if (tid < 9) sFsi[tid] = 0.0;
+  __syncthreads();
+  // pair slots: S_ij and dS/dlambda (nine O,H,H site pairs, summed in order)
+  for (int ql = tid; ql < npb; ql += 256) {
+    const int q = q0 + ql;
+    const double* sj = D.site + (long)snb[q] * NS3 * 3;
+    const double xij[3] = {-sx[q], -sy[q], -sz[q]};
+    double S = 0.0, Sl = 0.0;
+    for (int is = 0; is < 3; ++is)
+      for (int js = 0; js < 3; ++js) {
+        double v[3];
+        for (int c = 0; c < 3; ++c) v[c] = xij[c] + (si[is * 3 + c] - sj[js * 3 + c]);
+        const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        const double e = ETA3 * exp(-BETA3 * r);
+        S += e;
+        Sl += -BETA3 * r * e;
+      }
+    sS[ql] = S;
+    sWT[ql] = 0.0;
+    sFj[3 * ql] = sFj[3 * ql + 1] = sFj[3 * ql + 2] = 0.0;
+    const long p = (long)D.poff2[i] + (q - up);
     D.s2pair[2 * p] = S;
     D.s2pair[2 * p + 1] = Sl;
   }
-  // spectators
-  double WT = 0.0, Fi[3] = {0.0, 0.0, 0.0}, Fj[3] = {0.0, 0.0, 0.0};
-  for (int a = lane; a < n; a += 64) {
-    if (a == jq) continue;
-    const int k = nb[a];
-    Fs2Geo G;
-    G.xij[0] = xij[0];
-    G.xij[1] = xij[1];
-    G.xij[2] = xij[2];
-    G.xki[0] = minimg_nc(D.cx[k] - xi, L);
-    G.xki[1] = minimg_nc(D.cy[k] - yi, L);
-    G.xki[2] = minimg_nc(D.cz[k] - zi, L);
-    G.xkj[0] = G.xki[0] + xij[0];
-    G.xkj[1] = G.xki[1] + xij[1];
-    G.xkj[2] = G.xki[2] + xij[2];
-    if (!(sq3_nc(G.xkj[0], G.xkj[1], G.xkj[2]) < K.rc2_2)) continue;
-    G.Rij = Rij;
-    G.Rki = sqrt(G.xki[0] * G.xki[0] + G.xki[1] * G.xki[1] + G.xki[2] * G.xki[2]);
-    G.Rkj = sqrt(G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2]);
-    double dTdRi[3], dTdRj[3], Tl;
-    const double T = fs2_T<false, true>(G, D.site + (long)k * NS3 * 3, nullptr, dTdRi, dTdRj, Tl);
-    double fki, fpki, fkj, fpkj;
-    fs2_damp(G.Rki, K, fki, fpki);
-    fs2_damp(G.Rkj, K, fkj, fpkj);
-    const double ff = fki * fkj, ffS = ff * S;
-    const double ci = S * T * fpki * fkj / G.Rki, cj = S * T * fki * fpkj / G.Rkj;
-    for (int c = 0; c < 3; ++c) {
-      Fi[c] += -ffS * dTdRi[c] + ci * G.xki[c];
-      Fj[c] += -ffS * dTdRj[c] + cj * G.xkj[c];
+  __syncthreads();
+  // candidates (ql, a) in order; accepted ones are queued, then evaluated
+  double Fi[3] = {0.0, 0.0, 0.0};
+  const int ncand = npb * n;
+  int qn = 0;
+  for (int c0 = 0; c0 < ncand; c0 += 256) {
+    const int cidx = c0 + tid;
+    bool ok = false;
+    int val = 0;
+    if (cidx < ncand) {
+      const int ql = cidx / n, a = cidx - ql * n;
+      const int q = q0 + ql;
+      if (a != q) {
+        // x_kj = x_ki + x_ij with x_ij = -(slot q)
+        ok = sq3_nc(sx[a] - sx[q], sy[a] - sy[q], sz[a] - sz[q]) < K.rc2_2;
+        val = (ql << 16) | a;
+      }
     }
-    WT += ff * T;
+    qn += block_append(ok, val, queue, qn, wcnt);
+    if (qn <= FS2_QCAP - 256 && c0 + 256 < ncand) continue;  // block-uniform
+    for (int e = tid; e < qn; e += 256) {
+      const int ql = queue[e] >> 16, a = queue[e] & 0xffff;
+      const int q = q0 + ql;
+      Fs2Geo G;
+      G.xij[0] = -sx[q];
+      G.xij[1] = -sy[q];
+      G.xij[2] = -sz[q];
+      G.xki[0] = sx[a];
+      G.xki[1] = sy[a];
+      G.xki[2] = sz[a];
+      G.xkj[0] = sx[a] - sx[q];
+      G.xkj[1] = sy[a] - sy[q];
+      G.xkj[2] = sz[a] - sz[q];
+      G.Rij = sR[q];
+      G.Rki = sR[a];
+      G.Rkj = sqrt(G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2]);
+      double dTdRi[3], dTdRj[3], Tl;
+      const double T = fs2_T<false, true>(G, D.site + (long)snb[a] * NS3 * 3, nullptr, dTdRi, dTdRj, Tl);
+      const double fki = sF[a], fpki = sFp[a];
+      double fkj, fpkj;
+      fs2_damp(G.Rkj, K, fkj, fpkj);
+      const double S = sS[ql];
+      const double ff = fki * fkj, ffS = ff * S;
+      const double ci = S * T * fpki * fkj / G.Rki, cj = S * T * fki * fpkj / G.Rkj;
+      for (int c = 0; c < 3; ++c) {
+        Fi[c] += -ffS * dTdRi[c] + ci * G.xki[c];
+        atomicAdd(sFj + 3 * ql + c, -ffS * dTdRj[c] + cj * G.xkj[c]);
+      }
+      atomicAdd(sWT + ql, ff * T);
+    }
+    qn = 0;
+    __syncthreads();  // the queue is refilled
   }
-  WT = wave_sum(WT);
+  // exchange-site forces, F = (sum_k f f T) beta e_ab / r_ab v_ab on i's site a and
+  // minus that on j's site b: six threads per pair, three summing over i's
+  // sites for one site of j (and adding j's COM force component), three over
+  // j's sites for one site of i
+  for (int t = tid; t < npb * 6; t += 256) {
+    const int ql = t / 6, role = t - 6 * ql;
+    const int q = q0 + ql, j = snb[q];
+    const double* sj = D.site + (long)j * NS3 * 3;
+    const double xij[3] = {-sx[q], -sy[q], -sz[q]};
+    const double WTb = sWT[ql] * BETA3;
+    const int own = role < 3 ? role : role - 3;
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int o = 0; o < 3; ++o) {
+      const int is = role < 3 ? o : own, js = role < 3 ? own : o;
+      double v[3];
+      for (int c = 0; c < 3; ++c) v[c] = xij[c] + (si[is * 3 + c] - sj[js * 3 + c]);
+      const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      const double g = WTb * ETA3 * exp(-BETA3 * r) / r;
+      for (int c = 0; c < 3; ++c) f[c] += g * v[c];
+    }
+    if (role < 3) {
+      double* accj = D.acc + (long)j * ACC_W;
+      for (int c = 0; c < 3; ++c) unsafeAtomicAdd(accj + own * 3 + c, -f[c]);
+      unsafeAtomicAdd(accj + ACC_C2 + own, sFj[3 * ql + own]);
+    } else {
+      for (int c = 0; c < 3; ++c) atomicAdd(sFsi + own * 3 + c, f[c]);
+    }
+  }
   for (int c = 0; c < 3; ++c) {
-    Fi[c] = wave_sum(Fi[c]);
-    Fj[c] = wave_sum(Fj[c]);
+    const double v = wave_sum(Fi[c]);
+    if (lane == 0) wred[w][c] = v;
   }
-  // exchange-site forces: lane (is, js) holds -WT dS/dv v-part; sum over js for
-  // i's site is (lanes 3 is .. 3 is + 2) and over is for j's site js (lanes js, js+3, js+6)
-  const double g1 = lane < 9 ? WT * BETA3 * e9 / r9 : 0.0;
+  __syncthreads();
   double* acci = D.acc + (long)i * ACC_W;
-  double* accj = D.acc + (long)j * ACC_W;
-  for (int c = 0; c < 3; ++c) {
-    const double f = g1 * v9[c];
-    const double fi = f + __shfl(f, lane + 1, 64) + __shfl(f, lane + 2, 64);
-    const double fj = f + __shfl(f, lane + 3, 64) + __shfl(f, lane + 6, 64);
-    if (lane < 9 && js == 0) unsafeAtomicAdd(acci + is * 3 + c, fi);
-    if (lane < 3) unsafeAtomicAdd(accj + lane * 3 + c, -fj);
-  }
-  if (lane < 3) {
-    unsafeAtomicAdd(acci + ACC_C2 + lane, lane == 0 ? Fi[0] : lane == 1 ? Fi[1] : Fi[2]);
-    unsafeAtomicAdd(accj + ACC_C2 + lane, lane == 0 ? Fj[0] : lane == 1 ? Fj[1] : Fj[2]);
+  if (tid < 9) unsafeAtomicAdd(acci + tid, sFsi[tid]);
+  else if (tid < 12) {
+    const int c = tid - 9;
+    unsafeAtomicAdd(acci + ACC_C2 + c, (wred[0][c] + wred[1][c]) + (wred[2][c] + wred[3][c]));
   }
 }
 
 // ---------------------------------------------------------------- spectator pass
-// D.fs2_parts blocks per molecule k, each taking an equal share of k's items
-// (k; i = nb2(k)[a], j = nb2(i)[q]) with q >= up2(i), flattened in the
-// reference's order (a, then q).
+// D.fs2_parts blocks per molecule k, each taking an equal share of k's
+// candidates (k; i = nb2(k)[a], j = nb2(i)[q]) with q >= up2(i), flattened in
+// the reference's order (a, then q).  Accepted items are queued in that order
+// and every thread sums its queue entries in order, so the energy and virial
+// sums are the same run to run.
 __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   extern __shared__ double dsm[];
   __shared__ int wtot[4];
@@ -208,7 +294,7 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int n = D.nb2_cnt[k];
   const int cap = D.cap_nb;
-  double* sx = dsm;  // x_ki = minimg(c_k - c_i) of neighbour slot a
+  double* sx = dsm;  // x_ki = x_k - x_i (min image) of neighbour slot a
   double* sy = sx + cap;
   double* sz = sy + cap;
   double* sRk = sz + cap;
@@ -218,8 +304,8 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   int* off = nbl + cap;   // [cap+1] first candidate of sub-list a
   int* u0s = off + cap + 1;
   int* pofs = u0s + cap;  // pair index of candidate q of sub-list a: pofs[a] + q
-  const double L = D.boxL[D.mol_box[k]];
-  const double xk = D.cx[k], yk = D.cy[k], zk = D.cz[k];
+  int* queue = pofs + cap;
+  const double* vk = D.nbv2 + (long)k * 3 * cap;
   if (tid == 0) carry_s = 0;
   for (int a0 = 0; a0 < n; a0 += 256) {
     const int a = a0 + tid;
@@ -227,7 +313,7 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
     if (a < n) {
       const int i = D.nb2[(long)k * cap + a];
       nbl[a] = i;
-      const double x = minimg_nc(xk - D.cx[i], L), y = minimg_nc(yk - D.cy[i], L), z = minimg_nc(zk - D.cz[i], L);
+      const double x = vk[a], y = vk[cap + a], z = vk[2 * cap + a];
       sx[a] = x;
       sy[a] = y;
       sz[a] = z;
@@ -259,54 +345,69 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   __syncthreads();
   const int total = off[n];
   const int share = (total + parts - 1) / parts;
-  const int f0 = part * share, f1 = min(total, f0 + share);
+  const int f0 = min(total, part * share), f1 = min(total, f0 + share);
   const double* skp = D.site + (long)k * NS3 * 3;
   double Fk[NS2 * 3 + 3];
 #pragma unroll
   for (int a = 0; a < NS2 * 3 + 3; ++a) Fk[a] = 0.0;
   double E = 0.0, W = 0.0;
-  int nitems = 0;
+  int nitems = 0, qn = 0;
   for (int base = f0; base < f1; base += 256) {
     const int f = base + tid;
-    if (f >= f1) continue;
-    int lo = 0, hi = n;  // off[lo] <= f < off[hi]
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (off[mid] <= f) lo = mid; else hi = mid;
+    bool ok = false;
+    int val = 0;
+    if (f < f1) {
+      int lo = 0, hi = n;  // off[lo] <= f < off[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= f) lo = mid; else hi = mid;
+      }
+      const int a = lo, i = nbl[a];
+      const int qj = u0s[a] + (f - off[a]);
+      const long e = (long)i * cap + qj;
+      if (D.nb2[e] != k) {
+        const double* vi = D.nbv2 + (long)i * 3 * cap + qj;  // x_ij = x_i - x_j
+        ok = sq3_nc(sx[a] + vi[0], sy[a] + vi[cap], sz[a] + vi[2 * cap]) < K.rc2_2;
+        val = (a << 16) | qj;
+      }
     }
-    const int a = lo, i = nbl[a];
-    const int qj = u0s[a] + (f - off[a]);
-    const int j = D.nb2[(long)i * cap + qj];
-    if (j == k) continue;
-    Fs2Geo G;
-    G.xki[0] = sx[a];
-    G.xki[1] = sy[a];
-    G.xki[2] = sz[a];
-    G.xij[0] = minimg_nc(D.cx[i] - D.cx[j], L);
-    G.xij[1] = minimg_nc(D.cy[i] - D.cy[j], L);
-    G.xij[2] = minimg_nc(D.cz[i] - D.cz[j], L);
-    G.xkj[0] = G.xki[0] + G.xij[0];
-    G.xkj[1] = G.xki[1] + G.xij[1];
-    G.xkj[2] = G.xki[2] + G.xij[2];
-    if (!(sq3_nc(G.xkj[0], G.xkj[1], G.xkj[2]) < K.rc2_2)) continue;
-    ++nitems;
-    G.Rij = sqrt(G.xij[0] * G.xij[0] + G.xij[1] * G.xij[1] + G.xij[2] * G.xij[2]);
-    G.Rki = sRk[a];
-    G.Rkj = sqrt(G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2]);
-    const long pr = (long)pofs[a] + qj;
-    const double S = D.s2pair[2 * pr], Sl = D.s2pair[2 * pr + 1];
-    double dTdsk[NS2][3], Tl;
-    const double T = fs2_T<true, false>(G, skp, dTdsk, nullptr, nullptr, Tl);
-    const double fki = sF[a], fpki = sFp[a];
-    double fkj, fpkj;
-    fs2_damp(G.Rkj, K, fkj, fpkj);
-    const double ff = fki * fkj, ffS = ff * S;
-    for (int s2 = 0; s2 < NS2; ++s2)
-      for (int c = 0; c < 3; ++c) Fk[s2 * 3 + c] += -ffS * dTdsk[s2][c];
-    const double ci = S * T * fpki * fkj / G.Rki, cj = S * T * fki * fpkj / G.Rkj;
-    for (int c = 0; c < 3; ++c) Fk[NS2 * 3 + c] += -(ci * G.xki[c] + cj * G.xkj[c]);
-    E += ff * S * T;
-    W += ff * (Sl * T + S * Tl) + S * T * (G.Rki * fpki * fkj + G.Rkj * fki * fpkj);
+    qn += block_append(ok, val, queue, qn, wtot);
+    if (qn <= FS2_QCAP - 256 && base + 256 < f1) continue;  // block-uniform
+    for (int e = tid; e < qn; e += 256) {
+      const int a = queue[e] >> 16, qj = queue[e] & 0xffff;
+      const int i = nbl[a];
+      const double* vi = D.nbv2 + (long)i * 3 * cap + qj;
+      Fs2Geo G;
+      G.xki[0] = sx[a];
+      G.xki[1] = sy[a];
+      G.xki[2] = sz[a];
+      G.xij[0] = vi[0];
+      G.xij[1] = vi[cap];
+      G.xij[2] = vi[2 * cap];
+      G.xkj[0] = G.xki[0] + G.xij[0];
+      G.xkj[1] = G.xki[1] + G.xij[1];
+      G.xkj[2] = G.xki[2] + G.xij[2];
+      ++nitems;
+      G.Rij = sqrt(G.xij[0] * G.xij[0] + G.xij[1] * G.xij[1] + G.xij[2] * G.xij[2]);
+      G.Rki = sRk[a];
+      G.Rkj = sqrt(G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2]);
+      const long pr = (long)pofs[a] + qj;
+      const double S = D.s2pair[2 * pr], Sl = D.s2pair[2 * pr + 1];
+      double dTdsk[NS2][3], Tl;
+      const double T = fs2_T<true, false>(G, skp, dTdsk, nullptr, nullptr, Tl);
+      const double fki = sF[a], fpki = sFp[a];
+      double fkj, fpkj;
+      fs2_damp(G.Rkj, K, fkj, fpkj);
+      const double ff = fki * fkj, ffS = ff * S;
+      for (int s2 = 0; s2 < NS2; ++s2)
+        for (int c = 0; c < 3; ++c) Fk[s2 * 3 + c] += -ffS * dTdsk[s2][c];
+      const double ci = S * T * fpki * fkj / G.Rki, cj = S * T * fki * fpkj / G.Rkj;
+      for (int c = 0; c < 3; ++c) Fk[NS2 * 3 + c] += -(ci * G.xki[c] + cj * G.xkj[c]);
+      E += ff * S * T;
+      W += ff * (Sl * T + S * Tl) + S * T * (G.Rki * fpki * fkj + G.Rkj * fki * fpkj);
+    }
+    qn = 0;
+    __syncthreads();  // the queue is refilled
   }
   // fixed-order block sums of the 18 spectator force components, E, W and the
   // item count: wavefront butterflies, then one pass over the four wavefronts

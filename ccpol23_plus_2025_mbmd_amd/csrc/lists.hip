@@ -24,11 +24,6 @@ __device__ __forceinline__ double sq3(double x, double y, double z) {
   return x * x + y * y + z * z;  // (x*x + y*y) + z*z, no contraction
 }
 
-__device__ __forceinline__ uint64_t lanemask_lt() {
-  const int lane = threadIdx.x & 63;
-  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
-
 // first position in an ascending list whose entry exceeds `v`
 __device__ __forceinline__ int upper_start(const int* list, int n, int v) {
   int lo = 0, hi = n;
@@ -80,13 +75,15 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D, Consts K) {
   int n3 = 0, n2 = 0, u3 = 0, u2 = 0;
   int* nb3 = D.nb3 + (long)i * D.cap_nb;
   int* nb2 = D.nb2 + (long)i * D.cap_nb;
+  double* v2 = D.nbv2 + (long)i * 3 * D.cap_nb;
   for (int base = j0; base < j1; base += 64) {
     const int j = base + lane;
     bool in3 = false, in2 = false;
+    double dx = 0.0, dy = 0.0, dz = 0.0;
     if (j < j1 && j != i) {
-      const double dx = minimg(xi - D.cx[j], L);
-      const double dy = minimg(yi - D.cy[j], L);
-      const double dz = minimg(zi - D.cz[j], L);
+      dx = minimg(xi - D.cx[j], L);
+      dy = minimg(yi - D.cy[j], L);
+      dz = minimg(zi - D.cz[j], L);
       const double d2 = sq3(dx, dy, dz);
       in3 = d2 < K.rc2_3;
       in2 = d2 < K.rc2_2;
@@ -99,7 +96,12 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D, Consts K) {
     }
     if (in2) {
       const int p = n2 + __popcll(m2 & lanemask_lt());
-      if (p < D.cap_nb) nb2[p] = j;
+      if (p < D.cap_nb) {
+        nb2[p] = j;
+        v2[p] = dx;
+        v2[D.cap_nb + p] = dy;
+        v2[2 * D.cap_nb + p] = dz;
+      }
     }
     n3 += __popcll(m3);
     n2 += __popcll(m2);

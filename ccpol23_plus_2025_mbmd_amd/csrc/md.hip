@@ -288,6 +288,13 @@ int ccpol3b_md_create(ccpol3b_ctx* ctx, int nm, double box, int ns, const double
     ccpol3b_md_destroy(md);
     return CCPOL3B_ENOMEM;
   }
+  // an inactive term leaves its COM force untouched (as the reference does), and
+  // the kick always adds both: they start at zero
+  if (hipMemset(M.fc2, 0, sizeof(double) * n3) != hipSuccess ||
+      hipMemset(M.fc3, 0, sizeof(double) * n3) != hipSuccess) {
+    ccpol3b_md_destroy(md);
+    return CCPOL3B_EHIP;
+  }
   *out = md;
   return CCPOL3B_OK;
 }

@@ -154,7 +154,7 @@ typedef struct {
   int fs3_lds_coef;           /* 1: FS3 blocks stage the region's coefficient table in LDS;
                                  0: they read it through the vector cache (default) */
   int fs2_parts;              /* FS2 spectator-pass blocks per molecule, 1..8 (default 4) */
-  int fs2_pair_waves_per_cu;  /* FS2 pair-pass wavefronts per CU, 1..1024 (default 64) */
+  int fs2_pair_parts;         /* FS2 pair-pass blocks per molecule, 1..8 (default 2) */
   int list_waves_per_cu;      /* FS3 sub-list sweep wavefronts per CU, 1..1024 (default 32) */
 } ccpol3b_tuning;
 
