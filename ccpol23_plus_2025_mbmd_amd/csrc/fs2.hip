@@ -46,12 +46,13 @@ __device__ __forceinline__ void fs2_damp(double R, const Consts& K, double& f, d
 }
 
 // T_k and its derivatives.  sk: spectator site offsets [5][3].
-// dTdsk: dT/d(site ks), dTdRi, dTdRj: explicit COM derivatives, Tl: dT/dlambda
-// under uniform scaling of all vectors except the fixed shift g.  Inverse
-// distances use the FP64 reciprocal square root (within 1 ulp) and one
-// reciprocal of R_ij.
+// WANT_SITES: Fk[3 ks + c] += mult * dT/d(site ks)_c (the site derivatives are
+// used at once, so they need no registers beyond the current site);
+// dTdRi, dTdRj: explicit COM derivatives; Tl: dT/dlambda under uniform scaling
+// of all vectors except the fixed shift g.  Inverse distances use the FP64
+// reciprocal square root (within 1 ulp) and one reciprocal of R_ij.
 template <bool WANT_SITES, bool WANT_COM>
-__device__ __forceinline__ double fs2_T(const Fs2Geo& G, const double* __restrict__ sk, double (*dTdsk)[3],
+__device__ __forceinline__ double fs2_T(const Fs2Geo& G, const double* __restrict__ sk, double* Fk, double mult,
                                         double* dTdRi, double* dTdRj, double& Tl) {
   const double iRij = 1.0 / G.Rij;
   const double uij[3] = {G.xij[0] * iRij, G.xij[1] * iRij, G.xij[2] * iRij};
@@ -84,7 +85,7 @@ __device__ __forceinline__ double fs2_T(const Fs2Geo& G, const double* __restric
     Tl += -q * term0 + G3 * (pqi * qqi3 - pqj * qqj3);
     if constexpr (WANT_SITES)
       for (int c = 0; c < 3; ++c)
-        dTdsk[ks][c] = -dci[c] * qci3 - dcj[c] * qcj3 + dqi[c] * qqi3 + dqj[c] * qqj3;
+        Fk[ks * 3 + c] += mult * (-dci[c] * qci3 - dcj[c] * qcj3 + dqi[c] * qqi3 + dqj[c] * qqj3);
     if constexpr (WANT_COM) {
       // projector P = I - u u^T applied to dqi, dqj
       for (int c = 0; c < 3; ++c) {
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
       G.Rki = sR[a];
       G.Rkj = sqrt(G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2]);
       double dTdRi[3], dTdRj[3], Tl;
-      const double T = fs2_T<false, true>(G, D.site + (long)snb[a] * NS3 * 3, nullptr, dTdRi, dTdRj, Tl);
+      const double T = fs2_T<false, true>(G, D.site + (long)snb[a] * NS3 * 3, nullptr, 0.0, dTdRi, dTdRj, Tl);
       const double fki = sF[a], fpki = sFp[a];
       double fkj, fpkj;
       fs2_damp(G.Rkj, K, fkj, fpkj);
@@ -393,14 +394,12 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
       G.Rkj = sqrt(G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2]);
       const long pr = (long)pofs[a] + qj;
       const double S = D.s2pair[2 * pr], Sl = D.s2pair[2 * pr + 1];
-      double dTdsk[NS2][3], Tl;
-      const double T = fs2_T<true, false>(G, skp, dTdsk, nullptr, nullptr, Tl);
       const double fki = sF[a], fpki = sFp[a];
       double fkj, fpkj;
       fs2_damp(G.Rkj, K, fkj, fpkj);
       const double ff = fki * fkj, ffS = ff * S;
-      for (int s2 = 0; s2 < NS2; ++s2)
-        for (int c = 0; c < 3; ++c) Fk[s2 * 3 + c] += -ffS * dTdsk[s2][c];
+      double Tl;
+      const double T = fs2_T<true, false>(G, skp, Fk, -ffS, nullptr, nullptr, Tl);
       const double ci = S * T * fpki * fkj / G.Rki, cj = S * T * fki * fpkj / G.Rkj;
       for (int c = 0; c < 3; ++c) Fk[NS2 * 3 + c] += -(ci * G.xki[c] + cj * G.xkj[c]);
       E += ff * S * T;
