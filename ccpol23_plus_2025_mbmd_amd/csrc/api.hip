@@ -19,11 +19,9 @@ namespace ccpol3b {
 __global__ void k_prep(int, int, const double*, const double*, const double*, const double*,
                        const double*, const double*, Dev);
 __global__ void k_neighbours(Dev, Consts);
-__global__ void k_fs3_list(Dev, Consts, int);
+template <int mode> __global__ void k_fs3_list(Dev, Consts);
 __global__ void k_fs2_items(Dev, Consts, int);
 __global__ void k_scan(Dev, int, int, int);
-__global__ void k_box_ranges(Dev, int, int);
-__global__ void k_fs3_geo(Dev, Consts);
 __global__ void k_fs3_eval_f64(Dev, Consts, const double4*, const double*);
 __global__ void k_fs3_eval_f32(Dev, Consts, const double4*, const float*);
 __global__ void k_fs3_eval_f64_lds(Dev, Consts, const double4*, const double*);
@@ -39,8 +37,7 @@ __global__ void k_fs2_pair_pass(Dev, Consts);
 __global__ void k_fs2_spectator_pass(Dev, Consts);
 size_t fs2_lds_bytes(int cap_nb);
 size_t fs2_pair_lds_bytes(int cap_nb);
-__global__ void k_finish(Dev, int, int, int, int, double*, double*, double*, double*, double*);
-__global__ void k_energy_chunks(Dev, int, long);
+__global__ void k_epilogue(Dev, int, int, int, int, double*, double*, double*, double*, double*, long, long, long);
 __global__ void k_energy_boxes(Dev, int, int, double*);
 }  // namespace ccpol3b
 
@@ -155,6 +152,9 @@ struct ccpol3b_ctx {
   std::vector<int> h_molbox, h_boxstart;
   std::vector<double> h_boxL;
   hipEvent_t ev[9] = {};
+  // the FS3 list build runs on a second stream beside the FS2 passes
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 
 namespace {
@@ -310,68 +310,77 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   LAUNCHCHK(c, "neighbour lists");
   // sub-list sweeps (grid-stride over the sub-lists, count then fill)
   const long max_sl = std::min<long>((long)nm * D.cap_nb, (long)c->cu_count * c->tune.list_waves_per_cu);
-  if (do3) {
-    hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 0);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 1, do3, do2);
-    hipLaunchKernelGGL(k_fs3_list, dim3(waves_grid(max_sl)), dim3(256), 0, st, D, c->K, 1);
-  }
-  hipLaunchKernelGGL(k_box_ranges, dim3((nbox + 256) / 256), dim3(256), 0, st, D, do3, do2);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 3, do3, do2);
-  LAUNCHCHK(c, "lists");
+  const size_t lds = fs2_lds_bytes(D.cap_nb), lds_p = fs2_pair_lds_bytes(D.cap_nb);
+  if (do2 && std::max(lds, lds_p) > 65536)
+    return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the FS2 passes");
   EVREC(1);
+  // Two branches from here: the FS2 passes on `st`, and the FS3 list build
+  // (count, offsets with the per-box ranges, fill with the geometry) followed
+  // by the FS3 evaluation on the context's second stream.  Neither reads what
+  // the other writes (both add forces into acc with atomics); `st` waits for
+  // the FS3 branch before the epilogue.  Under capture the fork and join
+  // become graph edges.  (With the branches the other way round, the FS3
+  // kernel shares the chip with the FS2 passes for longer and a call of 512
+  // waters takes 3 % more.)
+#define EVREC_SIDE(i) \
+  if (!capture) HIPCHK(c, hipEventRecord(c->ev[i], c->side))
+  HIPCHK(c, hipEventRecord(c->fork, st));
+  HIPCHK(c, hipStreamWaitEvent(c->side, c->fork, 0));
   if (do2) {
     // pair pass (S_ij, pair-owned forces) before the spectator pass that reads S_ij
-    const size_t lds = fs2_lds_bytes(D.cap_nb), lds_p = fs2_pair_lds_bytes(D.cap_nb);
-    if (std::max(lds, lds_p) > 65536)
-      return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the FS2 passes");
     hipLaunchKernelGGL(k_fs2_pair_pass, dim3(nm * D.fs2_pair_parts), dim3(256), lds_p, st, D, c->K);
     hipLaunchKernelGGL(k_fs2_spectator_pass, dim3(nm * D.fs2_parts), dim3(256), lds, st, D, c->K);
     LAUNCHCHK(c, "FS2");
   }
   EVREC(2);
+  hipStream_t s3 = c->side;
+  if (do3) hipLaunchKernelGGL(k_fs3_list<0>, dim3(waves_grid(max_sl)), dim3(256), 0, s3, D, c->K);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s3, D, 1, do3, do2);
+  if (do3) hipLaunchKernelGGL(k_fs3_list<1>, dim3(waves_grid(max_sl)), dim3(256), 0, s3, D, c->K);
+  LAUNCHCHK(c, "FS3 lists");
   const long cap3 = D.cap3;
+  EVREC_SIDE(8);
   if (do3) {
-    hipLaunchKernelGGL(k_fs3_geo, dim3((cap3 + 255) / 256), dim3(256), 0, st, D, c->K);
-    EVREC(8);
     const int grid = (int)cap3;  // one block per triplet; blocks past the list exit at once
     if (phi) {
       if (c->precision == CCPOL3B_MIXED)
-        hipLaunchKernelGGL(k_fs3_phi_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+        hipLaunchKernelGGL(k_fs3_phi_f32, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
                            (const float*)c->d_coef32, c->d_phitrip);
       else
-        hipLaunchKernelGGL(k_fs3_phi_f64, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+        hipLaunchKernelGGL(k_fs3_phi_f64, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
                            (const double*)c->d_coef64, c->d_phitrip);
     } else if (c->precision == CCPOL3B_MIXED) {
       if (c->tune.fs3_lds_coef)
-        hipLaunchKernelGGL(k_fs3_eval_f32_lds, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+        hipLaunchKernelGGL(k_fs3_eval_f32_lds, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
                            (const float*)c->d_coef32);
       else
-        hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+        hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
                            (const float*)c->d_coef32);
     } else {
       if (c->tune.fs3_lds_coef)
-        hipLaunchKernelGGL(k_fs3_eval_f64_lds, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+        hipLaunchKernelGGL(k_fs3_eval_f64_lds, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
                            (const double*)c->d_coef64);
       else
-        hipLaunchKernelGGL(k_fs3_eval_f64, dim3(grid), dim3(256), 0, st, D, c->K, c->d_pairpar,
+        hipLaunchKernelGGL(k_fs3_eval_f64, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
                            (const double*)c->d_coef64);
     }
     LAUNCHCHK(c, "k_fs3_eval");
-  } else {
-    EVREC(8);
   }
   if (phi) {
-    hipLaunchKernelGGL(k_phi_boxes, dim3((2 * NLIN3 + 255) / 256, nbox), dim3(256), 0, st, D,
+    hipLaunchKernelGGL(k_phi_boxes, dim3((2 * NLIN3 + 255) / 256, nbox), dim3(256), 0, s3, D,
                        (const double*)c->d_phitrip, phi);
     LAUNCHCHK(c, "k_phi_boxes");
   }
-  EVREC(3);
-  const long nfin = (long)nm * ACC_W;
-  hipLaunchKernelGGL(k_finish, dim3((nfin + 255) / 256), dim3(256), 0, st, D, ld, do3 ? NS3 : (do2 ? NS2 : 0),
-                     do2, do3, fxs, fys, fzs, fcom2, fcom3);
-  const long nch3 = cap3 / RED_CHUNK + nbox + 1, nch2 = (long)nm * D.fs2_parts / RED_CHUNK + nbox + 1;
-  if (do3) hipLaunchKernelGGL(k_energy_chunks, dim3(nch3), dim3(256), 0, st, D, 3, nch3);
-  if (do2) hipLaunchKernelGGL(k_energy_chunks, dim3(nch2), dim3(256), 0, st, D, 2, nch2);
+  EVREC_SIDE(3);
+  HIPCHK(c, hipEventRecord(c->join, s3));
+  HIPCHK(c, hipStreamWaitEvent(st, c->join, 0));
+#undef EVREC_SIDE
+  // write-back and the first level of the energy sums (one launch), then the per-box sums
+  const long nfin = ((long)nm * ACC_W + 255) / 256;
+  const long nch3 = do3 ? cap3 / RED_CHUNK + nbox + 1 : 0;
+  const long nch2 = do2 ? (long)nm * D.fs2_parts / RED_CHUNK + nbox + 1 : 0;
+  hipLaunchKernelGGL(k_epilogue, dim3(nfin + nch3 + nch2), dim3(256), 0, st, D, ld, do3 ? NS3 : (do2 ? NS2 : 0),
+                     do2, do3, fxs, fys, fzs, fcom2, fcom3, nfin, nch3, nch2);
   hipLaunchKernelGGL(k_energy_boxes, dim3(nbox), dim3(64), 0, st, D, do2, do3, energies);
   LAUNCHCHK(c, "finish");
   EVREC(4);
@@ -421,8 +430,9 @@ void finish_stats(ccpol3b_ctx* c) {
   c->stats.n_fs2_items = c->s2 ? tot[T_NI2] : 0;
   c->stats.ms_lists = event_ms(c->ev[0], c->ev[1]);
   c->stats.ms_fs2 = event_ms(c->ev[1], c->ev[2]);
-  c->stats.ms_fs3 = event_ms(c->ev[2], c->ev[3]);
-  c->stats.ms_reduce = event_ms(c->ev[3], c->ev[4]);
+  c->stats.ms_fs3 = event_ms(c->ev[1], c->ev[3]);
+  // the epilogue starts when the later of the two branches has ended
+  c->stats.ms_reduce = std::min(event_ms(c->ev[2], c->ev[4]), event_ms(c->ev[3], c->ev[4]));
   c->stats.ms_total = event_ms(c->ev[0], c->ev[4]);
   c->stats.ms_fs3_eval = c->stats.n_fs3 > 0 ? event_ms(c->ev[8], c->ev[3]) : 0.0;
 }
@@ -621,6 +631,12 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
       ccpol3b_destroy(c);
       return CCPOL3B_EHIP;
     }
+  if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+    ccpol3b_destroy(c);
+    return CCPOL3B_EHIP;
+  }
   *out = c;
   return CCPOL3B_OK;
 }
@@ -641,6 +657,9 @@ void ccpol3b_destroy(ccpol3b_ctx* c) {
   if (c->h_totals) (void)hipHostFree(c->h_totals);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   delete c;
 }
 

@@ -52,8 +52,8 @@ struct Consts {
   double rmin, rmax;  // switch window in bohr, :774-775
 };
 
-// Geometry of one FS3 triplet, computed once per triplet by k_fs3_geo (one
-// thread per triplet) and read by every thread of its evaluation block.
+// Geometry of one FS3 triplet, computed once per triplet by the lane of the
+// list fill that emits it, and read by every thread of its evaluation block.
 struct TriGeo {
   double c[3][3];      // COM vectors x_ij, x_ik, x_jk (bohr), as the reference forms them (:862-884)
   double R[3];         // |x_ij|, |x_ik|, |x_jk|

@@ -92,10 +92,6 @@ __device__ __forceinline__ double seg_sum(double v, int key, int lane) {
   return v;
 }
 
-template <typename T> __device__ __forceinline__ T t_exp(T x);
-template <> __device__ __forceinline__ double t_exp<double>(double x) { return exp(x); }
-template <> __device__ __forceinline__ float t_exp<float>(float x) { return __expf(x); }
-
 // x**delta for the damping exponent: exponentiation by squaring when delta is
 // a small whole number (10 in both shipped CONTROL files), pow otherwise.
 __device__ __forceinline__ double pow_delta(double x, const Consts& K) {
@@ -109,5 +105,45 @@ __device__ __forceinline__ double pow_delta(double x, const Consts& K) {
   }
   return pow(x, K.delta33);
 }
+
+// Geometry of an FS3 triplet from its COM vectors x_ij, x_ik and the derived
+// x_jk (:862-884): distances, R_avg (:894-898), the damping f_d and its
+// derivative (:900-904), and the SR/LR weights of the cubic switch (:1420-1423).
+__device__ __forceinline__ void tri_geometry(const double* cij, const double* cik, const double* cjk,
+                                             const Consts& K, int mi, int mj, int mk, TriGeo& g) {
+  for (int c = 0; c < 3; ++c) {
+    g.c[0][c] = cij[c];
+    g.c[1][c] = cik[c];
+    g.c[2][c] = cjk[c];
+  }
+  for (int b = 0; b < 3; ++b) g.R[b] = sqrt(g.c[b][0] * g.c[b][0] + g.c[b][1] * g.c[b][1] + g.c[b][2] * g.c[b][2]);
+  const double Ravg = (g.R[0] + g.R[2] + g.R[1]) / 3.0;
+  const double pom = exp(K.alp3 * (Ravg - K.r0dmp3));
+  const double pp = 1.0 / (1.0 + pom);
+  const double fd = pow_delta(pp, K);
+  g.Ravg = Ravg;
+  g.fd = fd;
+  g.dfd = -K.alp3 * K.delta33 * pom * pp * fd;  // d f_d / d R_avg
+  g.w[0] = 1.0; g.wp[0] = 0.0; g.w[1] = 0.0; g.wp[1] = 0.0;
+  if (Ravg <= K.rmin) {
+    g.nreg = 1; g.reg0 = 0;
+  } else if (Ravg >= K.rmax) {
+    g.nreg = 1; g.reg0 = 1;
+  } else {
+    const double span = K.rmax - K.rmin;
+    const double c = (Ravg - K.rmin) / span;
+    const double sm = 1.0 + 2.0 * c * c * c - 3.0 * c * c;
+    const double dsm = (6.0 * c * c - 6.0 * c) / span;
+    g.nreg = 2; g.reg0 = 0;
+    g.w[0] = sm; g.wp[0] = dsm;
+    g.w[1] = 1.0 - sm; g.wp[1] = -dsm;
+  }
+  g.m[0] = mi; g.m[1] = mj; g.m[2] = mk;
+  g.pad = 0;
+}
+
+template <typename T> __device__ __forceinline__ T t_exp(T x);
+template <> __device__ __forceinline__ double t_exp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float t_exp<float>(float x) { return __expf(x); }
 
 }  // namespace ccpol3b

@@ -29,76 +29,6 @@ __constant__ int c_tile1[9] = {-1, 2, 4, 6, 8, 10, 12, 14, 16};
 __constant__ int c_tilet[9] = {0, 1, 2, 3, 3, 4, 4, 5, 5};
 
 // ====================================================================== FS3
-// pairpar: [2][289] (beta, gamma, r0, -) for SR and LR; coef: [2][NCOEF] dense
-// [tx][ty][tz][p][q][s].
-// Geometry of triplet t (COM vectors, R_avg, damping, region weights).
-__device__ void fs3_geometry(const Dev& D, const Consts& K, long t, TriGeo& g) {
-  const int4 tr = D.tri[t];
-  const int mi = tr.x, mj = tr.y, mk = tr.z;
-  const double L = D.boxL[D.mol_box[mi]];
-  const double pi_[3] = {D.cx[mi], D.cy[mi], D.cz[mi]};
-  const double pj_[3] = {D.cx[mj], D.cy[mj], D.cz[mj]};
-  const double pk_[3] = {D.cx[mk], D.cy[mk], D.cz[mk]};
-  for (int c = 0; c < 3; ++c) {
-    const double dij = pi_[c] - pj_[c], dik = pi_[c] - pk_[c], djk = pj_[c] - pk_[c];
-    if (L > 0.0) {
-      const double nij = round(dij / L), nik = round(dik / L);
-      g.c[0][c] = dij - L * nij;
-      g.c[1][c] = dik - L * nik;
-      g.c[2][c] = djk + L * nij - L * nik;
-    } else {
-      g.c[0][c] = dij;
-      g.c[1][c] = dik;
-      g.c[2][c] = djk;
-    }
-  }
-  for (int b = 0; b < 3; ++b) g.R[b] = sqrt(g.c[b][0] * g.c[b][0] + g.c[b][1] * g.c[b][1] + g.c[b][2] * g.c[b][2]);
-  const double Ravg = (g.R[0] + g.R[2] + g.R[1]) / 3.0;  // (:894-898)
-  const double pom = exp(K.alp3 * (Ravg - K.r0dmp3));
-  const double pp = 1.0 / (1.0 + pom);
-  const double fd = pow_delta(pp, K);
-  g.Ravg = Ravg;
-  g.fd = fd;
-  g.dfd = -K.alp3 * K.delta33 * pom * pp * fd;  // d f_d / d R_avg (:900-904)
-  g.w[0] = 1.0; g.wp[0] = 0.0; g.w[1] = 0.0; g.wp[1] = 0.0;
-  if (Ravg <= K.rmin) {
-    g.nreg = 1; g.reg0 = 0;
-  } else if (Ravg >= K.rmax) {
-    g.nreg = 1; g.reg0 = 1;
-  } else {  // cubic switch (:1420-1423)
-    const double span = K.rmax - K.rmin;
-    const double c = (Ravg - K.rmin) / span;
-    const double sm = 1.0 + 2.0 * c * c * c - 3.0 * c * c;
-    const double dsm = (6.0 * c * c - 6.0 * c) / span;
-    g.nreg = 2; g.reg0 = 0;
-    g.w[0] = sm; g.wp[0] = dsm;
-    g.w[1] = 1.0 - sm; g.wp[1] = -dsm;
-  }
-  g.m[0] = mi; g.m[1] = mj; g.m[2] = mk;
-  g.pad = 0;
-}
-
-__global__ __launch_bounds__(256) void k_fs3_geo(Dev D, Consts K) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  // the list holds min(n3, cap3) triplets (an overflowing call is repeated)
-  const long n3 = min((long)D.totals[T_N3], D.cap3);
-  if (t - (threadIdx.x & 63) >= n3) return;  // whole waves past the end
-  int hist = -1;
-  if (t < n3) {
-    TriGeo g;
-    fs3_geometry(D, K, t, g);
-    D.geo3[t] = g;
-    hist = g.nreg == 2 ? 1 : 2 * g.reg0;
-  }
-  // region histogram: one atomic per wave and region (same-address atomics
-  // from every triplet would serialise)
-  const int lane = threadIdx.x & 63;
-  for (int h = 0; h < 3; ++h) {
-    const int n = __popcll(__ballot(hist == h));
-    if (lane == 0 && n > 0) atomicAdd(D.totals + T_HIST + h, n);
-  }
-}
-
 // Mixed-precision helpers: r from the FP64 r^2 with a FP32 square root, and a
 // FP32 reciprocal; the FP64 variant keeps sqrt and the IEEE division.
 template <typename T> __device__ __forceinline__ double t_dist(double r2);
@@ -137,6 +67,8 @@ __constant__ int c_nz[NTYPE] = {1, 2, 2, 4, 4, 4};
 // 364 SR then 364 LR) to phi_trip[t][728].
 // CSM: stage the region's coefficient table in LDS; otherwise the tile threads
 // read it through the vector cache (less LDS per block, more blocks per CU).
+// pairpar: [2][289] (beta, gamma, r0, -) for SR and LR; coef: [2][NCOEF] dense
+// [tx][ty][tz][p][q][s].
 template <typename T, bool TR, bool PHI, bool CSM>
 __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __restrict__ pairpar,
                                               const T* __restrict__ coef, double* __restrict__ phi_trip) {
@@ -426,11 +358,11 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
 }
 
 // Accumulators (hartree/bohr) -> the caller's arrays in 10 J/mol/A: site
-// forces added (:574-585, 1964-1977), COM forces overwritten.
-__global__ void k_finish(Dev D, int ld, int nsite, int do2, int do3, double* fxs, double* fys,
-                         double* fzs, double* fcom2, double* fcom3) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)D.nm * ACC_W || lists_overflowed(D)) return;
+// forces added (:574-585, 1964-1977), COM forces overwritten.  Thread per
+// (molecule, accumulator) t.
+__device__ __forceinline__ void finish_one(const Dev& D, long t, int ld, int nsite, int do2, int do3, double* fxs,
+                                           double* fys, double* fzs, double* fcom2, double* fcom3) {
+  if (t >= (long)D.nm * ACC_W) return;
   const int m = (int)(t / ACC_W), a = (int)(t - (long)m * ACC_W);
   const double conv = H2KCAL * 418.4 / BOHR2A;
   const double v = D.acc[t] * conv;
@@ -448,11 +380,11 @@ __global__ void k_finish(Dev D, int ld, int nsite, int do2, int do3, double* fxs
 }
 
 // Energy/virial reduction, level 1: one 256-thread block per box-aligned chunk
-// of RED_CHUNK entries (per-triplet FS3 or per-item FS2 values), fixed tree.
-__global__ __launch_bounds__(256) void k_energy_chunks(Dev D, int which, long nchunks) {
+// q of RED_CHUNK entries (per-triplet FS3 or per-(spectator, part) FS2
+// values), fixed tree.  Called by a whole block.
+__device__ __forceinline__ void energy_chunk(const Dev& D, int which, long q) {
   __shared__ double red[2][4];
-  const long q = blockIdx.x;
-  if (q >= nchunks || lists_overflowed(D) || q >= (which == 3 ? D.box_c3 : D.box_c2)[D.nbox]) return;
+  if (q >= (which == 3 ? D.box_c3 : D.box_c2)[D.nbox]) return;
   const int* bt = which == 3 ? D.box_t3 : D.box_t2;
   const int* bc = which == 3 ? D.box_c3 : D.box_c2;
   const double* ew = which == 3 ? D.ew3 : D.ew2;
@@ -481,6 +413,22 @@ __global__ __launch_bounds__(256) void k_energy_chunks(Dev D, int which, long nc
     out[2 * q] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
     out[2 * q + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
   }
+}
+
+// The call's epilogue in one launch: blocks [0, nfin) write the accumulators
+// back, the next nch3 blocks reduce FS3 energy chunks and the last nch2 blocks
+// FS2 energy chunks.  Nothing is written when a list overflowed.
+__global__ __launch_bounds__(256) void k_epilogue(Dev D, int ld, int nsite, int do2, int do3, double* fxs,
+                                                  double* fys, double* fzs, double* fcom2, double* fcom3,
+                                                  long nfin, long nch3, long nch2) {
+  if (lists_overflowed(D)) return;
+  const long b = blockIdx.x;
+  if (b < nfin)
+    finish_one(D, b * blockDim.x + threadIdx.x, ld, nsite, do2, do3, fxs, fys, fzs, fcom2, fcom3);
+  else if (b < nfin + nch3)
+    energy_chunk(D, 3, b - nfin);
+  else if (b < nfin + nch3 + nch2)
+    energy_chunk(D, 2, b - nfin - nch3);
 }
 
 // Level 2: one wavefront per box; lane l sums chunks l, l+64, ... of its box in

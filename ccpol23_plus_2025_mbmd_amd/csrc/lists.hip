@@ -127,8 +127,14 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D, Consts K) {
 // candidates k are the later entries of nb3(i) (so i-j and i-k passed the
 // minimum-image test); the j-k test uses the reference's derived vector
 // x_jk = dx_jk + L*nint(dx_ij/L) - L*nint(dx_ik/L) (:882-884).
-// mode 0 counts into cnt3s[s]; mode 1 writes at off3s[s] in ascending k.
-__global__ __launch_bounds__(256) void k_fs3_list(Dev D, Consts K, int mode) {
+// mode 0 counts into cnt3s[s]; mode 1 writes at off3s[s] in ascending k, and
+// the lane that emits a triplet also writes its geometry (TriGeo) and counts
+// its region in the SR / switch / LR histogram.
+template <int mode>
+__global__ __launch_bounds__(256) void k_fs3_list(Dev D, Consts K) {
+  __shared__ int shist[3];
+  if (threadIdx.x < 3) shist[threadIdx.x] = 0;
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const long nsl = D.totals[T_SL3];
   const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
@@ -147,36 +153,57 @@ __global__ __launch_bounds__(256) void k_fs3_list(Dev D, Consts K, int mode) {
     nyij = round((yi - yj) / L);
     nzij = round((zi - zj) / L);
   }
+  const double cij[3] = {(xi - xj) - L * nxij, (yi - yj) - L * nyij, (zi - zj) - L * nzij};
   const long out = mode ? D.off3s[s] : 0;
   int count = 0;
   for (int bb = a + 1; bb < n; bb += 64) {
     const int q = bb + lane;
     bool ok = false;
     int k = 0;
+    double cik[3] = {0.0, 0.0, 0.0}, cjk[3] = {0.0, 0.0, 0.0};
     if (q < n) {
       k = nb[q];
       const double xk = D.cx[k], yk = D.cy[k], zk = D.cz[k];
       double xjk = xj - xk, yjk = yj - yk, zjk = zj - zk;
+      double nxik = 0.0, nyik = 0.0, nzik = 0.0;
       if (L > 0.0) {
-        const double nxik = round((xi - xk) / L);
-        const double nyik = round((yi - yk) / L);
-        const double nzik = round((zi - zk) / L);
+        nxik = round((xi - xk) / L);
+        nyik = round((yi - yk) / L);
+        nzik = round((zi - zk) / L);
         xjk = xjk + L * nxij - L * nxik;
         yjk = yjk + L * nyij - L * nyik;
         zjk = zjk + L * nzij - L * nzik;
       }
       ok = sq3(xjk, yjk, zjk) < K.rc2_3;
+      cik[0] = (xi - xk) - L * nxik;
+      cik[1] = (yi - yk) - L * nyik;
+      cik[2] = (zi - zk) - L * nzik;
+      cjk[0] = xjk;
+      cjk[1] = yjk;
+      cjk[2] = zjk;
     }
     const uint64_t m = __ballot(ok);
-    if (mode && ok) {
+    if (mode == 1 && ok) {
       const long p = out + count + __popcll(m & lanemask_lt());
-      if (p < D.cap3) D.tri[p] = make_int4(i, j, k, 0);
+      if (p < D.cap3) {
+        D.tri[p] = make_int4(i, j, k, 0);
+        TriGeo g;
+        tri_geometry(cij, cik, cjk, K, i, j, k, g);
+        D.geo3[p] = g;
+        atomicAdd(shist + (g.nreg == 2 ? 1 : 2 * g.reg0), 1);
+      }
     }
     count += __popcll(m);
   }
   if (mode == 0 && lane == 0) D.cnt3s[s] = count;
   }
+  // one global atomic per block and region (same-address atomics from every
+  // triplet would serialise)
+  __syncthreads();
+  if (mode && threadIdx.x < 3 && shist[threadIdx.x] > 0) atomicAdd(D.totals + T_HIST + threadIdx.x, shist[threadIdx.x]);
 }
+template __global__ void k_fs3_list<0>(Dev, Consts);
+template __global__ void k_fs3_list<1>(Dev, Consts);
 
 // ------------------------------------------------------------------ FS2 items
 // Ordered items (k; i<j): one wavefront per sub-list (k; i = nb2(k)[a]); the
@@ -278,10 +305,36 @@ __device__ void block_exclusive_scan(const int* cnt, int* off, long n, int* tota
   __syncthreads();
 }
 
+// Triplet range of each box (the list is sorted by its first molecule, and
+// boxes are contiguous molecule ranges) and the number of reduction chunks of
+// each box, for FS3 triplets and for the FS2 energies (fs2_parts entries per
+// molecule); the chunk counts go to box_c3/box_c2 and are scanned in place.
+// Called by a whole block after the FS3 offsets are final.
+__device__ void box_ranges(const Dev& D, int do3, int do2) {
+  for (int b = threadIdx.x; b <= D.nbox; b += blockDim.x) {
+    const int m = D.box_start[b];
+    const int t3 = do3 ? (m < D.nm ? D.off3s[D.sl3_off[m]] : D.totals[T_N3]) : 0;
+    const int t2 = do2 ? m * D.fs2_parts : 0;
+    D.box_t3[b] = t3;
+    D.box_t2[b] = t2;
+    int c3 = 0, c2 = 0;
+    if (b < D.nbox) {
+      const int m1 = D.box_start[b + 1];
+      const int e3 = do3 ? (m1 < D.nm ? D.off3s[D.sl3_off[m1]] : D.totals[T_N3]) : 0;
+      const int e2 = do2 ? m1 * D.fs2_parts : 0;
+      c3 = (e3 - t3 + RED_CHUNK - 1) / RED_CHUNK;
+      c2 = (e2 - t2 + RED_CHUNK - 1) / RED_CHUNK;
+    }
+    D.box_c3[b] = c3;
+    D.box_c2[b] = c2;
+  }
+  __syncthreads();
+}
+
 // phase 0: per-molecule counts -> sub-list and pair offsets
-// phase 1: per-sub-list candidate counts -> FS3 triplet offsets
+// phase 1: per-sub-list candidate counts -> FS3 triplet offsets; then the
+//          per-box triplet ranges and the first reduction chunk of each box
 // phase 2: per-sub-list candidate counts -> FS2 item offsets (list export only)
-// phase 3: per-box reduction chunk counts -> chunk offsets
 __global__ __launch_bounds__(1024) void k_scan(Dev D, int phase, int do3, int do2) {
   if (phase == 0) {
     if (do3) block_exclusive_scan(D.sl3_cnt, D.sl3_off, D.nm, D.totals + T_SL3);
@@ -291,39 +344,12 @@ __global__ __launch_bounds__(1024) void k_scan(Dev D, int phase, int do3, int do
     }
   } else if (phase == 1) {
     if (do3) block_exclusive_scan(D.cnt3s, D.off3s, D.totals[T_SL3], D.totals + T_N3);
-  } else if (phase == 2) {
-    if (do2) block_exclusive_scan(D.cnt2s, D.off2s, D.totals[T_SL2], D.totals + T_NI2L);
-  } else {
-    // reduction chunk counts per box -> first chunk of each box (in place)
+    box_ranges(D, do3, do2);
     block_exclusive_scan(D.box_c3, D.box_c3, D.nbox, D.totals + T_NCH3);
     block_exclusive_scan(D.box_c2, D.box_c2, D.nbox, D.totals + T_NCH2);
+  } else if (phase == 2) {
+    if (do2) block_exclusive_scan(D.cnt2s, D.off2s, D.totals[T_SL2], D.totals + T_NI2L);
   }
-}
-
-// Triplet range of each box (the list is sorted by its first molecule, and
-// boxes are contiguous molecule ranges) and the number of reduction chunks of
-// each box, for FS3 triplets and for the FS2 energies (fs2_parts entries per
-// molecule).
-// Thread per box b (and b = nbox for the ends): range starts, and the chunk
-// counts of b in box_c3/box_c2, which k_scan phase 3 turns into offsets.
-__global__ void k_box_ranges(Dev D, int do3, int do2) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b > D.nbox) return;
-  const int m = D.box_start[b];
-  const int t3 = do3 ? (m < D.nm ? D.off3s[D.sl3_off[m]] : D.totals[T_N3]) : 0;
-  const int t2 = do2 ? m * D.fs2_parts : 0;
-  D.box_t3[b] = t3;
-  D.box_t2[b] = t2;
-  int c3 = 0, c2 = 0;
-  if (b < D.nbox) {
-    const int m1 = D.box_start[b + 1];
-    const int e3 = do3 ? (m1 < D.nm ? D.off3s[D.sl3_off[m1]] : D.totals[T_N3]) : 0;
-    const int e2 = do2 ? m1 * D.fs2_parts : 0;
-    c3 = (e3 - t3 + RED_CHUNK - 1) / RED_CHUNK;
-    c2 = (e2 - t2 + RED_CHUNK - 1) / RED_CHUNK;
-  }
-  D.box_c3[b] = c3;
-  D.box_c2[b] = c2;
 }
 
 }  // namespace ccpol3b

@@ -71,7 +71,10 @@ typedef struct {
   long n_fs2_items;                  /* ordered FS2 items (k; i<j) accepted */
   long n_fs2_pairs;                  /* unordered FS2 pairs within rcut3 */
   double ms_total;                   /* device time of the whole evaluation (HIP events) */
-  double ms_lists, ms_fs2, ms_fs3, ms_reduce;
+  double ms_lists;                   /* conversion, neighbour lists and their offsets */
+  double ms_fs2, ms_fs3;             /* the two concurrent branches after the lists: the FS2 passes,
+                                        and the FS3 list build plus the FS3 evaluation */
+  double ms_reduce;                  /* from the later branch's end: write-back and energy sums */
   double ms_h2d, ms_d2h;             /* host-pointer entry point only */
   double ms_fs3_eval;                /* the FS3 evaluation kernel alone (inside ms_fs3) */
 } ccpol3b_stats;
