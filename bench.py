@@ -163,21 +163,33 @@ def snapshot_errors(p, tb_mix, tb_64, n, count=50):
     (a three-body-only NVE trajectory collapses within ~20 fs, since the
     short-range repulsion is the host's two-body term)."""
     from ccpol23_plus_2025_mbmd_amd.geometry import lattice_box
-    fe, ce, ee = [], [], []
+    def rel(a, b):
+        return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+    fe, ce, me, te, ee = [], [], [], [], []
     for s in range(count):
         box = lattice_box(n, seed=10_000 + s)
         r64, rmx = tb_64.evaluate(box), tb_mix.evaluate(box)
-        f64 = np.stack([r64[k][:, :17] for k in ("fxs", "fys", "fzs")])
-        fmx = np.stack([rmx[k][:, :17] for k in ("fxs", "fys", "fzs")])
-        fe.append(float(np.sqrt(((fmx - f64) ** 2).sum() / (f64 ** 2).sum())))
+        f64 = np.stack([r64[k][:, :17] for k in ("fxs", "fys", "fzs")], -1)
+        fmx = np.stack([rmx[k][:, :17] for k in ("fxs", "fys", "fzs")], -1)
+        fe.append(rel(fmx, f64))
         c64 = np.stack([r64[k] for k in ("fxfs2", "fyfs2", "fzfs2", "fxfs3", "fyfs3", "fzfs3")])
         cmx = np.stack([rmx[k] for k in ("fxfs2", "fyfs2", "fzfs2", "fxfs3", "fyfs3", "fzfs3")])
-        ce.append(float(np.sqrt(((cmx - c64) ** 2).sum() / (c64 ** 2).sum())))
+        ce.append(rel(cmx, c64))
+        # what an integrator uses: molecular force (sites + both COM terms) and torque about the COM
+        m64 = f64.sum(1) + c64[:3].T + c64[3:].T
+        mmx = fmx.sum(1) + cmx[:3].T + cmx[3:].T
+        me.append(rel(mmx, m64))
+        sites = box.site[:, :17]
+        te.append(rel(np.cross(sites, fmx).sum(1), np.cross(sites, f64).sum(1)))
         e64 = r64["qfs2"] + r64["qfs3"]
         ee.append(float(abs(rmx["qfs2"] + rmx["qfs3"] - e64) / abs(e64)))
-    log(f"snapshots: site-force rel RMS max {max(fe):.2e}, COM {max(ce):.2e}, energy rel max {max(ee):.2e}")
+    log(f"snapshots: site-force rel RMS max {max(fe):.2e}, COM {max(ce):.2e}, molecular force {max(me):.2e}, "
+        f"torque {max(te):.2e}, energy rel max {max(ee):.2e}")
     return {"snapshots": count, "site_force_rel_rms_max": max(fe), "site_force_rel_rms_mean": float(np.mean(fe)),
-            "com_force_rel_rms_max": max(ce), "energy_rel_max": max(ee), "energy_rel_mean": float(np.mean(ee))}
+            "com_force_rel_rms_max": max(ce), "molecular_force_rel_rms_max": max(me),
+            "molecular_force_rel_rms_mean": float(np.mean(me)), "torque_rel_rms_max": max(te),
+            "energy_rel_max": max(ee), "energy_rel_mean": float(np.mean(ee))}
 
 
 def read_traffic():

@@ -129,7 +129,7 @@ struct ccpol3b_ctx {
   // parameters on the device
   double4* d_pairpar = nullptr;  // [2][289]
   double* d_coef64 = nullptr;    // [2][NCOEF]
-  float* d_coef32 = nullptr;
+  float* d_coef32s = nullptr;    // [2][NCOEF] FP32, shifted pair-vector basis of the mixed kernels
   double* d_phitrip = nullptr;   // [cap_phi][728] per-triplet coefficient-gradient rows
   long cap_phi = 0;
   ccpol3b_tuning tune{0, 4, 2, 32};
@@ -345,17 +345,17 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     if (phi) {
       if (c->precision == CCPOL3B_MIXED)
         hipLaunchKernelGGL(k_fs3_phi_f32, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
-                           (const float*)c->d_coef32, c->d_phitrip);
+                           (const float*)c->d_coef32s, c->d_phitrip);
       else
         hipLaunchKernelGGL(k_fs3_phi_f64, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
                            (const double*)c->d_coef64, c->d_phitrip);
     } else if (c->precision == CCPOL3B_MIXED) {
       if (c->tune.fs3_lds_coef)
         hipLaunchKernelGGL(k_fs3_eval_f32_lds, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
-                           (const float*)c->d_coef32);
+                           (const float*)c->d_coef32s);
       else
         hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
-                           (const float*)c->d_coef32);
+                           (const float*)c->d_coef32s);
     } else {
       if (c->tune.fs3_lds_coef)
         hipLaunchKernelGGL(k_fs3_eval_f64_lds, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
@@ -451,20 +451,31 @@ int check_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* bo
 void set_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len);
 int clone_ctx(const ccpol3b_ctx* c, ccpol3b_ctx** out);
 
-// Dense coefficient tables C[X][tx][ty][tz][p][q][s] = cs_X[orbit], FP64 and FP32.
+// Dense coefficient tables C[X][tx][ty][tz][p][q][s] = cs_X[orbit] (FP64), and
+// the FP32 table in the shifted pair-vector basis of the mixed kernels: with v = (e, e r) = T v', v' = (e, e (r - c)), T = [[1, 0], [c, 1]],
+// the energy sum_pqs C v_p v_q v_s equals sum C' v'_p v'_q v'_s with
+// C'[p'q's'] = sum_pqs C[pqs] T[p][p'] T[q][q'] T[s][s'] (each type block of 8).
 int upload_coefficients(ccpol3b_ctx* c, const double* cs_sr, const double* cs_lr) {
   std::vector<double> c64(2 * NCOEF);
-  std::vector<float> c32(2 * NCOEF);
+  std::vector<float> c32s(2 * NCOEF);
   const std::vector<int>& dob = dense_orbit();
   for (int X = 0; X < 2; ++X) {
     const double* cs = X ? cs_lr : cs_sr;
     for (int d = 0; d < NCOEF; ++d) {
       c64[(size_t)X * NCOEF + d] = cs[dob[d]];
-      c32[(size_t)X * NCOEF + d] = (float)cs[dob[d]];
+    }
+    for (int blk = 0; blk < NCOEF / 8; ++blk) {
+      double b[8];
+      for (int a = 0; a < 8; ++a) b[a] = c64[(size_t)X * NCOEF + blk * 8 + a];
+      // index (p q s) = 4 p + 2 q + s; along each axis: new[0] = old[0] + c old[1], new[1] = old[1]
+      for (int stride : {4, 2, 1})
+        for (int a = 0; a < 8; ++a)
+          if ((a & stride) == 0) b[a] += MIXED_VSHIFT * b[a + stride];
+      for (int a = 0; a < 8; ++a) c32s[(size_t)X * NCOEF + blk * 8 + a] = (float)b[a];
     }
   }
   if (hipMemcpy(c->d_coef64, c64.data(), sizeof(double) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_coef32, c32.data(), sizeof(float) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(c->d_coef32s, c32s.data(), sizeof(float) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess)
     return fail(c, CCPOL3B_EHIP, "coefficient upload");
   return CCPOL3B_OK;
 }
@@ -617,7 +628,8 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
   }
   int rc = CCPOL3B_OK;
   if ((rc = dev_alloc(c, c->d_pairpar, 2 * NPAIR)) || (rc = dev_alloc(c, c->d_coef64, 2 * NCOEF)) ||
-      (rc = dev_alloc(c, c->d_coef32, 2 * NCOEF)) || (rc = upload_coefficients(c, p->cs_sr, p->cs_lr))) {
+      (rc = dev_alloc(c, c->d_coef32s, 2 * NCOEF)) ||
+      (rc = upload_coefficients(c, p->cs_sr, p->cs_lr))) {
     ccpol3b_destroy(c);
     return rc;
   }
@@ -648,7 +660,7 @@ void ccpol3b_destroy(ccpol3b_ctx* c) {
   free_dev(c);
   dev_free(c->d_pairpar);
   dev_free(c->d_coef64);
-  dev_free(c->d_coef32);
+  dev_free(c->d_coef32s);
   dev_free(c->d_phitrip);
   dev_free(c->d_gdig);
   dev_free(c->d_gex);

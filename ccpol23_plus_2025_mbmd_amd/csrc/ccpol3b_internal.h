@@ -40,6 +40,15 @@ __host__ __device__ constexpr int site_type(int s) {
   return s == 0 ? 0 : s <= 2 ? 1 : s <= 4 ? 2 : s <= 8 ? 3 : s <= 12 ? 4 : 5;
 }
 
+// Mixed-precision FS3 evaluation: the second component of each site-pair
+// vector is e (r - MIXED_VSHIFT) instead of e r, with the coefficient table
+// transformed to that basis exactly (in FP64, once per coefficient upload).
+// The energy is the same polynomial; the shifted basis cancels 6-10x less
+// (sum of |terms| / |E| per triplet), and so loses that much less to FP32
+// rounding of the terms.  6.5 bohr is the typical site-site distance of the
+// triplets; the optimum of either shipped parameter set lies within 6-7 bohr.
+constexpr double MIXED_VSHIFT = 6.5;
+
 // Constants of one evaluation (bohr / hartree units).
 struct Consts {
   double alp3;     // alpha3 * bohr2a (1/bohr)

@@ -73,6 +73,8 @@ template <typename T, bool TR, bool PHI, bool CSM>
 __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __restrict__ pairpar,
                                               const T* __restrict__ coef, double* __restrict__ phi_trip) {
   using T2 = typename Pair2<T>::type;
+  // pair-vector basis (e, e (r - VSHIFT)); the coefficient table passed in is in the same basis
+  constexpr double VSHIFT = sizeof(T) == 4 ? MIXED_VSHIFT : 0.0;
   // sR holds r and then h = (dE/dr)/r.  The mixed variant's r is FP32-exact
   // (FP32 square root), so T storage loses nothing.
   __shared__ T sC[CSM ? NCOEF : 1];
@@ -152,7 +154,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
       const T e = t_exp<T>(-T(par.x) * rt) * damp;
       T2 v;
       v.x = e;
-      v.y = e * rt;
+      v.y = e * (rt - T(VSHIFT));
       sV[p] = v;
       if (TR) sV[(blk + 3) * NPAIR + y * NS3 + x] = v;
       sDer[p] = -T(par.x) + T(par.y) * tg * damp;
@@ -274,6 +276,17 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
     }
     __syncthreads();
     if constexpr (PHI) {
+      if constexpr (VSHIFT != 0.0) {
+        // back to the plain basis: dE/dC = (T x T x T) dE/dC', i.e. along each
+        // of the three power indices of a type block, d[1] += VSHIFT d[0]
+        for (int blk = tid; blk < NCOEF / 8; blk += 256) {
+          T* b = sDC + blk * 8;
+          for (int stride = 4; stride >= 1; stride >>= 1)
+            for (int a = 0; a < 8; ++a)
+              if (a & stride) b[a] += T(VSHIFT) * b[a - stride];
+        }
+        __syncthreads();
+      }
       // orbit sums of the dense gradient, scaled to the triplet's energy in kcal/mol
       const double scale = H2KCAL * g.fd * g.w[rr];
       double* row = phi_trip + t * (2 * NLIN3);
@@ -462,7 +475,8 @@ __global__ __launch_bounds__(64) void k_energy_boxes(Dev D, int do2, int do3, do
 
 // Coefficients come through the vector cache; LDS per block is ~29 KB (FP64,
 // 5 blocks per CU, at the cost of a few spilled registers) and ~15 KB (mixed,
-// 7 blocks per CU by registers).
+// 7 blocks per CU by registers).  The mixed kernels take the table in the
+// shifted basis (MIXED_VSHIFT), the FP64 kernels the plain one.
 __global__ __launch_bounds__(256, 5) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const double* __restrict__ coef) {
   fs3_eval_body<double, false, false, false>(D, K, pairpar, coef, nullptr);

@@ -169,7 +169,9 @@ def test_repeatable(tb_fs2fs3):
 
 
 def test_mixed_precision_error_512(params_fs2fs3, oracle):
-    """Mixed (FP32 exponential/polynomial work) vs the FP64 path at 512 waters."""
+    """Mixed (FP32 exponential/polynomial work in the shifted pair-vector basis)
+    vs the FP64 path at 512 waters: site forces, COM forces, and what an
+    integrator consumes (molecular force = sites + both COM terms, torque)."""
     from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
     box = lattice_box(512, seed=3)
     print("mixed 512", flush=True)
@@ -181,17 +183,29 @@ def test_mixed_precision_error_512(params_fs2fs3, oracle):
     err_all = rel_rms(all_forces(got, True, True), all_forces(ref, True, True))
     err_c = rel_rms(com_forces(got, "fs3"), com_forces(ref, "fs3"))
     err_e = abs(got["qfs3"] - ref["qfs3"]) / abs(ref["qfs3"])
-    print(f"mixed vs fp64: site force rms {err_f:.3e}, all forces {err_all:.3e}, "
-          f"FS3 COM {err_c:.3e}, FS3 energy {err_e:.3e}")
+
+    def molecular(r):
+        return site_forces(r).sum(2) + com_forces(r, "fs2") + com_forces(r, "fs3")
+
+    def torque(r):
+        f = site_forces(r).transpose(1, 2, 0)
+        return np.cross(box.site[:, :17], f).sum(1)
+
+    err_m = rel_rms(molecular(got), molecular(ref))
+    err_t = rel_rms(torque(got), torque(ref))
+    print(f"mixed vs fp64: site force rms {err_f:.3e}, all forces {err_all:.3e}, FS3 COM {err_c:.3e}, "
+          f"molecular force {err_m:.3e}, torque {err_t:.3e}, FS3 energy {err_e:.3e}")
     # the bar of the mixed path is on the force output
     assert err_f <= 1e-5
     assert err_all <= 1e-5
-    # The FS3 energy of this fit cancels by ~1e6 (sum of |terms| over |E| per
-    # triplet, coefficients up to 4e5), so an FP32 energy and the COM forces
-    # derived from it (f_d' E + f_d w' E) carry ~1e-2 relative error; they are
-    # reported, and only checked for gross errors here.
-    assert err_e <= 0.1
-    assert err_c <= 0.1
+    # The FS3 energy cancels by ~2e5 per triplet even in the shifted basis
+    # (sum of |terms| over |E|), so the FP32 energy and the COM forces derived
+    # from it (f_d' E + f_d w' E) carry ~1e-3; the site-force errors of a
+    # molecule add up against a molecular force ~15x smaller than its site forces.
+    assert err_e <= 3e-3
+    assert err_c <= 1e-2
+    assert err_m <= 1e-3
+    assert err_t <= 5e-4
 
 
 def test_batch_matches_single(params_fs2fs3):
