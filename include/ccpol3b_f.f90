@@ -36,7 +36,8 @@ module ccpol3b_f
   end type
 
   public :: ccpol3b_create, ccpol3b_destroy, ccpol3b_set_precision, ccpol3b_threebody_term, &
-            ccpol3b_last_stats, ccpol3b_read_data3b, ccpol3b_orbit_index, ccpol3b_threebody_term_f
+            ccpol3b_last_stats, ccpol3b_read_data3b, ccpol3b_orbit_index, ccpol3b_threebody_term_f, &
+            ccpol3b_md_create, ccpol3b_md_set_state, ccpol3b_md_get_state, ccpol3b_md_run, ccpol3b_md_destroy
 
   interface
     integer(c_int) function ccpol3b_create(params, ctx) bind(C, name="ccpol3b_create")
@@ -90,6 +91,45 @@ module ccpol3b_f
       import :: c_int
       integer(c_int), value :: is, js, ks, ip, jp, kp
     end function
+
+    ! Rigid-water NVE driver on the device (velocity Verlet + NO_SQUISH, the
+    ! reference's nve_1, evolve.f90:621-1062, 8107-8204).  Fortran arrays map
+    ! onto the C layouts directly: pcoord(3,ns), com/vel/jbody(3,nm),
+    ! quat(4,nm), records(4,nsteps).
+    integer(c_int) function ccpol3b_md_create(ctx, nm, box, ns, pcoord, mass, pmi, md) &
+        bind(C, name="ccpol3b_md_create")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nm, ns
+      real(c_double), value :: box, mass
+      real(c_double), intent(in) :: pcoord(*), pmi(3)
+      type(c_ptr), intent(out) :: md
+    end function
+
+    integer(c_int) function ccpol3b_md_set_state(md, com, quat, vel, jbody) bind(C, name="ccpol3b_md_set_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: md
+      real(c_double), intent(in) :: com(*), quat(*), vel(*), jbody(*)
+    end function
+
+    integer(c_int) function ccpol3b_md_get_state(md, com, quat, vel, jbody) bind(C, name="ccpol3b_md_get_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: md
+      real(c_double), intent(out) :: com(*), quat(*), vel(*), jbody(*)
+    end function
+
+    integer(c_int) function ccpol3b_md_run(md, nsteps, dt, records) bind(C, name="ccpol3b_md_run")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: md
+      integer(c_int), value :: nsteps
+      real(c_double), value :: dt
+      real(c_double), intent(out) :: records(*)
+    end function
+
+    subroutine ccpol3b_md_destroy(md) bind(C, name="ccpol3b_md_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: md
+    end subroutine
   end interface
 
 contains
