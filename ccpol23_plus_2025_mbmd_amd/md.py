@@ -75,6 +75,21 @@ class RigidMD:
                    "ccpol3b_md_run")
         return rec[:nsteps]
 
+    @classmethod
+    def from_revcon(cls, tb, path, nm=None, constants=None):
+        """Restart from a REVCON file as the reference does (initialization.f90:55-67):
+        COM mapped into the box, velocities, body-frame angular momenta, quaternions."""
+        from .formats import read_revcon
+        r = read_revcon(path, nm=nm)
+        return cls(tb, r.com, r.quat, r.box, r.vel, r.jbody, constants=constants)
+
+    def write_revcon(self, path, step: int, time: float, chit: float = 0.0, chip: float = 0.0):
+        """Write the current state in the reference's REVCON layout (md.f90:266-275)."""
+        from .formats import Revcon, write_revcon
+        st = self.state()
+        write_revcon(path, Revcon(step=int(step), time=float(time), box=self.box, chit=chit, chip=chip,
+                                  com=st["com"], vel=st["vel"], jbody=st["jbody"], quat=st["quat"]))
+
     def close(self):
         if self._md:
             self._L.ccpol3b_md_destroy(self._md)

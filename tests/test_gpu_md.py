@@ -78,3 +78,33 @@ def test_md_energy_conservation():
     assert np.all(np.isfinite(rec))
     assert np.abs(rec[-1, 0] - rec[0, 0]) > 10 * drift  # energy flows between E3 and K
     assert drift <= 2e-3 * k.mean(), (drift, k.mean())
+
+
+def test_md_revcon_restart_continues_the_run(tmp_path, params_fs2fs3):
+    """10 steps, REVCON, restart, 10 more steps == 20 steps without the restart
+    (the REVCON values round-trip exactly, and the COM wrap only shifts a
+    molecule by a whole box, which no kernel can see)."""
+    from ccpol23_plus_2025_mbmd_amd.geometry import lattice_box
+    from ccpol23_plus_2025_mbmd_amd.md import RigidMD, maxwell_state, water_constants
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    box = lattice_box(64, seed=6)
+    pc, mass, pmi = water_constants()
+    v, j = maxwell_state(box.nm, 298.0, mass, pmi, seed=8)
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    print("md revcon restart: 64 waters, 10 + 10 steps", flush=True)
+    a = RigidMD(tb, box.com, box.quats, box.box, v, j)
+    ra = a.run(20, 5e-4)
+    sa = a.state()
+    b = RigidMD(tb, box.com, box.quats, box.box, v, j)
+    b.run(10, 5e-4)
+    b.write_revcon(tmp_path / "REVCON", step=10, time=10 * 5e-4)
+    b.close()
+    c = RigidMD.from_revcon(tb, tmp_path / "REVCON")
+    rc = c.run(10, 5e-4)
+    sc = c.state()
+    np.testing.assert_allclose(rc, ra[10:], rtol=1e-10, atol=1e-10)
+    L = box.box
+    d = sc["com"] - sa["com"]
+    np.testing.assert_allclose(d - L * np.round(d / L), 0.0, atol=1e-9)
+    for key in ("quat", "vel", "jbody"):
+        np.testing.assert_allclose(sc[key], sa[key], rtol=1e-9, atol=1e-12 * np.abs(sa[key]).max(), err_msg=key)
