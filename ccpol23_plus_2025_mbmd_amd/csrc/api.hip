@@ -30,9 +30,8 @@ __global__ void k_fs3_phi_f64(Dev, Consts, const double4*, const double*, double
 __global__ void k_fs3_phi_f32(Dev, Consts, const double4*, const float*, double*);
 __global__ void k_phi_boxes(Dev, const double*, double*);
 int set_orbit_tables(const short* off, const short* mem);
-__global__ void k_gram_exps(int, int, const double*, int, int*);
-__global__ void k_gram_digits(int, int, int, int, const double*, int, const int*, float*);
-__global__ void k_gram_mfma(int, int, int, const float*, const int*, double*, int);
+__global__ void k_gram_f64(int, int, const double*, int, int, double*);
+__global__ void k_gram_sum(int, int, const double*, double*, int);
 __global__ void k_fs2_pair_pass(Dev, Consts);
 __global__ void k_fs2_spectator_pass(Dev, Consts);
 size_t fs2_lds_bytes(int cap_nb);
@@ -137,9 +136,8 @@ struct ccpol3b_ctx {
   // kept so a graph can own a private copy of the context
   std::vector<double> h_tab[6], h_cs[2];
   double alpha3 = 0.0, delta33 = 0.0, rcut3 = 0.0;
-  float* d_gdig = nullptr;       // Gram digits [6][rows][cols]
-  int* d_gex = nullptr;          // Gram column exponents
-  size_t cap_gdig = 0, cap_gex = 0;
+  double* d_gpart = nullptr;     // Gram partial sums [row ranges][cols][cols]
+  size_t cap_gpart = 0;
   int cu_count = 256;
   // evaluation buffers
   Dev D{};
@@ -662,8 +660,7 @@ void ccpol3b_destroy(ccpol3b_ctx* c) {
   dev_free(c->d_coef64);
   dev_free(c->d_coef32s);
   dev_free(c->d_phitrip);
-  dev_free(c->d_gdig);
-  dev_free(c->d_gex);
+  dev_free(c->d_gpart);
   dev_free(c->d_in);
   dev_free(c->d_out);
   if (c->h_totals) (void)hipHostFree(c->h_totals);
@@ -704,24 +701,21 @@ int ccpol3b_gram_device(ccpol3b_ctx* c, int nrow, int ncol, const double* phi, i
     return fail(c, CCPOL3B_EINVAL, "invalid argument");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
   hipStream_t st = (hipStream_t)stream;
-  const int nrp = std::max(64, (nrow + 63) / 64 * 64), ncp = (ncol + 31) / 32 * 32;
-  const size_t ndig = (size_t)6 * nrp * ncp;
+  // row ranges of at least 512 rows (up to 16) so a 728-column matrix fills the chip
+  const int nsplit = std::min(16, std::max(1, (nrow + 511) / 512));
+  const int rps = ((nrow + nsplit - 1) / nsplit + 15) / 16 * 16;
+  const size_t npart = (size_t)nsplit * ncol * ncol;
   int rc;
-  if (ndig > c->cap_gdig) {
-    if ((rc = dev_alloc(c, c->d_gdig, ndig))) return rc;
-    c->cap_gdig = ndig;
+  if (npart > c->cap_gpart) {
+    if ((rc = dev_alloc(c, c->d_gpart, npart))) return rc;
+    c->cap_gpart = npart;
   }
-  if ((size_t)ncol > c->cap_gex) {
-    if ((rc = dev_alloc(c, c->d_gex, ncol))) return rc;
-    c->cap_gex = ncol;
-  }
-  hipLaunchKernelGGL(k_gram_exps, dim3((ncol + 255) / 256), dim3(256), 0, st, nrow, ncol, phi, ld_phi, c->d_gex);
-  const long nd = (long)nrp * ncp;
-  hipLaunchKernelGGL(k_gram_digits, dim3((nd + 255) / 256), dim3(256), 0, st, nrow, ncol, nrp, ncp, phi, ld_phi,
-                     (const int*)c->d_gex, c->d_gdig);
-  const int nt = ncp / 32;
-  hipLaunchKernelGGL(k_gram_mfma, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, ncol, nrp, ncp,
-                     (const float*)c->d_gdig, (const int*)c->d_gex, gram, ld_gram);
+  const int nt = (ncol + 63) / 64;
+  hipLaunchKernelGGL(k_gram_f64, dim3(nt * (nt + 1) / 2, nsplit), dim3(256), 0, st, nrow, ncol, phi, ld_phi, rps,
+                     c->d_gpart);
+  const long nn = (long)ncol * ncol;
+  hipLaunchKernelGGL(k_gram_sum, dim3((nn + 255) / 256), dim3(256), 0, st, ncol, nsplit,
+                     (const double*)c->d_gpart, gram, ld_gram);
   LAUNCHCHK(c, "Gram");
   return CCPOL3B_OK;
 }

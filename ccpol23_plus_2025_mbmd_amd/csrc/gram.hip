@@ -1,101 +1,104 @@
 // Gram matrix G = Phi^T Phi of the coefficient-gradient design matrix
 // (rows = configurations, columns = the 728 SR/LR linear coefficients) on the
-// FP32 matrix cores, with FP64 accuracy.
+// FP64 matrix cores (v_mfma_f64_16x16x4f64): products and sums in FP64, so
+// the result carries the ordinary FP64 dot-product error bound.
 //
-// Every column a is scaled by its largest magnitude, 2^e_a, and written as six
-// base-2^9 digits in FP32: Phi[r,a] = 2^e_a sum_k d_k[r,a] 2^(-9(k+1)), |d_k| <= 2^9.
-// Digit products are integers below 2^18, so a 64-row block of products sums
-// to an integer below 2^24: v_mfma_f32_16x16x4f32 accumulates it exactly.
-// Each 64-row block is then scaled and added in FP64.  Digit pairs with
-// k + l <= 5 are kept (21 pairs); the dropped ones are below 2^-54 of
-// 2^(e_a+e_b) per row, the FP64 level.
+// Work split: 64 x 64 output tiles of the upper triangle (the lower one is the
+// mirror) times `nsplit` row ranges, one 256-thread block each, so a 4096 x 728
+// matrix gives 78 x 8 blocks.  Each block stages 16 rows of its two column
+// strips in LDS per step; wavefront w owns the 32 x 32 quarter (w / 2, w % 2)
+// as 2 x 2 MFMA tiles.  Row-range partial tiles go to a scratch array and a
+// second kernel adds them in a fixed order (results are the same run to run).
 #include "devcommon.h"
 
 namespace ccpol3b {
 
-constexpr int GR_DIG = 6;       // digits per value
-constexpr int GR_BITS = 9;      // bits per digit
-constexpr int GR_ROWBLK = 64;   // rows accumulated exactly in FP32
-constexpr int GR_TILE = 32;     // output tile per 256-thread block (2 x 2 wavefronts of 16 x 16)
+constexpr int GR_T = 64;   // output tile edge
+constexpr int GR_K = 16;   // rows per LDS stage
+constexpr int GR_LD = GR_T + 1;
 
-// Column exponents: e_a with max_r |Phi[r,a]| < 2^e_a (0 for an all-zero column).
-__global__ __launch_bounds__(256) void k_gram_exps(int nrow, int ncol, const double* __restrict__ phi, int ld,
-                                                   int* __restrict__ ex) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= ncol) return;
-  double m = 0.0;
-  for (int r = 0; r < nrow; ++r) m = fmax(m, fabs(phi[(long)r * ld + a]));
-  int e = 0;
-  if (m > 0.0) frexp(m, &e);  // m = f 2^e, f in [0.5, 1)
-  ex[a] = e;
-}
+typedef double v4d __attribute__((ext_vector_type(4)));
 
-// Digits dig[k][r][a] (rows padded to a multiple of 64, columns to 32, padding zero).
-__global__ __launch_bounds__(256) void k_gram_digits(int nrow, int ncol, int nrp, int ncp,
-                                                     const double* __restrict__ phi, int ld,
-                                                     const int* __restrict__ ex, float* __restrict__ dig) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long)nrp * ncp) return;
-  const int r = (int)(i / ncp), a = (int)(i - (long)r * ncp);
-  double x = (r < nrow && a < ncol) ? ldexp(phi[(long)r * ld + a], -ex[a]) : 0.0;
-  const long plane = (long)nrp * ncp;
-#pragma unroll
-  for (int k = 0; k < GR_DIG; ++k) {
-    const double y = ldexp(x, GR_BITS);
-    const double d = rint(y);
-    x = y - d;  // exact
-    dig[k * plane + i] = (float)d;
-  }
-}
-
-// One block per upper-triangle tile (ta <= tb) of 32 x 32 entries; wavefront w
-// owns the 16 x 16 sub-tile (w / 2, w % 2).  Lane layout of
-// v_mfma_f32_16x16x4f32: A[m][kk] from lane m + 16 kk, B[kk][n] from lane
-// n + 16 kk, D[4 (lane / 16) + i][lane % 16] in result register i.
-__global__ __launch_bounds__(256) void k_gram_mfma(int ncol, int nrp, int ncp, const float* __restrict__ dig,
-                                                   const int* __restrict__ ex, double* __restrict__ gram,
-                                                   int ldg) {
-  // tile (ta, tb) from the linear upper-triangle index
-  const int nt = ncp / GR_TILE;
-  int t = blockIdx.x, ta = 0;
+// upper-triangle tile (ta <= tb) of the linear index t over nt x nt tiles
+__device__ __forceinline__ void gram_tile(int t, int nt, int& ta, int& tb) {
+  ta = 0;
   while (t >= nt - ta) {
     t -= nt - ta;
     ++ta;
   }
-  const int tb = ta + t;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int a0 = ta * GR_TILE + (w >> 1) * 16, b0 = tb * GR_TILE + (w & 1) * 16;
-  const int col = lane & 15, kk = lane >> 4;
-  const long plane = (long)nrp * ncp;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int k = 0; k < GR_DIG; ++k) {
-    for (int l = 0; k + l < GR_DIG; ++l) {
-      const double sc = ldexp(1.0, -GR_BITS * (k + l + 2));
-      const float* A = dig + k * plane + a0 + col;
-      const float* B = dig + l * plane + b0 + col;
-      for (int r0 = 0; r0 < nrp; r0 += GR_ROWBLK) {
-        typedef float v4f __attribute__((ext_vector_type(4)));
-        v4f c = {0.f, 0.f, 0.f, 0.f};
+  tb = ta + t;
+}
+
+// part[s][a][b] = sum over rows r of split s of phi[r][a] phi[r][b], for (a, b)
+// in the tile.  Lane layout of v_mfma_f64_16x16x4f64 (measured on gfx950):
+// A[m][k] from lane m + 16 k, B[k][n] from lane n + 16 k, and result element
+// i of a lane is D[lane / 16 + 4 i][lane % 16] (rows interleaved, unlike the
+// four consecutive rows of v_mfma_f32_16x16x4f32).
+__global__ __launch_bounds__(256) void k_gram_f64(int nrow, int ncol, const double* __restrict__ phi, int ld,
+                                                  int rows_per_split, double* __restrict__ part) {
+  __shared__ double sA[GR_K][GR_LD];
+  __shared__ double sB[GR_K][GR_LD];
+  const int nt = (ncol + GR_T - 1) / GR_T;
+  int ta, tb;
+  gram_tile(blockIdx.x, nt, ta, tb);
+  const int s = blockIdx.y;
+  const int r0 = s * rows_per_split, r1 = min(nrow, r0 + rows_per_split);
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int a0 = ta * GR_T, b0 = tb * GR_T;
+  const int m = lane & 15, k4 = lane >> 4;
+  const int wa = (w >> 1) * 32, wb = (w & 1) * 32;
+  v4d acc[2][2];
 #pragma unroll
-        for (int s = 0; s < GR_ROWBLK / 4; ++s) {
-          const long row = r0 + 4 * s + kk;
-          c = __builtin_amdgcn_mfma_f32_16x16x4f32(A[row * ncp], B[row * ncp], c, 0, 0, 0);
-        }
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i] += sc * (double)c[i];
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  // staging: thread t loads column (t % 64) of rows (t / 64) + 4 q, q = 0..3
+  const int lc = tid & 63, lr = tid >> 6;
+  for (int rb = r0; rb < r1; rb += GR_K) {
+#pragma unroll
+    for (int q = 0; q < GR_K / 4; ++q) {
+      const int r = rb + lr + 4 * q;
+      const bool rin = r < r1;
+      const int ca = a0 + lc, cb = b0 + lc;
+      sA[lr + 4 * q][lc] = (rin && ca < ncol) ? phi[(long)r * ld + ca] : 0.0;
+      sB[lr + 4 * q][lc] = (rin && cb < ncol) ? phi[(long)r * ld + cb] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GR_K / 4; ++kk) {
+      const int row = 4 * kk + k4;
+      const double av0 = sA[row][wa + m], av1 = sA[row][wa + 16 + m];
+      const double bv0 = sB[row][wb + m], bv1 = sB[row][wb + 16 + m];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double* out = part + (long)s * ncol * ncol;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int a = a0 + wa + 16 * i + k4 + 4 * e, b = b0 + wb + 16 * j + m;
+        if (a < ncol && b < ncol) out[(long)a * ncol + b] = acc[i][j][e];
       }
-    }
-  }
-  const int b = b0 + col;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int a = a0 + 4 * kk + i;
-    if (a < ncol && b < ncol && a <= b) {
-      const double v = ldexp(acc[i], ex[a] + ex[b]);
-      gram[(long)a * ldg + b] = v;
-      gram[(long)b * ldg + a] = v;
-    }
-  }
+}
+
+// gram[a][b] = gram[b][a] = sum over splits, in order, of part[s][a][b] (a <= b).
+__global__ __launch_bounds__(256) void k_gram_sum(int ncol, int nsplit, const double* __restrict__ part,
+                                                  double* __restrict__ gram, int ldg) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)ncol * ncol) return;
+  const int a = (int)(i / ncol), b = (int)(i - (long)a * ncol);
+  if (a > b) return;
+  double v = 0.0;
+  for (int s = 0; s < nsplit; ++s) v += part[(long)s * ncol * ncol + i];
+  gram[(long)a * ldg + b] = v;
+  gram[(long)b * ldg + a] = v;
 }
 
 }  // namespace ccpol3b

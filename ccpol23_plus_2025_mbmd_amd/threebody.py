@@ -205,7 +205,7 @@ class ThreeBody:
 
     def gram(self, phi, stream=None):
         """G = phi^T phi (float64 [ncol, ncol]) of a float64 cuda matrix phi [nrow, ncol]
-        on the matrix cores (FP32 digits, FP64-accurate)."""
+        on the FP64 matrix cores."""
         import torch
         assert phi.is_cuda and phi.dtype == torch.float64 and phi.dim() == 2
         phi = phi.contiguous()

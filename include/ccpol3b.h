@@ -139,8 +139,9 @@ int ccpol3b_eval_batch_grad_device(ccpol3b_ctx *ctx, int nbox, const int *mol_st
                                    double *d_energies, double *d_phi, void *stream);
 
 /* Gram matrix gram[a][b] = sum_r phi[r][a] phi[r][b] (row-major, FP64) of a
- * design matrix of coefficient gradients (rows = configurations), on the FP32
- * matrix cores with an exact digit decomposition that gives FP64 accuracy.
+ * design matrix of coefficient gradients (rows = configurations), on the FP64
+ * matrix cores (the FP64 dot-product error bound holds; partial sums over row
+ * ranges are added in a fixed order, so results repeat bit for bit).
  * No counterpart in the reference (its refit code is commented out,
  * threebody_potential.f90:1071,1304).  Enqueued on `stream`; the context owns
  * the scratch. */

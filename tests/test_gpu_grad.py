@@ -162,22 +162,28 @@ def test_autograd_matches_finite_differences(params_fs2fs3, trimers):
 
 
 def test_gram_matches_extended_precision(params_fs2fs3):
-    """Matrix-core Gram against a long-double reference (small) and FP64 BLAS (config size)."""
+    """FP64 matrix-core Gram against a long-double reference (small, ragged
+    shapes) and FP64 BLAS (the 4096 x 728 refit shape), within the FP64
+    dot-product bound |fl(sum a b) - sum a b| <= n u sum |a| |b|."""
     from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
     tb = ThreeBody(params_fs2fs3, "fp64")
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(3)
-    # columns of very different scales and signs, like SR/LR coefficient gradients
-    a = rng.normal(size=(300, 70)) * np.exp(rng.uniform(-20, 5, size=70))
-    a[:, 7] = 0.0
-    g = tb.gram(torch.tensor(a, device=dev)).cpu().numpy()
-    al = a.astype(np.longdouble)
-    ref = (al.T @ al).astype(np.float64)
-    # the digit decomposition is exact to 2^-53 of the column maxima per row
-    m = np.abs(a).max(0)
-    bound = 16 * 2.0 ** -53 * a.shape[0] * np.outer(m, m)
-    assert np.all(np.abs(g - ref) <= bound), np.max(np.abs(g - ref) / (bound + 1e-300))
-    assert np.array_equal(g, g.T)
+    u = 2.0 ** -53
+    # columns of very different scales and signs, like SR/LR coefficient gradients;
+    # shapes that are not multiples of the 64 x 64 tiles or of the 16-row stages
+    for nrow, ncol in ((300, 70), (1, 5), (1037, 129)):
+        a = rng.normal(size=(nrow, ncol)) * np.exp(rng.uniform(-20, 5, size=ncol))
+        a[:, min(7, ncol - 1)] = 0.0
+        print(f"gram {nrow} x {ncol}", flush=True)
+        g = tb.gram(torch.tensor(a, device=dev)).cpu().numpy()
+        al = a.astype(np.longdouble)
+        ref = (al.T @ al).astype(np.float64)
+        bound = (nrow + 2) * u * (np.abs(a).T @ np.abs(a)) + 1e-300
+        assert np.all(np.abs(g - ref) <= bound), np.max(np.abs(g - ref) / bound)
+        assert np.array_equal(g, g.T)
+    # no rows: the zero matrix
+    assert not tb.gram(torch.zeros((0, 9), dtype=torch.float64, device=dev)).any()
     # the config-4 shape: 4096 x 728
     b = rng.normal(size=(4096, 728)) * np.exp(rng.uniform(-10, 3, size=728))
     bt = torch.tensor(b, device=dev)
