@@ -193,3 +193,39 @@ def test_gram_matches_extended_precision(params_fs2fs3):
     bound = 1e-12 * (bt.abs().T @ bt.abs()) + 16 * 2.0 ** -53 * 4096 * torch.outer(m, m)
     err = ((g - ref).abs() / bound).max().item()
     assert err <= 1.0, err
+
+
+def test_refit_reproduces_energies_and_writes_tables(tmp_path, params_fs2fs3):
+    """Fit the 728 coefficients to FS3 energies of 1024 trimers made with the
+    shipped coefficients; the fit reproduces them, and the data3b_full tables
+    written from it, read back into a new context, give the fitted energies."""
+    import shutil
+    from ccpol23_plus_2025_mbmd_amd.params import DATA_DIR, load_variant
+    from ccpol23_plus_2025_mbmd_amd.refit import design_matrix, solve_coefficients, write_fit
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    ts = [b for b in trimer_set(1400, seed=12) if away_from_cutoffs(b)][:1024]
+    com, site, ms, bl = pack(ts)
+    dev = torch.device("cuda:0")
+    com_t = torch.tensor(com.T.copy(), device=dev)
+    site_t = torch.tensor(site.transpose(2, 0, 1).copy(), device=dev)
+    print(f"refit: {len(ts)} trimers", flush=True)
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    phi, y = design_matrix(tb, ms, bl, com_t, site_t)
+    c, res = solve_coefficients(tb, phi, y)
+    scale = float(torch.sqrt(torch.mean(y ** 2)))
+    print(f"refit residual {res:.3e} kcal/mol, energy RMS {scale:.3e}", flush=True)
+    assert res <= 2e-5 * scale, (res, scale)
+    # weights only reweight the residual; an exact target set stays reproduced
+    w = torch.linspace(0.5, 2.0, len(ts), dtype=torch.float64, device=dev)
+    _, res_w = solve_coefficients(tb, phi, y, weights=w)
+    assert res_w <= 2e-5 * scale
+    # tables written from the fit, loaded as a parameter set of their own
+    d = tmp_path / "fit"
+    write_fit(d, params_fs2fs3, c)
+    shutil.copy(DATA_DIR / "fs2fs3" / "CONTROL", d / "CONTROL")
+    q = load_variant("fs2fs3", d)
+    assert np.array_equal(np.concatenate([q.sr.cs, q.lr.cs]), c.cpu().numpy())
+    tb2 = ThreeBody(q, "fp64")
+    _, _, _, en2 = tb2.eval_batch_device(ms, bl, com_t, site_t)
+    fitted = phi @ c
+    assert torch.allclose(en2[:, 1], fitted, rtol=1e-9, atol=1e-9 * scale)
