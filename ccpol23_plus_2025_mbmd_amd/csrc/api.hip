@@ -23,11 +23,11 @@ template <int mode> __global__ void k_fs3_list(Dev, Consts);
 __global__ void k_fs2_items(Dev, Consts, int);
 __global__ void k_scan(Dev, int, int, int);
 __global__ void k_fs3_eval_f64(Dev, Consts, const double4*, const double*);
-__global__ void k_fs3_eval_f32(Dev, Consts, const double4*, const float*);
+__global__ void k_fs3_eval_f32(Dev, Consts, const float4*, const float*);
 __global__ void k_fs3_eval_f64_lds(Dev, Consts, const double4*, const double*);
-__global__ void k_fs3_eval_f32_lds(Dev, Consts, const double4*, const float*);
+__global__ void k_fs3_eval_f32_lds(Dev, Consts, const float4*, const float*);
 __global__ void k_fs3_phi_f64(Dev, Consts, const double4*, const double*, double*);
-__global__ void k_fs3_phi_f32(Dev, Consts, const double4*, const float*, double*);
+__global__ void k_fs3_phi_f32(Dev, Consts, const float4*, const float*, double*);
 __global__ void k_phi_boxes(Dev, const double*, double*);
 int set_orbit_tables(const short* off, const short* mem);
 __global__ void k_gram_f64(int, int, const double*, int, int, double*);
@@ -127,6 +127,7 @@ struct ccpol3b_ctx {
   ccpol3b_stats stats{};
   // parameters on the device
   double4* d_pairpar = nullptr;  // [2][289]
+  float4* d_pairpar32 = nullptr; // [2][289] FP32 copy for the mixed kernels
   double* d_coef64 = nullptr;    // [2][NCOEF]
   float* d_coef32s = nullptr;    // [2][NCOEF] FP32, shifted pair-vector basis of the mixed kernels
   double* d_phitrip = nullptr;   // [cap_phi][728] per-triplet coefficient-gradient rows
@@ -342,17 +343,17 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     const int grid = (int)cap3;  // one block per triplet; blocks past the list exit at once
     if (phi) {
       if (c->precision == CCPOL3B_MIXED)
-        hipLaunchKernelGGL(k_fs3_phi_f32, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
+        hipLaunchKernelGGL(k_fs3_phi_f32, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar32,
                            (const float*)c->d_coef32s, c->d_phitrip);
       else
         hipLaunchKernelGGL(k_fs3_phi_f64, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
                            (const double*)c->d_coef64, c->d_phitrip);
     } else if (c->precision == CCPOL3B_MIXED) {
       if (c->tune.fs3_lds_coef)
-        hipLaunchKernelGGL(k_fs3_eval_f32_lds, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
+        hipLaunchKernelGGL(k_fs3_eval_f32_lds, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar32,
                            (const float*)c->d_coef32s);
       else
-        hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
+        hipLaunchKernelGGL(k_fs3_eval_f32, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar32,
                            (const float*)c->d_coef32s);
     } else {
       if (c->tune.fs3_lds_coef)
@@ -601,6 +602,7 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
   K.rmax = 5.5 / BOHR2A;
   // parameter tables
   std::vector<double4> pairpar(2 * NPAIR);
+  std::vector<float4> pairpar32(2 * NPAIR);
   for (int X = 0; X < 2; ++X) {
     const double* b = X ? p->beta_lr : p->beta_sr;
     const double* g = X ? p->gamma_lr : p->gamma_sr;
@@ -609,6 +611,7 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
       for (int js = 0; js < 17; ++js) {
         const size_t q = is + (size_t)p->ld_pair * js;
         pairpar[X * NPAIR + is * 17 + js] = make_double4(b[q], g[q], r[q], 0.0);
+        pairpar32[X * NPAIR + is * 17 + js] = make_float4((float)b[q], (float)g[q], (float)r[q], 0.f);
       }
   }
   // orbit -> dense entries (CSR) for the coefficient-gradient kernels
@@ -625,13 +628,15 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
     }
   }
   int rc = CCPOL3B_OK;
-  if ((rc = dev_alloc(c, c->d_pairpar, 2 * NPAIR)) || (rc = dev_alloc(c, c->d_coef64, 2 * NCOEF)) ||
+  if ((rc = dev_alloc(c, c->d_pairpar, 2 * NPAIR)) || (rc = dev_alloc(c, c->d_pairpar32, 2 * NPAIR)) ||
+      (rc = dev_alloc(c, c->d_coef64, 2 * NCOEF)) ||
       (rc = dev_alloc(c, c->d_coef32s, 2 * NCOEF)) ||
       (rc = upload_coefficients(c, p->cs_sr, p->cs_lr))) {
     ccpol3b_destroy(c);
     return rc;
   }
   if (hipMemcpy(c->d_pairpar, pairpar.data(), sizeof(double4) * 2 * NPAIR, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_pairpar32, pairpar32.data(), sizeof(float4) * 2 * NPAIR, hipMemcpyHostToDevice) != hipSuccess ||
       hipHostMalloc((void**)&c->h_totals, 2 * T_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     ccpol3b_destroy(c);
     return CCPOL3B_EHIP;
@@ -657,6 +662,7 @@ void ccpol3b_destroy(ccpol3b_ctx* c) {
   (void)hipDeviceSynchronize();
   free_dev(c);
   dev_free(c->d_pairpar);
+  dev_free(c->d_pairpar32);
   dev_free(c->d_coef64);
   dev_free(c->d_coef32s);
   dev_free(c->d_phitrip);

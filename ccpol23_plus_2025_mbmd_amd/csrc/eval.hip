@@ -31,9 +31,14 @@ __constant__ int c_tilet[9] = {0, 1, 2, 3, 3, 4, 4, 5, 5};
 // ====================================================================== FS3
 // Mixed-precision helpers: r from the FP64 r^2 with a FP32 square root, and a
 // FP32 reciprocal; the FP64 variant keeps sqrt and the IEEE division.
-template <typename T> __device__ __forceinline__ double t_dist(double r2);
+template <typename T> __device__ __forceinline__ T t_dist(double r2);
 template <> __device__ __forceinline__ double t_dist<double>(double r2) { return sqrt(r2); }
-template <> __device__ __forceinline__ double t_dist<float>(double r2) { return double(__fsqrt_rn(float(r2))); }
+// the hardware square root (1 ulp); r^2 >= ~1 bohr^2 here, far from denormals
+template <> __device__ __forceinline__ float t_dist<float>(double r2) { return __builtin_amdgcn_sqrtf(float(r2)); }
+// (beta, gamma, r0, -) of a site pair in the kernel's precision
+template <typename T> struct PairPar;
+template <> struct PairPar<double> { using type = double4; };
+template <> struct PairPar<float> { using type = float4; };
 template <typename T> __device__ __forceinline__ T t_rcp(T x);
 template <> __device__ __forceinline__ double t_rcp<double>(double x) { return 1.0 / x; }
 template <> __device__ __forceinline__ float t_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -67,10 +72,11 @@ __constant__ int c_nz[NTYPE] = {1, 2, 2, 4, 4, 4};
 // 364 SR then 364 LR) to phi_trip[t][728].
 // CSM: stage the region's coefficient table in LDS; otherwise the tile threads
 // read it through the vector cache (less LDS per block, more blocks per CU).
-// pairpar: [2][289] (beta, gamma, r0, -) for SR and LR; coef: [2][NCOEF] dense
+// pairpar: [2][289] (beta, gamma, r0, -) for SR and LR (FP32 copy for the mixed
+// kernels); coef: [2][NCOEF] dense
 // [tx][ty][tz][p][q][s].
 template <typename T, bool TR, bool PHI, bool CSM>
-__device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __restrict__ pairpar,
+__device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename PairPar<T>::type* __restrict__ pairpar,
                                               const T* __restrict__ coef, double* __restrict__ phi_trip) {
   using T2 = typename Pair2<T>::type;
   // pair-vector basis (e, e (r - VSHIFT)); the coefficient table passed in is in the same basis
@@ -138,27 +144,53 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
     if constexpr (PHI)
       for (int q = tid; q < NCOEF; q += 256) sDC[q] = T(0);
     if (rr == 0) __syncthreads();  // sP and g
-    // ---------------- pair stage: e, e*r and d ln e / dr for the 867 site pairs
-    for (int p = tid; p < NPAIR3; p += 256) {
-      const int blk = p / NPAIR, rem = p - blk * NPAIR;
-      const int x = rem / NS3, y = rem - x * NS3;
-      const int ra = blk == 2 ? 1 : 0, rb = blk == 0 ? 1 : 2;
-      const double vx = g.c[blk][0] + (sP[ra][x][0] - sP[rb][y][0]);
-      const double vy = g.c[blk][1] + (sP[ra][x][1] - sP[rb][y][1]);
-      const double vz = g.c[blk][2] + (sP[ra][x][2] - sP[rb][y][2]);
-      const double r = t_dist<T>(vx * vx + vy * vy + vz * vz);
-      const double4 par = pairpar[X * NPAIR + rem];
-      const T rt = T(r);
-      const T tg = t_exp<T>(-T(par.y) * (rt - T(par.z)));
-      const T damp = t_rcp<T>(T(1) + tg);
-      const T e = t_exp<T>(-T(par.x) * rt) * damp;
-      T2 v;
-      v.x = e;
-      v.y = e * (rt - T(VSHIFT));
-      sV[p] = v;
-      if (TR) sV[(blk + 3) * NPAIR + y * NS3 + x] = v;
-      sDer[p] = -T(par.x) + T(par.y) * tg * damp;
-      if (rr == 0) sR[p] = rt;
+    // ---------------- pair stage: e, e*r and d ln e / dr for the 867 site pairs.
+    // Thread tid takes p = tid + 256 k.  In the mixed kernels (blk, x, y) of p
+    // advance incrementally (256 = 15 * 17 + 1), and the distances of the first
+    // region are reused from sR in the second.
+    {
+      int blk = 0, x = tid / NS3, y = tid - (tid / NS3) * NS3;
+      for (int p = tid; p < NPAIR3; p += 256) {
+        if constexpr (sizeof(T) == 8) {  // FP64: fewer live registers with the divisions
+          blk = p / NPAIR;
+          x = (p - blk * NPAIR) / NS3;
+          y = p - blk * NPAIR - x * NS3;
+        }
+        const int rem = x * NS3 + y;
+        T rt;
+        if (rr == 0 || sizeof(T) == 8) {  // FP64: recomputing costs fewer registers than the branch
+          const int ra = blk == 2 ? 1 : 0, rb = blk == 0 ? 1 : 2;
+          const double vx = g.c[blk][0] + (sP[ra][x][0] - sP[rb][y][0]);
+          const double vy = g.c[blk][1] + (sP[ra][x][1] - sP[rb][y][1]);
+          const double vz = g.c[blk][2] + (sP[ra][x][2] - sP[rb][y][2]);
+          rt = t_dist<T>(vx * vx + vy * vy + vz * vz);
+          if (rr == 0) sR[p] = rt;
+        } else {
+          rt = sR[p];
+        }
+        const auto par = pairpar[X * NPAIR + rem];
+        const T tg = t_exp<T>(-T(par.y) * (rt - T(par.z)));
+        const T damp = t_rcp<T>(T(1) + tg);
+        const T e = t_exp<T>(-T(par.x) * rt) * damp;
+        T2 v;
+        v.x = e;
+        v.y = e * (rt - T(VSHIFT));
+        sV[p] = v;
+        if (TR) sV[(blk + 3) * NPAIR + y * NS3 + x] = v;
+        sDer[p] = -T(par.x) + T(par.y) * tg * damp;
+        if constexpr (sizeof(T) == 4) {
+          y += 1;
+          x += 15;
+          if (y >= NS3) {
+            y -= NS3;
+            ++x;
+          }
+          if (x >= NS3) {
+            x -= NS3;
+            ++blk;
+          }
+        }
+      }
     }
     __syncthreads();
     // ---------------- site-triple stage
@@ -329,28 +361,27 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const double4* __
     const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
     const int s = rem / 3, c = rem - s * 3;
     const double aij = g.c[0][c], aik = g.c[1][c], ajk = g.c[2][c];
+    // site-pair vector component = (COM part + own site) -/+ other site; the
+    // first sum is the same for all 17 partners
     double F = 0.0;
     if (role == 0) {
+      const double bj = aij + sP[0][s][c], bk = aik + sP[0][s][c];
 #pragma unroll 4
-      for (int y = 0; y < NS3; ++y)
-        F -= double(sR[s * NS3 + y]) * (aij + (sP[0][s][c] - sP[1][y][c]));
+      for (int y = 0; y < NS3; ++y) F -= double(sR[s * NS3 + y]) * (bj - sP[1][y][c]);
 #pragma unroll 4
-      for (int z = 0; z < NS3; ++z)
-        F -= double(sR[NPAIR + s * NS3 + z]) * (aik + (sP[0][s][c] - sP[2][z][c]));
+      for (int z = 0; z < NS3; ++z) F -= double(sR[NPAIR + s * NS3 + z]) * (bk - sP[2][z][c]);
     } else if (role == 1) {
+      const double bi = aij - sP[1][s][c], bk = ajk + sP[1][s][c];
 #pragma unroll 4
-      for (int x = 0; x < NS3; ++x)
-        F += double(sR[x * NS3 + s]) * (aij + (sP[0][x][c] - sP[1][s][c]));
+      for (int x = 0; x < NS3; ++x) F += double(sR[x * NS3 + s]) * (bi + sP[0][x][c]);
 #pragma unroll 4
-      for (int z = 0; z < NS3; ++z)
-        F -= double(sR[2 * NPAIR + s * NS3 + z]) * (ajk + (sP[1][s][c] - sP[2][z][c]));
+      for (int z = 0; z < NS3; ++z) F -= double(sR[2 * NPAIR + s * NS3 + z]) * (bk - sP[2][z][c]);
     } else {
+      const double bi = aik - sP[2][s][c], bj = ajk - sP[2][s][c];
 #pragma unroll 4
-      for (int x = 0; x < NS3; ++x)
-        F += double(sR[NPAIR + x * NS3 + s]) * (aik + (sP[0][x][c] - sP[2][s][c]));
+      for (int x = 0; x < NS3; ++x) F += double(sR[NPAIR + x * NS3 + s]) * (bi + sP[0][x][c]);
 #pragma unroll 4
-      for (int y = 0; y < NS3; ++y)
-        F += double(sR[2 * NPAIR + y * NS3 + s]) * (ajk + (sP[1][y][c] - sP[2][s][c]));
+      for (int y = 0; y < NS3; ++y) F += double(sR[2 * NPAIR + y * NS3 + s]) * (bj + sP[1][y][c]);
     }
     unsafeAtomicAdd(D.acc + (long)g.m[role] * ACC_W + rem, F);
   } else if (tid < 3 * NS3 * 3 + 9) {
@@ -482,7 +513,7 @@ __global__ __launch_bounds__(256, 5) void k_fs3_eval_f64(Dev D, Consts K, const 
   fs3_eval_body<double, false, false, false>(D, K, pairpar, coef, nullptr);
 }
 
-__global__ __launch_bounds__(256, 7) void k_fs3_eval_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
+__global__ __launch_bounds__(256, 7) void k_fs3_eval_f32(Dev D, Consts K, const float4* __restrict__ pairpar,
                                                          const float* __restrict__ coef) {
   fs3_eval_body<float, false, false, false>(D, K, pairpar, coef, nullptr);
 }
@@ -493,7 +524,7 @@ __global__ __launch_bounds__(256, 3) void k_fs3_eval_f64_lds(Dev D, Consts K, co
   fs3_eval_body<double, false, false, true>(D, K, pairpar, coef, nullptr);
 }
 
-__global__ __launch_bounds__(256, 7) void k_fs3_eval_f32_lds(Dev D, Consts K, const double4* __restrict__ pairpar,
+__global__ __launch_bounds__(256, 7) void k_fs3_eval_f32_lds(Dev D, Consts K, const float4* __restrict__ pairpar,
                                                              const float* __restrict__ coef) {
   fs3_eval_body<float, false, false, true>(D, K, pairpar, coef, nullptr);
 }
@@ -504,7 +535,7 @@ __global__ __launch_bounds__(256, 2) void k_fs3_phi_f64(Dev D, Consts K, const d
   fs3_eval_body<double, false, true, true>(D, K, pairpar, coef, phi_trip);
 }
 
-__global__ __launch_bounds__(256, 4) void k_fs3_phi_f32(Dev D, Consts K, const double4* __restrict__ pairpar,
+__global__ __launch_bounds__(256, 4) void k_fs3_phi_f32(Dev D, Consts K, const float4* __restrict__ pairpar,
                                                         const float* __restrict__ coef, double* phi_trip) {
   fs3_eval_body<float, false, true, true>(D, K, pairpar, coef, phi_trip);
 }
