@@ -235,3 +235,80 @@ def test_batch_matches_single(params_fs2fs3):
         for a, b_ in ((fs[:, sl], fs1), (c2[:, sl], c21), (c3[:, sl], c31)):
             assert torch.allclose(a, b_, rtol=0, atol=1e-12 * float(b_.abs().max()))
         assert torch.equal(en[bi], en1)
+
+
+def test_empty_and_tiny_inputs(tb_fs2fs3, oracle, params_fs2fs3):
+    """nm = 0, 1, 2 and three far-apart molecules: no triplets or items, zero
+    energies, accumulated outputs untouched, vir unchanged."""
+    from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox, random_quaternions
+    rng = np.random.default_rng(2)
+    z = np.zeros(0)
+    q2, q3, vir = tb_fs2fs3.threebody_term(z, z, z, z, z, z, 10.0, z, z, z, z, z, z, z, z, z, 3.5, ld=28)
+    assert (q2, q3, vir) == (0.0, 0.0, 3.5)
+    for com in ([[0.0, 0.0, 0.0]], [[0.0, 0.0, 0.0], [2.9, 0.0, 0.0]],
+                [[0.0, 0.0, 0.0], [9.0, 0.0, 0.0], [0.0, 9.0, 0.0]]):
+        com = np.array(com)
+        box = WaterBox(com, random_quaternions(len(com), rng), 0.0)
+        print(f"tiny input: {len(com)} molecules", flush=True)
+        got = tb_fs2fs3.evaluate(box)
+        ref = oracle.threebody_term(params_fs2fs3, box)
+        assert got["n_fs3"] == ref["n_fs3"] == 0 and got["n_fs2_items"] == ref["n_fs2"] == 0
+        assert got["qfs2"] == 0.0 and got["qfs3"] == 0.0 and got["vir"] == 0.0
+        for k in ("fxs", "fys", "fzs", "fxfs2", "fxfs3"):
+            assert not np.asarray(got[k]).any(), k
+
+
+def test_open_cluster(tb_fs2fs3, oracle, params_fs2fs3):
+    """A 40-water cluster without periodic images (box <= 0)."""
+    from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox
+    b = lattice_box(40, seed=12)
+    box = WaterBox(b.com - b.com.mean(0), b.quats, 0.0)
+    print("open cluster 40 waters", flush=True)
+    got = tb_fs2fs3.evaluate(box)
+    ref = oracle.threebody_term(params_fs2fs3, box, nthreads=NTHREADS)
+    assert got["n_fs3"] > 0 and got["n_fs2_items"] > 0
+    assert_parity(got, ref)
+
+
+def _triangles(com_a, box_a, cut_a):
+    """Triangles of the graph 'COM distance < cut' (minimum image, the list
+    kernels' FP64 arithmetic in bohr): the FS3 triplet count when the cutoff is
+    below half the box, and a third of the FS2 item count."""
+    b2a = 0.529177249
+    x = com_a / b2a
+    L = box_a / b2a
+    d = x[:, None, :] - x[None, :, :]
+    d = d - L * (np.sign(d / L) * np.floor(np.abs(d / L) + 0.5))
+    r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    a = (r2 < (cut_a / b2a) ** 2).astype(np.float32)
+    np.fill_diagonal(a, 0.0)
+    return int(round(float(((a @ a) * a).sum(dtype=np.float64)) / 6))
+
+
+def test_large_box_lists_and_invariances(tb_fs2fs3, params_fs2fs3):
+    """2048 waters (L = 39.4 A, both cutoffs below L/2): triplet and item counts
+    against the triangles of the cutoff graphs; energies invariant and forces
+    equivariant under a permutation of the molecules; total force zero
+    (translation invariance of E)."""
+    from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox
+    box = lattice_box(2048, seed=31)
+    print("large box 2048 waters", flush=True)
+    got = tb_fs2fs3.evaluate(box)
+    assert got["n_fs3"] == _triangles(box.com, box.box, 6.3)
+    assert got["n_fs2_items"] == 3 * _triangles(box.com, box.box, params_fs2fs3.rcut3)
+    l3 = tb_fs2fs3.fs3_list()
+    assert len(l3) == got["n_fs3"] and np.all(np.diff(l3[:, 0]) >= 0)
+    assert np.all((l3[:, 0] < l3[:, 1]) & (l3[:, 1] < l3[:, 2]))
+    perm = np.random.default_rng(4).permutation(box.nm)
+    pb = WaterBox(box.com[perm], box.quats[perm], box.box)
+    gp = tb_fs2fs3.evaluate(pb)
+    assert gp["n_fs3"] == got["n_fs3"] and gp["n_fs2_items"] == got["n_fs2_items"]
+    for k in ("qfs2", "qfs3"):
+        assert abs(gp[k] - got[k]) <= 1e-11 * abs(got[k]), k
+    assert abs(gp["vir"] - got["vir"]) <= 1e-10 * abs(got["vir"])
+    fs, fsp = site_forces(got), site_forces(gp)
+    assert rel_rms(fsp, fs[:, perm]) <= 1e-11
+    for t in ("fs2", "fs3"):
+        assert rel_rms(com_forces(gp, t), com_forces(got, t)[:, perm]) <= 1e-9
+    total = fs.sum(axis=(1, 2)) + com_forces(got, "fs2").sum(1) + com_forces(got, "fs3").sum(1)
+    assert np.all(np.abs(total) <= 1e-10 * np.abs(fs).sum())
