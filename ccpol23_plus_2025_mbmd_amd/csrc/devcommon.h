@@ -64,6 +64,26 @@ __device__ __forceinline__ int block_append(bool flag, int val, int* q, int base
   return tot;
 }
 
+// 1/sqrt(x) and 1/x in FP64 for positive normal x: the hardware estimates
+// (v_rsq_f64, v_rcp_f64) refined by two Newton-Raphson steps each,
+// y <- y + y (1 - x y^2) / 2 and y <- y + y (1 - x y); a few ulp, against
+// ~27 and ~12 instructions for the IEEE 1/sqrt(x) and division.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double r = fma(-(x * y), y, 1.0);
+  y = fma(0.5 * y, r, y);
+  r = fma(-(x * y), y, 1.0);
+  return fma(0.5 * y, r, y);
+}
+
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -40,7 +40,7 @@ struct Fs2Geo {
 
 __device__ __forceinline__ void fs2_damp(double R, const Consts& K, double& f, double& fp) {
   const double pom = exp(K.alp3 * (R - K.r0dmp2));
-  const double pp = 1.0 / (1.0 + pom);
+  const double pp = rcp_nr(1.0 + pom);
   f = pow_delta(pp, K);
   fp = -K.alp3 * K.delta33 * pom * pp * f;  // df/dR
 }
@@ -49,12 +49,12 @@ __device__ __forceinline__ void fs2_damp(double R, const Consts& K, double& f, d
 // WANT_SITES: Fk[3 ks + c] += mult * dT/d(site ks)_c (the site derivatives are
 // used at once, so they need no registers beyond the current site);
 // dTdRi, dTdRj: explicit COM derivatives; Tl: dT/dlambda under uniform scaling
-// of all vectors except the fixed shift g.  Inverse distances use the FP64
-// reciprocal square root (within 1 ulp) and one reciprocal of R_ij.
+// of all vectors except the fixed shift g.  Inverse distances use rsqrt_nr
+// (hardware estimate and two Newton steps) and one reciprocal of R_ij.
 template <bool WANT_SITES, bool WANT_COM>
 __device__ __forceinline__ double fs2_T(const Fs2Geo& G, const double* __restrict__ sk, double* Fk, double mult,
                                         double* dTdRi, double* dTdRj, double& Tl) {
-  const double iRij = 1.0 / G.Rij;
+  const double iRij = rcp_nr(G.Rij);
   const double uij[3] = {G.xij[0] * iRij, G.xij[1] * iRij, G.xij[2] * iRij};
   const double gR = G3 * iRij;
   double T = 0.0;
@@ -72,10 +72,10 @@ __device__ __forceinline__ double fs2_T(const Fs2Geo& G, const double* __restric
       dqi[c] = dci[c] - G3 * uij[c];
       dqj[c] = dcj[c] + G3 * uij[c];
     }
-    const double ici = rsqrt(dci[0] * dci[0] + dci[1] * dci[1] + dci[2] * dci[2]);
-    const double icj = rsqrt(dcj[0] * dcj[0] + dcj[1] * dcj[1] + dcj[2] * dcj[2]);
-    const double iqi = rsqrt(dqi[0] * dqi[0] + dqi[1] * dqi[1] + dqi[2] * dqi[2]);
-    const double iqj = rsqrt(dqj[0] * dqj[0] + dqj[1] * dqj[1] + dqj[2] * dqj[2]);
+    const double ici = rsqrt_nr(dci[0] * dci[0] + dci[1] * dci[1] + dci[2] * dci[2]);
+    const double icj = rsqrt_nr(dcj[0] * dcj[0] + dcj[1] * dcj[1] + dcj[2] * dcj[2]);
+    const double iqi = rsqrt_nr(dqi[0] * dqi[0] + dqi[1] * dqi[1] + dqi[2] * dqi[2]);
+    const double iqj = rsqrt_nr(dqj[0] * dqj[0] + dqj[1] * dqj[1] + dqj[2] * dqj[2]);
     const double qci3 = q * ici * ici * ici, qcj3 = q * icj * icj * icj;
     const double qqi3 = q * iqi * iqi * iqi, qqj3 = q * iqj * iqj * iqj;
     const double term0 = ici + icj - iqi - iqj;
@@ -220,7 +220,9 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
       G.xkj[2] = sz[a] - sz[q];
       G.Rij = sR[q];
       G.Rki = sR[a];
-      G.Rkj = sqrt(G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2]);
+      const double r2kj = G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2];
+      const double iRkj = rsqrt_nr(r2kj);
+      G.Rkj = r2kj * iRkj;
       double dTdRi[3], dTdRj[3], Tl;
       const double T = fs2_T<false, true>(G, D.site + (long)snb[a] * NS3 * 3, nullptr, 0.0, dTdRi, dTdRj, Tl);
       const double fki = sF[a], fpki = sFp[a];
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
       fs2_damp(G.Rkj, K, fkj, fpkj);
       const double S = sS[ql];
       const double ff = fki * fkj, ffS = ff * S;
-      const double ci = S * T * fpki * fkj / G.Rki, cj = S * T * fki * fpkj / G.Rkj;
+      const double ci = S * T * fpki * fkj * rcp_nr(G.Rki), cj = S * T * fki * fpkj * iRkj;
       for (int c = 0; c < 3; ++c) {
         Fi[c] += -ffS * dTdRi[c] + ci * G.xki[c];
         atomicAdd(sFj + 3 * ql + c, -ffS * dTdRj[c] + cj * G.xkj[c]);
@@ -389,9 +391,12 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
       G.xkj[1] = G.xki[1] + G.xij[1];
       G.xkj[2] = G.xki[2] + G.xij[2];
       ++nitems;
-      G.Rij = sqrt(G.xij[0] * G.xij[0] + G.xij[1] * G.xij[1] + G.xij[2] * G.xij[2]);
+      const double r2ij = G.xij[0] * G.xij[0] + G.xij[1] * G.xij[1] + G.xij[2] * G.xij[2];
+      G.Rij = r2ij * rsqrt_nr(r2ij);
       G.Rki = sRk[a];
-      G.Rkj = sqrt(G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2]);
+      const double r2kj = G.xkj[0] * G.xkj[0] + G.xkj[1] * G.xkj[1] + G.xkj[2] * G.xkj[2];
+      const double iRkj = rsqrt_nr(r2kj);
+      G.Rkj = r2kj * iRkj;
       const long pr = (long)pofs[a] + qj;
       const double S = D.s2pair[2 * pr], Sl = D.s2pair[2 * pr + 1];
       const double fki = sF[a], fpki = sFp[a];
@@ -400,7 +405,7 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
       const double ff = fki * fkj, ffS = ff * S;
       double Tl;
       const double T = fs2_T<true, false>(G, skp, Fk, -ffS, nullptr, nullptr, Tl);
-      const double ci = S * T * fpki * fkj / G.Rki, cj = S * T * fki * fpkj / G.Rkj;
+      const double ci = S * T * fpki * fkj * rcp_nr(G.Rki), cj = S * T * fki * fpkj * iRkj;
       for (int c = 0; c < 3; ++c) Fk[NS2 * 3 + c] += -(ci * G.xki[c] + cj * G.xkj[c]);
       E += ff * S * T;
       W += ff * (Sl * T + S * Tl) + S * T * (G.Rki * fpki * fkj + G.Rkj * fki * fpkj);
