@@ -29,10 +29,11 @@ __constant__ int c_tile1[9] = {-1, 2, 4, 6, 8, 10, 12, 14, 16};
 __constant__ int c_tilet[9] = {0, 1, 2, 3, 3, 4, 4, 5, 5};
 
 // ====================================================================== FS3
-// Mixed-precision helpers: r from the FP64 r^2 with a FP32 square root, and a
-// FP32 reciprocal; the FP64 variant keeps sqrt and the IEEE division.
+// Distance and reciprocal of the pair stage: FP32 hardware square root and
+// reciprocal in the mixed kernels; in the FP64 kernels the hardware estimates
+// refined by two Newton steps (a few ulp; rsqrt_nr, rcp_nr).
 template <typename T> __device__ __forceinline__ T t_dist(double r2);
-template <> __device__ __forceinline__ double t_dist<double>(double r2) { return sqrt(r2); }
+template <> __device__ __forceinline__ double t_dist<double>(double r2) { return r2 * rsqrt_nr(r2); }
 // the hardware square root (1 ulp); r^2 >= ~1 bohr^2 here, far from denormals
 template <> __device__ __forceinline__ float t_dist<float>(double r2) { return __builtin_amdgcn_sqrtf(float(r2)); }
 // (beta, gamma, r0, -) of a site pair in the kernel's precision
@@ -40,7 +41,7 @@ template <typename T> struct PairPar;
 template <> struct PairPar<double> { using type = double4; };
 template <> struct PairPar<float> { using type = float4; };
 template <typename T> __device__ __forceinline__ T t_rcp(T x);
-template <> __device__ __forceinline__ double t_rcp<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ double t_rcp<double>(double x) { return rcp_nr(x); }
 template <> __device__ __forceinline__ float t_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Orbit -> dense-table entries (CSR over the 364 linear coefficients), set by
