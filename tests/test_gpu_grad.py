@@ -229,3 +229,29 @@ def test_refit_reproduces_energies_and_writes_tables(tmp_path, params_fs2fs3):
     _, _, _, en2 = tb2.eval_batch_device(ms, bl, com_t, site_t)
     fitted = phi @ c
     assert torch.allclose(en2[:, 1], fitted, rtol=1e-9, atol=1e-9 * scale)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autograd_matches_reference_analytic_forces(params_fs2fs3, trimers, oracle, precision):
+    """dE/d(COM) through torch.autograd equals minus the oracle's analytic
+    molecular force (sites + FS2 and FS3 COM terms) of every trimer: FP64 within
+    1e-9 relative RMS; mixed within its molecular-force error, 1e-3."""
+    from ccpol23_plus_2025_mbmd_amd.autograd import KCAL_PER_FORCE_UNIT, threebody_energy
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    from helpers import System, molecular_forces, run_oracle
+    tb = ThreeBody(params_fs2fs3, precision)
+    dev = torch.device("cuda:0")
+    sel = trimers[:24]
+    ms = np.cumsum([0] + [3] * len(sel)).astype(np.int32)
+    bl = np.zeros(len(sel))
+    pc = torch.tensor(principal_coords()[1], device=dev)
+    com = torch.tensor(np.concatenate([b.com for b in sel]), device=dev, requires_grad=True)
+    quat = torch.tensor(np.concatenate([b.quats for b in sel]), device=dev)
+    coef = torch.tensor(coef_vector(params_fs2fs3), device=dev)
+    print(f"autograd vs oracle forces: {len(sel)} trimers, {precision}", flush=True)
+    threebody_energy(tb, com, quat, coef, pc, ms, bl).sum().backward()
+    got = com.grad.cpu().numpy()
+    ref = np.concatenate([-molecular_forces(run_oracle(oracle, params_fs2fs3, System(b.com, b.site, 0.0, ld=28)), 17)
+                          for b in sel]) * KCAL_PER_FORCE_UNIT
+    err = np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum())
+    assert err <= (1e-9 if precision == "fp64" else 1e-3), err

@@ -108,3 +108,37 @@ def test_md_revcon_restart_continues_the_run(tmp_path, params_fs2fs3):
     np.testing.assert_allclose(d - L * np.round(d / L), 0.0, atol=1e-9)
     for key in ("quat", "vel", "jbody"):
         np.testing.assert_allclose(sc[key], sa[key], rtol=1e-9, atol=1e-12 * np.abs(sa[key]).max(), err_msg=key)
+
+
+def test_md_1000_steps_snapshot_parity(oracle, params_fs2fs3):
+    """1000 NVE steps of 0.5 fs, 256 waters (L = 19.71 A), FS2+FS3 in FP64, as
+    the device driver runs them; every 100 steps the snapshot's triplet and
+    item lists are bit-exact with the oracle's and its energies, forces and
+    virial are within the FP64 bar.  (With the three-body term as the only
+    force the trajectory is not physical after ~20 fs -- there is no two-body
+    repulsion -- but every snapshot is a valid geometry to compare on.)"""
+    from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox, lattice_box
+    from ccpol23_plus_2025_mbmd_amd.md import RigidMD, maxwell_state, water_constants
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    box = lattice_box(256, seed=17)
+    pc, mass, pmi = water_constants()
+    v, j = maxwell_state(box.nm, 298.0, mass, pmi, seed=5)
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    md = RigidMD(tb, box.com, box.quats, box.box, v, j)
+    for block in range(10):
+        rec = md.run(100, 5e-4)
+        assert np.all(np.isfinite(rec)), block
+        st = md.state()
+        snap = WaterBox(st["com"], st["quat"], box.box)
+        print(f"md snapshot at step {100 * (block + 1)}: E3 {rec[-1, 0]:.6f} kcal/mol", flush=True)
+        got = tb.evaluate(snap)
+        np.testing.assert_array_equal(tb.fs3_list(), oracle.fs3_list(params_fs2fs3, snap.x, snap.y, snap.z, snap.box))
+        np.testing.assert_array_equal(tb.fs2_list(), oracle.fs2_list(params_fs2fs3, snap.x, snap.y, snap.z, snap.box))
+        ref = oracle.threebody_term(params_fs2fs3, snap, nthreads=16)
+        for k in ("qfs2", "qfs3"):
+            assert abs(got[k] - ref[k]) <= 1e-10 * abs(ref[k]) + 2e-12, (block, k, got[k], ref[k])
+        f = np.stack([got[k][:, :17] for k in ("fxs", "fys", "fzs")])
+        fr = np.stack([ref[k][:, :17] for k in ("fxs", "fys", "fzs")])
+        assert np.sqrt(((f - fr) ** 2).sum() / (fr ** 2).sum()) <= 1e-9, block
+        assert abs(got["vir"] - ref["vir"]) <= 1e-9 * abs(ref["vir"]) + 1e-9, block
+    md.close()
