@@ -345,7 +345,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
         const int po = P * NPAIR + x * NS3 + y;
         const T r = sR[po];
         wsum += double(gacc[xs][ys]) * double(r);
-        sR[po] = gacc[xs][ys] / r;
+        sR[po] = gacc[xs][ys] * t_rcp<T>(r);
       }
     }
   }
@@ -507,14 +507,15 @@ __global__ __launch_bounds__(64) void k_energy_boxes(Dev D, int do2, int do3, do
 
 // Coefficients come through the vector cache; LDS per block is ~29 KB (FP64,
 // 5 blocks per CU, at the cost of a few spilled registers) and ~15 KB (mixed,
-// 7 blocks per CU by registers).  The mixed kernels take the table in the
-// shifted basis (MIXED_VSHIFT), the FP64 kernels the plain one.
+// 8 blocks per CU by registers, also with a few spills: measured 2-3 % faster
+// than 7 without).  The mixed kernels take the table in the shifted basis
+// (MIXED_VSHIFT), the FP64 kernels the plain one.
 __global__ __launch_bounds__(256, 5) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const double* __restrict__ coef) {
   fs3_eval_body<double, false, false, false>(D, K, pairpar, coef, nullptr);
 }
 
-__global__ __launch_bounds__(256, 7) void k_fs3_eval_f32(Dev D, Consts K, const float4* __restrict__ pairpar,
+__global__ __launch_bounds__(256, 8) void k_fs3_eval_f32(Dev D, Consts K, const float4* __restrict__ pairpar,
                                                          const float* __restrict__ coef) {
   fs3_eval_body<float, false, false, false>(D, K, pairpar, coef, nullptr);
 }
