@@ -239,11 +239,30 @@ def cpu_baseline(p, box, threads):
     r = O.threebody_term(p, box, nthreads=threads, rebuild=True)
     wall = time.perf_counter() - t0
     rate = r["n_fs3"] / wall
+    n3, n2 = r["n_fs3"], r["n_fs2"]
+    # one core: the lists in full, the evaluations on the first 1/16 of each list
+    # (list order), scaled to the whole call
+    w3, w2 = max(1, n3 // 16), max(1, n2 // 16)
+    log(f"cpu baseline: one thread, first {w3} FS3 triplets and {w2} FS2 items")
+    s = O.threebody_term(p, box, nthreads=1, rebuild=True, fs3_window=(0, w3), fs2_window=(0, w2))
+    t1 = s["t_fs3_list"] + s["t_fs2_list"] + s["t_fs3_eval"] * n3 / w3 + s["t_fs2_eval"] * n2 / w2
+    # the orbit table built once per thread instead of per triplet
+    log(f"cpu baseline: hoisted orbit table, {threads} threads, full call")
+    t0 = time.perf_counter()
+    O.threebody_term(p, box, nthreads=threads, rebuild=False)
+    wall_h = time.perf_counter() - t0
     return {"value": rate, "unit": "triplet evals/s", "cores": threads, "kind": "port",
-            "sample": (f"one full FS2+FS3 E+F call at {box.nm} waters ({r['n_fs3']} FS3 triplets, {r['n_fs2']} "
+            "sample": (f"one full FS2+FS3 E+F call at {box.nm} waters ({n3} FS3 triplets, {n2} "
                        f"FS2 items) of the C FP64 restatement in the reference's loop order with the "
                        f"per-triplet orbit-table rebuild, {threads} OpenMP threads, {wall:.2f} s wall"),
-            "ms_per_call": wall * 1e3}
+            "ms_per_call": wall * 1e3,
+            "one_core": {"value": n3 / t1, "cores": 1, "ms_per_call_estimate": t1 * 1e3,
+                         "sample": (f"the same restatement on one thread: both lists in full, the first {w3} "
+                                    f"FS3 triplets and {w2} FS2 items evaluated, eval time scaled by the "
+                                    f"list lengths")},
+            "hoisted_table": {"value": n3 / wall_h, "cores": threads, "ms_per_call": wall_h * 1e3,
+                              "sample": ("the same full call with the orbit table built once per thread "
+                                         "instead of per triplet (the reference's loop order otherwise)")}}
 
 
 def main():

@@ -23,10 +23,20 @@
 namespace ccpol3b {
 
 // 2x2 site tiles of one molecule's 17 sites, aligned to the type blocks
-// (O | H H | D1 D1 | N1 N1 | N1 N1 | N2 N2 | N2 N2 | N3 N3 | N3 N3); -1 pads O.
-__constant__ int c_tile0[9] = {0, 1, 3, 5, 7, 9, 11, 13, 15};
-__constant__ int c_tile1[9] = {-1, 2, 4, 6, 8, 10, 12, 14, 16};
-__constant__ int c_tilet[9] = {0, 1, 2, 3, 3, 4, 4, 5, 5};
+// (O | H H | D1 D1 | N1 N1 | N1 N1 | N2 N2 | N2 N2 | N3 N3 | N3 N3): tile b >= 1
+// holds sites 2b-1 and 2b; tile 0 holds O and a padding row (-1).
+__device__ __forceinline__ int tile_first(int b) { return b == 0 ? 0 : 2 * b - 1; }
+__device__ __forceinline__ int tile_second(int b) { return b == 0 ? -1 : 2 * b; }
+__device__ __forceinline__ int tile_type(int b) { return b < 3 ? b : (b + 3) >> 1; }
+// (pass, tile row, tile column) of tile thread q < 243
+struct TileId {
+  int P, x0, x1r, y0, y1r;
+};
+__device__ __forceinline__ TileId tile_of(int q) {
+  const int P = q / 81, rem = q - P * 81;
+  const int xb = rem / 9, yb = rem - xb * 9;
+  return {P, tile_first(xb), tile_second(xb), tile_first(yb), tile_second(yb)};
+}
 
 // ====================================================================== FS3
 // Distance and reciprocal of the pair stage: FP32 hardware square root and
@@ -111,10 +121,9 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   // tile of this thread (threads 0..242): pass P, 2x2 site tile (x0,x1) x (y0,y1)
   const bool tile = tid < 243;
   const int P = tid / 81, trem = tid - P * 81;
-  const int xb = trem / 9, yb = trem - xb * 9;
-  const int tsel = tile ? xb : 0, usel = tile ? yb : 0;
-  const int x0 = c_tile0[tsel], x1r = c_tile1[tsel], tx = c_tilet[tsel];
-  const int y0 = c_tile0[usel], y1r = c_tile1[usel], ty = c_tilet[usel];
+  const int xb = tile ? trem / 9 : 0, yb = tile ? trem - (trem / 9) * 9 : 0;
+  const int x0 = tile_first(xb), x1r = tile_second(xb), tx = tile_type(xb);
+  const int y0 = tile_first(yb), y1r = tile_second(yb), ty = tile_type(yb);
   // The second row of a tile is the next site (the O tile reads the first H
   // row and discards it).  Rows of v_xz and v_yz:
   //   P=0 (X=i,Y=j,Z=k): v_xz = ik[x][z], v_yz = jk[y][z]
@@ -333,16 +342,21 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     }
   }
   // the tile owner turns dE/dr into h = (dE/dr)/r and the site virial sum r dE/dr
+  // (the tile's sites are recomputed from an opaque copy of tid: cheaper than
+  // keeping them live across the region loop)
   double wsum = 0.0;
   if (tile) {
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
+    const TileId ti = tile_of(tq);
 #pragma unroll
     for (int xs = 0; xs < 2; ++xs) {
-      const int x = xs ? x1r : x0;
+      const int x = xs ? ti.x1r : ti.x0;
 #pragma unroll
       for (int ys = 0; ys < 2; ++ys) {
-        const int y = ys ? y1r : y0;
+        const int y = ys ? ti.y1r : ti.y0;
         if (x < 0 || y < 0) continue;
-        const int po = P * NPAIR + x * NS3 + y;
+        const int po = ti.P * NPAIR + x * NS3 + y;
         const T r = sR[po];
         wsum += double(gacc[xs][ys]) * double(r);
         sR[po] = gacc[xs][ys] * t_rcp<T>(r);
@@ -357,38 +371,46 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   const double Eb = g.w[0] * EX0 + g.w[1] * EX1;
   const double Et = g.fd * Eb;
   const double dEdR = g.dfd * Eb + g.fd * (g.wp[0] * EX0 + g.wp[1] * EX1);  // dE/dR_avg
-  // ---------------- chain rule: site forces from h
-  if (tid < 3 * NS3 * 3) {
-    const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
-    const int s = rem / 3, c = rem - s * 3;
-    const double aij = g.c[0][c], aik = g.c[1][c], ajk = g.c[2][c];
-    // site-pair vector component = (COM part + own site) -/+ other site; the
-    // first sum is the same for all 17 partners
-    double F = 0.0;
-    if (role == 0) {
-      const double bj = aij + sP[0][s][c], bk = aik + sP[0][s][c];
-#pragma unroll 4
-      for (int y = 0; y < NS3; ++y) F -= double(sR[s * NS3 + y]) * (bj - sP[1][y][c]);
-#pragma unroll 4
-      for (int z = 0; z < NS3; ++z) F -= double(sR[NPAIR + s * NS3 + z]) * (bk - sP[2][z][c]);
-    } else if (role == 1) {
-      const double bi = aij - sP[1][s][c], bk = ajk + sP[1][s][c];
-#pragma unroll 4
-      for (int x = 0; x < NS3; ++x) F += double(sR[x * NS3 + s]) * (bi + sP[0][x][c]);
-#pragma unroll 4
-      for (int z = 0; z < NS3; ++z) F -= double(sR[2 * NPAIR + s * NS3 + z]) * (bk - sP[2][z][c]);
-    } else {
-      const double bi = aik - sP[2][s][c], bj = ajk - sP[2][s][c];
-#pragma unroll 4
-      for (int x = 0; x < NS3; ++x) F += double(sR[NPAIR + x * NS3 + s]) * (bi + sP[0][x][c]);
-#pragma unroll 4
-      for (int y = 0; y < NS3; ++y) F += double(sR[2 * NPAIR + y * NS3 + s]) * (bj + sP[1][y][c]);
+  // ---------------- chain rule: site forces from h.  Thread q < 102 takes one
+  // (molecule role, own site s, partner molecule) and sums over the partner's
+  // 17 sites u.  The site-pair vector is c + o - p_u with the own site first in
+  // the block (c the block's COM vector, o and p_u the site offsets) and
+  // c + p_u - o with it second, so the own site's force is
+  // T + (eps c - o) H, H = sum_u h_u, T = sum_u h_u p_u, eps = -1 / +1.  One
+  // code path for every lane; the two partners of a (role, site) sit in
+  // adjacent lanes and are added there.
+  if (tid < 102) {
+    const int role = tid / 34, r = tid - role * 34, s = r >> 1, half = r & 1;
+    const int blk = half ? (role == 0 ? 1 : 2) : (role == 2 ? 1 : 0);  // ij, ik, jk
+    const int pm = half ? (role == 2 ? 1 : 2) : (role == 0 ? 1 : 0);   // partner role
+    const bool first = role == 0 || (role == 1 && half);
+    const int hstride = first ? 1 : NS3;
+    const T* hp = sR + blk * NPAIR + (first ? s * NS3 : s);
+    const double* pp = pm == 0 ? &sP[0][0][0] : pm == 1 ? &sP[1][0][0] : &sP[2][0][0];
+    double H = 0.0, T0 = 0.0, T1 = 0.0, T2 = 0.0;
+#pragma unroll
+    for (int u = 0; u < NS3; ++u) {
+      const double h = double(hp[u * hstride]);
+      H += h;
+      T0 = fma(h, pp[3 * u], T0);
+      T1 = fma(h, pp[3 * u + 1], T1);
+      T2 = fma(h, pp[3 * u + 2], T2);
     }
-    unsafeAtomicAdd(D.acc + (long)g.m[role] * ACC_W + rem, F);
-  } else if (tid < 3 * NS3 * 3 + 9) {
+    const double eps = first ? -1.0 : 1.0;
+    double F0 = fma(eps * g.c[blk][0] - sP[role][s][0], H, T0);
+    double F1 = fma(eps * g.c[blk][1] - sP[role][s][1], H, T1);
+    double F2 = fma(eps * g.c[blk][2] - sP[role][s][2], H, T2);
+    F0 += __shfl_xor(F0, 1, 64);
+    F1 += __shfl_xor(F1, 1, 64);
+    F2 += __shfl_xor(F2, 1, 64);
+    double* a = D.acc + (long)g.m[role] * ACC_W + s * 3;
+    unsafeAtomicAdd(a + (half ? 2 : 0), half ? F2 : F0);
+    if (!half) unsafeAtomicAdd(a + 1, F1);
+  } else if (tid >= 192 && tid < 201) {
     // COM forces from the R_avg dependence of f_d and w (:1130-1166, 1613-1665)
-    const int q = tid - 3 * NS3 * 3, role = q / 3, c = q - role * 3;
-    const double uij = g.c[0][c] / g.R[0], uik = g.c[1][c] / g.R[1], ujk = g.c[2][c] / g.R[2];
+    const int q = tid - 192, role = q / 3, c = q - role * 3;
+    const double uij = g.c[0][c] * rcp_nr(g.R[0]), uik = g.c[1][c] * rcp_nr(g.R[1]);
+    const double ujk = g.c[2][c] * rcp_nr(g.R[2]);
     const double g3 = -dEdR / 3.0;
     double F;
     if (role == 0) F = g3 * (uij + uik);

@@ -712,9 +712,9 @@ void orc_fs3_term(const orc_params *P, int nm, int ld,
   make_geom(nm, ld, ORC_NS3, x, y, z, xs, ys, zs, box, &G, &B);
   t0 = wall_seconds();
   n = fs3_list_bohr(&G, rc2, NULL, 0);
-  t1 = wall_seconds();
   list = (int *)malloc(sizeof(int) * 3 * (n ? n : 1));
   fs3_list_bohr(&G, rc2, list, n);
+  t1 = wall_seconds();
   first = opts->fs3_first < 0 ? 0 : opts->fs3_first;
   last = (opts->fs3_last < 0 || opts->fs3_last > n) ? n : opts->fs3_last;
   if (first > last) first = last;
@@ -749,7 +749,7 @@ void orc_fs3_term(const orc_params *P, int nm, int ld,
   finish(nm, ld, ORC_NS3, A, nt, qfs3, fxs, fys, fzs, fxfs3, fyfs3, fzfs3, vir);
   if (stats) {
     stats->t_fs3_list = t1 - t0;
-    stats->t_fs3_eval = t2 - t1 - (t1 - t0);
+    stats->t_fs3_eval = t2 - t1;
     stats->n_fs3 = n;
     stats->n_fs3_sr = hist[0];
     stats->n_fs3_sw = hist[1];
@@ -782,9 +782,9 @@ void orc_fs2_term(const orc_params *P, int nm, int ld,
   make_geom(nm, ld, 5, x, y, z, xs, ys, zs, box, &G, &B);
   t0 = wall_seconds();
   n = fs2_list_bohr(&G, rc2, NULL, 0);
-  t1 = wall_seconds();
   list = (int *)malloc(sizeof(int) * 3 * (n ? n : 1));
   fs2_list_bohr(&G, rc2, list, n);
+  t1 = wall_seconds();
   first = opts->fs2_first < 0 ? 0 : opts->fs2_first;
   last = (opts->fs2_last < 0 || opts->fs2_last > n) ? n : opts->fs2_last;
   if (first > last) first = last;
@@ -811,7 +811,7 @@ void orc_fs2_term(const orc_params *P, int nm, int ld,
   if (stats) {
     stats->n_fs2 = n;
     stats->t_fs2_list = t1 - t0;
-    stats->t_fs2_eval = t2 - t1 - (t1 - t0);
+    stats->t_fs2_eval = t2 - t1;
   }
   for (a = 0; a < nt; ++a) acc_free(&A[a]);
   free(A);

@@ -51,7 +51,7 @@ typedef struct {
   long n_fs3;          /* accepted FS3 triplets (all, regardless of sample window) */
   long n_fs3_sr, n_fs3_sw, n_fs3_lr; /* region histogram */
   long n_fs2;          /* accepted ordered FS2 items (k; i<j) */
-  double t_fs3_list, t_fs3_eval; /* wall seconds: O(N^3) scan, evaluation of the window */
+  double t_fs3_list, t_fs3_eval; /* wall seconds: O(N^3) count + fill scans, evaluation of the window */
   double t_fs2_list, t_fs2_eval;
 } orc_stats;
 
