@@ -24,19 +24,9 @@ namespace ccpol3b {
 
 // 2x2 site tiles of one molecule's 17 sites, aligned to the type blocks
 // (O | H H | D1 D1 | N1 N1 | N1 N1 | N2 N2 | N2 N2 | N3 N3 | N3 N3): tile b >= 1
-// holds sites 2b-1 and 2b; tile 0 holds O and a padding row (-1).
+// holds sites 2b-1 and 2b; tile 0 holds O (its second row is padding).
 __device__ __forceinline__ int tile_first(int b) { return b == 0 ? 0 : 2 * b - 1; }
-__device__ __forceinline__ int tile_second(int b) { return b == 0 ? -1 : 2 * b; }
 __device__ __forceinline__ int tile_type(int b) { return b < 3 ? b : (b + 3) >> 1; }
-// (pass, tile row, tile column) of tile thread q < 243
-struct TileId {
-  int P, x0, x1r, y0, y1r;
-};
-__device__ __forceinline__ TileId tile_of(int q) {
-  const int P = q / 81, rem = q - P * 81;
-  const int xb = rem / 9, yb = rem - xb * 9;
-  return {P, tile_first(xb), tile_second(xb), tile_first(yb), tile_second(yb)};
-}
 
 // ====================================================================== FS3
 // Distance and reciprocal of the pair stage: FP32 hardware square root and
@@ -76,9 +66,6 @@ __constant__ int c_nz[NTYPE] = {1, 2, 2, 4, 4, 4};
 //                dE/dr of its site pairs in registers across regions and ends
 //                with h = (dE/dr)/r and the site virial;
 //  chain rule:   h of each site pair -> site forces; dE/dR_avg -> COM forces.
-// TR: the site-pair blocks are also stored transposed, so both rows of every
-// tile are read with unit stride (more LDS, two address registers per tile);
-// without TR the column-wise passes step with stride 17.
 // PHI: also write the triplet's row of dE/dc (kcal/mol per unit coefficient;
 // 364 SR then 364 LR) to phi_trip[t][728].
 // CSM: stage the region's coefficient table in LDS; otherwise the tile threads
@@ -86,7 +73,7 @@ __constant__ int c_nz[NTYPE] = {1, 2, 2, 4, 4, 4};
 // pairpar: [2][289] (beta, gamma, r0, -) for SR and LR (FP32 copy for the mixed
 // kernels); coef: [2][NCOEF] dense
 // [tx][ty][tz][p][q][s].
-template <typename T, bool TR, bool PHI, bool CSM>
+template <typename T, bool PHI, bool CSM>
 __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename PairPar<T>::type* __restrict__ pairpar,
                                               const T* __restrict__ coef, double* __restrict__ phi_trip) {
   using T2 = typename Pair2<T>::type;
@@ -96,7 +83,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   // (FP32 square root), so T storage loses nothing.
   __shared__ T sC[CSM ? NCOEF : 1];
   __shared__ double sP[3][NS3][3];
-  __shared__ T2 sV[(TR ? 2 : 1) * NPAIR3];
+  __shared__ T2 sV[NPAIR3];
   __shared__ T sDer[NPAIR3];
   __shared__ T sR[NPAIR3];
   __shared__ double sRed[4];
@@ -112,37 +99,34 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     reinterpret_cast<double*>(&g)[tid] = reinterpret_cast<const double*>(D.geo3 + t)[tid];
   const int nreg = D.geo3[t].nreg, reg0 = D.geo3[t].reg0;
   if (tid < 3 * NS3 * 3) {
-    const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3;
+    // site positions in a frame centred on molecule i: s_i, s_j - x_ij and
+    // s_k - x_ik, so that every site-pair vector is a plain difference
+    // (x_jk = x_ik - x_ij for the derived vector of :882-884)
+    const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3, c = rem - (rem / 3) * 3;
     const int4 tr = D.tri[t];
     const int m = role == 0 ? tr.x : role == 1 ? tr.y : tr.z;
-    (&sP[0][0][0])[tid] = D.site[(long)m * NS3 * 3 + rem];
+    const double shift = role == 0 ? 0.0 : D.geo3[t].c[role - 1][c];
+    (&sP[0][0][0])[tid] = D.site[(long)m * NS3 * 3 + rem] - shift;
   }
 
-  // tile of this thread (threads 0..242): pass P, 2x2 site tile (x0,x1) x (y0,y1)
-  const bool tile = tid < 243;
-  const int P = tid / 81, trem = tid - P * 81;
-  const int xb = tile ? trem / 9 : 0, yb = tile ? trem - (trem / 9) * 9 : 0;
-  const int x0 = tile_first(xb), x1r = tile_second(xb), tx = tile_type(xb);
-  const int y0 = tile_first(yb), y1r = tile_second(yb), ty = tile_type(yb);
-  // The second row of a tile is the next site (the O tile reads the first H
-  // row and discards it).  Rows of v_xz and v_yz:
+  // tile of this thread (threads 0..242): pass P, 2x2 site tile (x0,x0+1) x
+  // (y0,y0+1); the O tile's second row (xv1 / yv1 false) is the first H row,
+  // read and discarded.  Rows of v_xz and v_yz:
   //   P=0 (X=i,Y=j,Z=k): v_xz = ik[x][z], v_yz = jk[y][z]
   //   P=1 (X=i,Y=k,Z=j): v_xz = ij[x][z], v_yz = jk[z][y]
   //   P=2 (X=j,Y=k,Z=i): v_xz = ij[z][x], v_yz = ik[z][y]
-  int aoff, boff, ao2, bo2, as, bs;
-  if (TR) {  // transposed blocks ji, ki, kj at 3, 4, 5
-    const int xzblk = P == 0 ? 1 : P == 1 ? 0 : 3;
-    const int yzblk = P == 0 ? 2 : P == 1 ? 5 : 4;
-    aoff = xzblk * NPAIR + x0 * NS3; boff = yzblk * NPAIR + y0 * NS3;
-    ao2 = bo2 = NS3; as = bs = 1;
-  } else {
-    const bool acol = P == 2, bcol = P != 0;
-    const int xzblk = P == 0 ? 1 : 0, yzblk = P == 2 ? 1 : 2;
-    aoff = xzblk * NPAIR + (acol ? x0 : x0 * NS3);
-    boff = yzblk * NPAIR + (bcol ? y0 : y0 * NS3);
-    ao2 = acol ? 1 : NS3; as = acol ? NS3 : 1;
-    bo2 = bcol ? 1 : NS3; bs = bcol ? NS3 : 1;
-  }
+  const bool tile = tid < 243;
+  const int P = tid / 81, trem = tid - P * 81;
+  const int xb = tile ? trem / 9 : 0, yb = tile ? trem - (trem / 9) * 9 : 0;
+  const int x0 = tile_first(xb), tx = tile_type(xb);
+  const int y0 = tile_first(yb), ty = tile_type(yb);
+  const bool xv1 = xb > 0, yv1 = yb > 0;
+  const int po00 = P * NPAIR + x0 * NS3 + y0;  // site pair (x0, y0) of block P
+  const bool acol = P == 2, bcol = P != 0;
+  const int aoff = (P == 0 ? NPAIR : 0) + (acol ? x0 : x0 * NS3);
+  const int boff = (P == 2 ? NPAIR : 2 * NPAIR) + (bcol ? y0 : y0 * NS3);
+  const int ao2 = acol ? 1 : NS3, as = acol ? NS3 : 1;
+  const int bo2 = bcol ? 1 : NS3, bs = bcol ? NS3 : 1;
   T gacc[2][2] = {{T(0), T(0)}, {T(0), T(0)}};  // fd w dE/dr of the tile's site pairs
   double eacc0 = 0.0, eacc1 = 0.0;
   for (int rr = 0; rr < nreg; ++rr) {
@@ -169,10 +153,9 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
         const int rem = x * NS3 + y;
         T rt;
         if (rr == 0 || sizeof(T) == 8) {  // FP64: recomputing costs fewer registers than the branch
-          const int ra = blk == 2 ? 1 : 0, rb = blk == 0 ? 1 : 2;
-          const double vx = g.c[blk][0] + (sP[ra][x][0] - sP[rb][y][0]);
-          const double vy = g.c[blk][1] + (sP[ra][x][1] - sP[rb][y][1]);
-          const double vz = g.c[blk][2] + (sP[ra][x][2] - sP[rb][y][2]);
+          const double* pa = &sP[0][0][0] + __umul24(blk == 2 ? NS3 + x : x, 3);
+          const double* pb = &sP[0][0][0] + __umul24((blk == 0 ? NS3 : 2 * NS3) + y, 3);
+          const double vx = pa[0] - pb[0], vy = pa[1] - pb[1], vz = pa[2] - pb[2];
           rt = t_dist<T>(vx * vx + vy * vy + vz * vz);
           if (rr == 0) sR[p] = rt;
         } else {
@@ -186,7 +169,6 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
         v.x = e;
         v.y = e * (rt - T(VSHIFT));
         sV[p] = v;
-        if (TR) sV[(blk + 3) * NPAIR + y * NS3 + x] = v;
         sDer[p] = -T(par.x) + T(par.y) * tg * damp;
         if constexpr (sizeof(T) == 4) {
           y += 1;
@@ -221,9 +203,8 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
         for (int xs = 0; xs < 2; ++xs)
 #pragma unroll
           for (int ys = 0; ys < 2; ++ys) {
-            const int x = xs ? x1r : x0, y = ys ? y1r : y0;
-            const bool ok = P == 0 && x >= 0 && y >= 0;
-            const T2 v = sV[ok ? x * NS3 + y : 0];
+            const bool ok = P == 0 && (!xs || xv1) && (!ys || yv1);
+            const T2 v = sV[ok ? po00 + xs * NS3 + ys : 0];
             vxy[xs][ys][0] = ok ? v.x : T(0);
             vxy[xs][ys][1] = ok ? v.y : T(0);
           }
@@ -301,12 +282,10 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
       const T fdw = T(g.fd * g.w[rr]);
 #pragma unroll
       for (int xs = 0; xs < 2; ++xs) {
-        const int x = xs ? x1r : x0;
 #pragma unroll
         for (int ys = 0; ys < 2; ++ys) {
-          const int y = ys ? y1r : y0;
-          if (x < 0 || y < 0) continue;
-          const int po = P * NPAIR + x * NS3 + y;
+          if ((xs && !xv1) || (ys && !yv1)) continue;
+          const int po = po00 + xs * NS3 + ys;
           const T2 v = sV[po];
           const T ev = v.x * G[xs][ys][0] + v.y * G[xs][ys][1];
           gacc[xs][ys] += fdw * (sDer[po] * ev + v.x * G[xs][ys][1]);
@@ -348,15 +327,14 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   if (tile) {
     int tq = tid;
     asm volatile("" : "+v"(tq));
-    const TileId ti = tile_of(tq);
+    const int Pq = tq / 81, rq = tq - Pq * 81, xq = rq / 9, yq = rq - (rq / 9) * 9;
+    const int pq00 = Pq * NPAIR + tile_first(xq) * NS3 + tile_first(yq);
 #pragma unroll
     for (int xs = 0; xs < 2; ++xs) {
-      const int x = xs ? ti.x1r : ti.x0;
 #pragma unroll
       for (int ys = 0; ys < 2; ++ys) {
-        const int y = ys ? ti.y1r : ti.y0;
-        if (x < 0 || y < 0) continue;
-        const int po = ti.P * NPAIR + x * NS3 + y;
+        if ((xs && xq == 0) || (ys && yq == 0)) continue;
+        const int po = pq00 + xs * NS3 + ys;
         const T r = sR[po];
         wsum += double(gacc[xs][ys]) * double(r);
         sR[po] = gacc[xs][ys] * t_rcp<T>(r);
@@ -373,12 +351,11 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   const double dEdR = g.dfd * Eb + g.fd * (g.wp[0] * EX0 + g.wp[1] * EX1);  // dE/dR_avg
   // ---------------- chain rule: site forces from h.  Thread q < 102 takes one
   // (molecule role, own site s, partner molecule) and sums over the partner's
-  // 17 sites u.  The site-pair vector is c + o - p_u with the own site first in
-  // the block (c the block's COM vector, o and p_u the site offsets) and
-  // c + p_u - o with it second, so the own site's force is
-  // T + (eps c - o) H, H = sum_u h_u, T = sum_u h_u p_u, eps = -1 / +1.  One
-  // code path for every lane; the two partners of a (role, site) sit in
-  // adjacent lanes and are added there.
+  // 17 sites u.  With the positions Q of the frame above, the force on the own
+  // site is -sum_u h_u (Q_own - Q_u) = T - Q_own H, H = sum_u h_u,
+  // T = sum_u h_u Q_u, whichever molecule comes first in the block.  One code
+  // path for every lane; the two partners of a (role, site) sit in adjacent
+  // lanes and are added there.
   if (tid < 102) {
     const int role = tid / 34, r = tid - role * 34, s = r >> 1, half = r & 1;
     const int blk = half ? (role == 0 ? 1 : 2) : (role == 2 ? 1 : 0);  // ij, ik, jk
@@ -396,10 +373,9 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
       T1 = fma(h, pp[3 * u + 1], T1);
       T2 = fma(h, pp[3 * u + 2], T2);
     }
-    const double eps = first ? -1.0 : 1.0;
-    double F0 = fma(eps * g.c[blk][0] - sP[role][s][0], H, T0);
-    double F1 = fma(eps * g.c[blk][1] - sP[role][s][1], H, T1);
-    double F2 = fma(eps * g.c[blk][2] - sP[role][s][2], H, T2);
+    double F0 = fma(-sP[role][s][0], H, T0);
+    double F1 = fma(-sP[role][s][1], H, T1);
+    double F2 = fma(-sP[role][s][2], H, T2);
     F0 += __shfl_xor(F0, 1, 64);
     F1 += __shfl_xor(F1, 1, 64);
     F2 += __shfl_xor(F2, 1, 64);
@@ -534,34 +510,34 @@ __global__ __launch_bounds__(64) void k_energy_boxes(Dev D, int do2, int do3, do
 // (MIXED_VSHIFT), the FP64 kernels the plain one.
 __global__ __launch_bounds__(256, 5) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const double* __restrict__ coef) {
-  fs3_eval_body<double, false, false, false>(D, K, pairpar, coef, nullptr);
+  fs3_eval_body<double, false, false>(D, K, pairpar, coef, nullptr);
 }
 
 __global__ __launch_bounds__(256, 8) void k_fs3_eval_f32(Dev D, Consts K, const float4* __restrict__ pairpar,
                                                          const float* __restrict__ coef) {
-  fs3_eval_body<float, false, false, false>(D, K, pairpar, coef, nullptr);
+  fs3_eval_body<float, false, false>(D, K, pairpar, coef, nullptr);
 }
 
 // The same with the region's coefficient table staged in LDS (a tuning choice).
 __global__ __launch_bounds__(256, 3) void k_fs3_eval_f64_lds(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                              const double* __restrict__ coef) {
-  fs3_eval_body<double, false, false, true>(D, K, pairpar, coef, nullptr);
+  fs3_eval_body<double, false, true>(D, K, pairpar, coef, nullptr);
 }
 
 __global__ __launch_bounds__(256, 7) void k_fs3_eval_f32_lds(Dev D, Consts K, const float4* __restrict__ pairpar,
                                                              const float* __restrict__ coef) {
-  fs3_eval_body<float, false, false, true>(D, K, pairpar, coef, nullptr);
+  fs3_eval_body<float, false, true>(D, K, pairpar, coef, nullptr);
 }
 
 // The same evaluation plus the per-triplet coefficient-gradient rows.
 __global__ __launch_bounds__(256, 2) void k_fs3_phi_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                         const double* __restrict__ coef, double* phi_trip) {
-  fs3_eval_body<double, false, true, true>(D, K, pairpar, coef, phi_trip);
+  fs3_eval_body<double, true, true>(D, K, pairpar, coef, phi_trip);
 }
 
 __global__ __launch_bounds__(256, 4) void k_fs3_phi_f32(Dev D, Consts K, const float4* __restrict__ pairpar,
                                                         const float* __restrict__ coef, double* phi_trip) {
-  fs3_eval_body<float, false, true, true>(D, K, pairpar, coef, phi_trip);
+  fs3_eval_body<float, true, true>(D, K, pairpar, coef, phi_trip);
 }
 
 // Coefficient gradient of each box: the sum, in triplet order, of its
