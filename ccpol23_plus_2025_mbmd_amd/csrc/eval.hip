@@ -355,7 +355,9 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   // site is -sum_u h_u (Q_own - Q_u) = T - Q_own H, H = sum_u h_u,
   // T = sum_u h_u Q_u, whichever molecule comes first in the block.  One code
   // path for every lane; the two partners of a (role, site) sit in adjacent
-  // lanes and are added there.
+  // lanes and are added there, and the sums are staged in LDS for the atomics.
+  double* sF = reinterpret_cast<double*>(sV);  // site forces [role][site][c]; the pair vectors are dead here
+  static_assert(sizeof(sV) >= 3 * NS3 * 3 * sizeof(double), "site-force staging");
   if (tid < 102) {
     const int role = tid / 34, r = tid - role * 34, s = r >> 1, half = r & 1;
     const int blk = half ? (role == 0 ? 1 : 2) : (role == 2 ? 1 : 0);  // ij, ik, jk
@@ -379,9 +381,11 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     F0 += __shfl_xor(F0, 1, 64);
     F1 += __shfl_xor(F1, 1, 64);
     F2 += __shfl_xor(F2, 1, 64);
-    double* a = D.acc + (long)g.m[role] * ACC_W + s * 3;
-    unsafeAtomicAdd(a + (half ? 2 : 0), half ? F2 : F0);
-    if (!half) unsafeAtomicAdd(a + 1, F1);
+    if (!half) {
+      sF[role * NS3 * 3 + s * 3] = F0;
+      sF[role * NS3 * 3 + s * 3 + 1] = F1;
+      sF[role * NS3 * 3 + s * 3 + 2] = F2;
+    }
   } else if (tid >= 192 && tid < 201) {
     // COM forces from the R_avg dependence of f_d and w (:1130-1166, 1613-1665)
     const int q = tid - 192, role = q / 3, c = q - role * 3;
@@ -393,6 +397,13 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     else if (role == 1) F = g3 * (-uij + ujk);
     else F = g3 * (-uik - ujk);
     unsafeAtomicAdd(D.acc + (long)g.m[role] * ACC_W + ACC_C3 + c, F);
+  }
+  __syncthreads();
+  // one FP64 atomic per site-force component, consecutive lanes on consecutive
+  // accumulator entries (51 per molecule)
+  if (tid < 3 * NS3 * 3) {
+    const int role = tid / (NS3 * 3);
+    unsafeAtomicAdd(D.acc + (long)g.m[role] * ACC_W + (tid - role * NS3 * 3), sF[tid]);
   }
   if (tid == 0) {
     D.ew3[2 * t] = Et;
