@@ -132,7 +132,7 @@ struct ccpol3b_ctx {
   float* d_coef32s = nullptr;    // [2][NCOEF] FP32, shifted pair-vector basis of the mixed kernels
   double* d_phitrip = nullptr;   // [cap_phi][728] per-triplet coefficient-gradient rows
   long cap_phi = 0;
-  ccpol3b_tuning tune{0, 4, 2, 32};
+  ccpol3b_tuning tune{0, 1, 4, 8};
   // creation parameters (Fortran-ordered 17x17 tables, current coefficients),
   // kept so a graph can own a private copy of the context
   std::vector<double> h_tab[6], h_cs[2];
@@ -770,7 +770,7 @@ int ccpol3b_autotune(ccpol3b_ctx* c, int nbox, const int* mol_start, const doubl
   const ccpol3b_tuning keep = c->tune;
   ccpol3b_tuning win = keep;
   double win_ms = 1e300;
-  const int parts[] = {2, 4, 8}, pparts[] = {1, 2, 4}, lwaves[] = {16, 64};
+  const int parts[] = {1, 2, 4}, pparts[] = {1, 2, 4}, lwaves[] = {8, 32};
   for (int lds = 0; lds < 2 && rc == CCPOL3B_OK; ++lds)
     for (int pp : parts)
       for (int pw : pparts)

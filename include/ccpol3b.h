@@ -157,9 +157,9 @@ int ccpol3b_set_coefficients(ccpol3b_ctx *ctx, const double *cs_sr, const double
 typedef struct {
   int fs3_lds_coef;           /* 1: FS3 blocks stage the region's coefficient table in LDS;
                                  0: they read it through the vector cache (default) */
-  int fs2_parts;              /* FS2 spectator-pass blocks per molecule, 1..8 (default 4) */
-  int fs2_pair_parts;         /* FS2 pair-pass blocks per molecule, 1..8 (default 2) */
-  int list_waves_per_cu;      /* FS3 sub-list sweep wavefronts per CU, 1..1024 (default 32) */
+  int fs2_parts;              /* FS2 spectator-pass blocks per molecule, 1..8 (default 1) */
+  int fs2_pair_parts;         /* FS2 pair-pass blocks per molecule, 1..8 (default 4) */
+  int list_waves_per_cu;      /* FS3 sub-list sweep wavefronts per CU, 1..1024 (default 8) */
 } ccpol3b_tuning;
 
 int ccpol3b_get_tuning(const ccpol3b_ctx *ctx, ccpol3b_tuning *tuning);
