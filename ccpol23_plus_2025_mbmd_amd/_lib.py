@@ -44,8 +44,7 @@ class Stats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("fs3_lds_coef", C.c_int), ("fs2_parts", C.c_int), ("fs2_pair_parts", C.c_int),
-                ("list_waves_per_cu", C.c_int)]
+    _fields_ = [("fs3_lds_coef", C.c_int), ("fs2_parts", C.c_int), ("fs2_pair_parts", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -36,6 +36,7 @@ __global__ void k_fs2_pair_pass(Dev, Consts);
 __global__ void k_fs2_spectator_pass(Dev, Consts);
 size_t fs2_lds_bytes(int cap_nb);
 size_t fs2_pair_lds_bytes(int cap_nb);
+size_t fs3_list_lds_bytes(int cap_nb);
 __global__ void k_epilogue(Dev, int, int, int, int, double*, double*, double*, double*, double*, long, long, long);
 __global__ void k_energy_boxes(Dev, int, int, double*);
 }  // namespace ccpol3b
@@ -132,7 +133,7 @@ struct ccpol3b_ctx {
   float* d_coef32s = nullptr;    // [2][NCOEF] FP32, shifted pair-vector basis of the mixed kernels
   double* d_phitrip = nullptr;   // [cap_phi][728] per-triplet coefficient-gradient rows
   long cap_phi = 0;
-  ccpol3b_tuning tune{0, 1, 4, 8};
+  ccpol3b_tuning tune{0, 1, 4};
   // creation parameters (Fortran-ordered 17x17 tables, current coefficients),
   // kept so a graph can own a private copy of the context
   std::vector<double> h_tab[6], h_cs[2];
@@ -194,7 +195,7 @@ void free_dev(ccpol3b_ctx* c) {
   dev_free(D.mol_box); dev_free(D.boxL); dev_free(D.box_start);
   dev_free(D.nb3_cnt); dev_free(D.nb3); dev_free(D.up3);
   dev_free(D.nb2_cnt); dev_free(D.nb2); dev_free(D.up2); dev_free(D.nbv2);
-  dev_free(D.sl3_cnt); dev_free(D.sl3_off); dev_free(D.sl2_cnt); dev_free(D.sl2_off);
+  dev_free(D.sl2_cnt); dev_free(D.sl2_off);
   dev_free(D.cnt3s); dev_free(D.off3s); dev_free(D.cnt2s); dev_free(D.off2s);
   dev_free(D.tri); dev_free(D.geo3); dev_free(D.pcnt2); dev_free(D.poff2); dev_free(D.items2);
   dev_free(D.acc); dev_free(D.ew3); dev_free(D.ew2);
@@ -216,15 +217,14 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
         (rc = dev_alloc(c, D.nb3_cnt, n)) || (rc = dev_alloc(c, D.nb2_cnt, n)) ||
         (rc = dev_alloc(c, D.up3, n)) || (rc = dev_alloc(c, D.up2, n)) ||
         (rc = dev_alloc(c, D.nb3, nn)) || (rc = dev_alloc(c, D.nb2, nn)) || (rc = dev_alloc(c, D.nbv2, 3 * nn)) ||
-        (rc = dev_alloc(c, D.sl3_cnt, n)) || (rc = dev_alloc(c, D.sl3_off, n + 1)) ||
         (rc = dev_alloc(c, D.sl2_cnt, n)) || (rc = dev_alloc(c, D.sl2_off, n + 1)) ||
         (rc = dev_alloc(c, D.pcnt2, n)) || (rc = dev_alloc(c, D.poff2, n + 1)) ||
-        (rc = dev_alloc(c, D.cnt3s, nn)) || (rc = dev_alloc(c, D.off3s, nn + 1)) ||
+        (rc = dev_alloc(c, D.cnt3s, n)) || (rc = dev_alloc(c, D.off3s, n + 1)) ||
         (rc = dev_alloc(c, D.cnt2s, nn)) || (rc = dev_alloc(c, D.off2s, nn + 1)) ||
         (rc = dev_alloc(c, D.s2pair, 2 * nn)) || (rc = dev_alloc(c, D.ew2, (size_t)n * FS2_PARTS_MAX * 2)) ||
         (rc = dev_alloc(c, D.acc, (size_t)n * ACC_W)))
       return rc;
-    D.cap_sl3 = D.cap_sl2 = D.cap_pairs2 = (long)nn;
+    D.cap_sl2 = D.cap_pairs2 = (long)nn;
     c->cap_nm = n;
   }
   if (nbox > c->cap_box) {
@@ -307,11 +307,12 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   hipLaunchKernelGGL(k_neighbours, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
   LAUNCHCHK(c, "neighbour lists");
-  // sub-list sweeps (grid-stride over the sub-lists, count then fill)
-  const long max_sl = std::min<long>((long)nm * D.cap_nb, (long)c->cu_count * c->tune.list_waves_per_cu);
   const size_t lds = fs2_lds_bytes(D.cap_nb), lds_p = fs2_pair_lds_bytes(D.cap_nb);
+  const size_t lds3 = fs3_list_lds_bytes(D.cap_nb);
   if (do2 && std::max(lds, lds_p) > 65536)
     return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the FS2 passes");
+  if (do3 && lds3 > 65536)
+    return fail(c, CCPOL3B_ECAPACITY, "FS3 neighbour count too large for the FS3 list build");
   EVREC(1);
   // Two branches from here: the FS2 passes on `st`, and the FS3 list build
   // (count, offsets with the per-box ranges, fill with the geometry) followed
@@ -333,9 +334,9 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   }
   EVREC(2);
   hipStream_t s3 = c->side;
-  if (do3) hipLaunchKernelGGL(k_fs3_list<0>, dim3(waves_grid(max_sl)), dim3(256), 0, s3, D, c->K);
+  if (do3) hipLaunchKernelGGL(k_fs3_list<0>, dim3(nm), dim3(64), lds3, s3, D, c->K);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s3, D, 1, do3, do2);
-  if (do3) hipLaunchKernelGGL(k_fs3_list<1>, dim3(waves_grid(max_sl)), dim3(256), 0, s3, D, c->K);
+  if (do3) hipLaunchKernelGGL(k_fs3_list<1>, dim3(nm), dim3(64), lds3, s3, D, c->K);
   LAUNCHCHK(c, "FS3 lists");
   const long cap3 = D.cap3;
   EVREC_SIDE(8);
@@ -744,7 +745,7 @@ int ccpol3b_get_tuning(const ccpol3b_ctx* c, ccpol3b_tuning* t) {
 
 int ccpol3b_set_tuning(ccpol3b_ctx* c, const ccpol3b_tuning* t) {
   if (!c || !t || t->fs2_parts < 1 || t->fs2_parts > FS2_PARTS_MAX || t->fs2_pair_parts < 1 ||
-      t->fs2_pair_parts > FS2_PARTS_MAX || t->list_waves_per_cu < 1 || t->list_waves_per_cu > 1024)
+      t->fs2_pair_parts > FS2_PARTS_MAX)
     return fail(c, CCPOL3B_EINVAL, "invalid tuning");
   c->tune = *t;
   c->tune.fs3_lds_coef = t->fs3_lds_coef ? 1 : 0;
@@ -770,24 +771,23 @@ int ccpol3b_autotune(ccpol3b_ctx* c, int nbox, const int* mol_start, const doubl
   const ccpol3b_tuning keep = c->tune;
   ccpol3b_tuning win = keep;
   double win_ms = 1e300;
-  const int parts[] = {1, 2, 4}, pparts[] = {1, 2, 4}, lwaves[] = {8, 32};
+  const int parts[] = {1, 2, 4}, pparts[] = {1, 2, 4};
   for (int lds = 0; lds < 2 && rc == CCPOL3B_OK; ++lds)
     for (int pp : parts)
-      for (int pw : pparts)
-        for (int lw : lwaves) {
-          if (rc) break;
-          c->tune = ccpol3b_tuning{lds, pp, pw, lw};
-          double tot = 0.0;
-          for (int r = 0; r <= reps && rc == CCPOL3B_OK; ++r) {
-            rc = run_checked(c, nm, nbox, ld, x, y, z, xs, ys, zs, fs, fs + (size_t)ld * nm,
-                             fs + 2 * (size_t)ld * nm, c2, c3, en, st);
-            if (rc == CCPOL3B_OK && r > 0) tot += event_ms(c->ev[0], c->ev[4]);
-          }
-          if (rc == CCPOL3B_OK && tot / reps < win_ms) {
-            win_ms = tot / reps;
-            win = c->tune;
-          }
+      for (int pw : pparts) {
+        if (rc) break;
+        c->tune = ccpol3b_tuning{lds, pp, pw};
+        double tot = 0.0;
+        for (int r = 0; r <= reps && rc == CCPOL3B_OK; ++r) {
+          rc = run_checked(c, nm, nbox, ld, x, y, z, xs, ys, zs, fs, fs + (size_t)ld * nm,
+                           fs + 2 * (size_t)ld * nm, c2, c3, en, st);
+          if (rc == CCPOL3B_OK && r > 0) tot += event_ms(c->ev[0], c->ev[4]);
         }
+        if (rc == CCPOL3B_OK && tot / reps < win_ms) {
+          win_ms = tot / reps;
+          win = c->tune;
+        }
+      }
   dev_free(fs); dev_free(c2); dev_free(c3); dev_free(en);
   if (rc) {
     c->tune = keep;

@@ -23,7 +23,6 @@ constexpr int T_NP2 = 1;     // FS2 pairs
 constexpr int T_NI2 = 2;     // FS2 items
 constexpr int T_NBOVER = 3;  // largest neighbour count when a list overflowed
 constexpr int T_HIST = 4;    // [4..6] FS3 region histogram SR, switch, LR
-constexpr int T_SL3 = 7;     // FS3 sub-lists (i; j)
 constexpr int T_SL2 = 8;     // FS2 sub-lists (k; i)
 constexpr int T_NI2L = 9;    // FS2 items of an exported list
 constexpr int T_NCH3 = 10;   // FS3 / FS2 energy reduction chunks
@@ -77,7 +76,7 @@ struct Dev {
   int nm, nbox, cap_nb;
   int fs2_parts;              // spectator-pass blocks per molecule (1..FS2_PARTS_MAX)
   int fs2_pair_parts;         // pair-pass blocks per molecule (1..FS2_PARTS_MAX)
-  long cap3, cap_pairs2, cap_items2, cap_sl3, cap_sl2;
+  long cap3, cap_pairs2, cap_items2, cap_sl2;
   // geometry in bohr
   double *cx, *cy, *cz;   // [nm]
   double *site;           // [nm][17][3] site offsets (bohr)
@@ -88,10 +87,9 @@ struct Dev {
   int *nb3_cnt, *nb3, *up3;   // FS3 cutoff: [nm], [nm][cap_nb], [nm]
   int *nb2_cnt, *nb2, *up2;   // FS2 cutoff
   double *nbv2;               // [nm][3][cap_nb] x_i - x_nb (min image, bohr) of each FS2 entry
-  // sub-list counts/offsets: FS3 (i; j) and FS2 (k; i)
-  int *sl3_cnt, *sl3_off;     // [nm], [nm+1]
-  int *sl2_cnt, *sl2_off;
-  int *cnt3s, *off3s;         // [cap_sl3], [cap_sl3+1] triplets per FS3 sub-list
+  // FS2 sub-list (k; i) counts/offsets
+  int *sl2_cnt, *sl2_off;     // [nm], [nm+1]
+  int *cnt3s, *off3s;         // [nm], [nm+1] FS3 triplets per molecule i (first of the triplet)
   int *cnt2s, *off2s;         // [cap_sl2], [cap_sl2+1] items per FS2 sub-list
   // FS3 triplets (i<j<k, lexicographic)
   int4 *tri;                  // [cap3] (i,j,k,-)

@@ -6,10 +6,10 @@
 // zero, squares are summed left to right, and this file is compiled without
 // FMA contraction (the reference's x86-64 build has no FMA).
 //
-// Work decomposition: one wavefront per directed neighbour pair, so a 512-water
-// box gives ~10^4 independent wavefronts for the list sweeps.  Lists come out in
-// the reference's loop orders: FS3 (i,j,k) lexicographic with i<j<k (:859-860),
-// FS2 (k,i,j) with i<j (:297-301).
+// Work decomposition: one wavefront per molecule for the neighbour sweep and
+// the FS3 triplets, one per directed neighbour pair for the exported FS2 items.
+// Lists come out in the reference's loop orders: FS3 (i,j,k) lexicographic
+// with i<j<k (:859-860), FS2 (k,i,j) with i<j (:297-301).
 #pragma clang fp contract(off)
 
 #include "devcommon.h"
@@ -116,91 +116,104 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D, Consts K) {
     D.nb2_cnt[i] = n2;
     D.up3[i] = u3 < n3 ? u3 : n3;  // first upper position (entries < i come first)
     D.up2[i] = u2 < n2 ? u2 : n2;
-    D.sl3_cnt[i] = n3 - D.up3[i];  // FS3 sub-lists (i; j) over upper neighbours j
     D.sl2_cnt[i] = n2;             // FS2 sub-lists (k; i) over all neighbours i
     D.pcnt2[i] = n2 - D.up2[i];    // FS2 pairs i<j
   }
 }
 
 // ------------------------------------------------------------------ FS3 triplets
-// One wavefront per sub-list (i; j = nb3(i)[a], a >= first upper entry).  The
-// candidates k are the later entries of nb3(i) (so i-j and i-k passed the
-// minimum-image test); the j-k test uses the reference's derived vector
-// x_jk = dx_jk + L*nint(dx_ij/L) - L*nint(dx_ik/L) (:882-884).
-// mode 0 counts into cnt3s[s]; mode 1 writes at off3s[s] in ascending k, and
-// the lane that emits a triplet also writes its geometry (TriGeo) and counts
-// its region in the SR / switch / LR histogram.
+// One wavefront (one 64-thread block) per molecule i.  Its upper FS3
+// neighbours j > i (ascending, from the neighbour sweep) are staged in LDS with
+// their positions and integer images n_ij = nint((x_i - x_j)/L).  The wavefront
+// then takes the pairs (j, k), j < k, of those neighbours in the reference's
+// order (j, then k: :859-860), 64 at a time, and tests the derived vector
+// x_jk = dx_jk + L*n_ij - L*n_ik (:882-884) with the reference's arithmetic.
+// mode 0 counts per molecule into cnt3s[i]; mode 1 writes from off3s[i] in the
+// same order, with each triplet's geometry (TriGeo) and the SR / switch / LR
+// histogram.  Dynamic LDS: 6 doubles per neighbour slot.
+size_t fs3_list_lds_bytes(int cap_nb) { return (size_t)cap_nb * 6 * sizeof(double); }
+
+// pair (r, s), r < s < m, of flat index t in row-major order over r
+__device__ __forceinline__ void pair_of(long t, int m, int& r, int& s) {
+  const double b = 2.0 * m - 1.0;
+  int q = (int)((b - sqrt(b * b - 8.0 * (double)t)) * 0.5);  // rows before t: S_r = r m - r (r+1) / 2
+  q = q < 0 ? 0 : (q > m - 2 ? m - 2 : q);
+  while (q > 0 && (long)q * m - (long)q * (q + 1) / 2 > t) --q;
+  while (q < m - 2 && (long)(q + 1) * m - (long)(q + 1) * (q + 2) / 2 <= t) ++q;
+  r = q;
+  s = q + 1 + (int)(t - ((long)q * m - (long)q * (q + 1) / 2));
+}
+
 template <int mode>
-__global__ __launch_bounds__(256) void k_fs3_list(Dev D, Consts K) {
+__global__ __launch_bounds__(64) void k_fs3_list(Dev D, Consts K) {
+  extern __shared__ double lsm[];
   __shared__ int shist[3];
-  if (threadIdx.x < 3) shist[threadIdx.x] = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const long nsl = D.totals[T_SL3];
-  const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
-  for (long s = (long)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); s < nsl; s += nwaves) {
-  const int i = owner_of(D.sl3_off, D.nm, s);
-  const int a = D.up3[i] + (int)(s - D.sl3_off[i]);
-  const int* nb = D.nb3 + (long)i * D.cap_nb;
-  const int n = D.nb3_cnt[i];
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x;
+  if (i >= D.nm) return;  // the whole block
+  if (lane < 3) shist[lane] = 0;
+  const int cap = D.cap_nb;
+  double* px = lsm;
+  double* py = px + cap;
+  double* pz = py + cap;
+  double* qx = pz + cap;  // n_ij (integer-valued)
+  double* qy = qx + cap;
+  double* qz = qy + cap;
+  const int n = D.nb3_cnt[i], up = D.up3[i], m = n - up;
+  const int* nb = D.nb3 + (long)i * cap + up;
   const double L = D.boxL[D.mol_box[i]];
   const double xi = D.cx[i], yi = D.cy[i], zi = D.cz[i];
-  const int j = nb[a];
-  const double xj = D.cx[j], yj = D.cy[j], zj = D.cz[j];
-  double nxij = 0.0, nyij = 0.0, nzij = 0.0;
-  if (L > 0.0) {
-    nxij = round((xi - xj) / L);
-    nyij = round((yi - yj) / L);
-    nzij = round((zi - zj) / L);
+  for (int a = lane; a < m; a += 64) {
+    const int j = nb[a];
+    const double xj = D.cx[j], yj = D.cy[j], zj = D.cz[j];
+    px[a] = xj;
+    py[a] = yj;
+    pz[a] = zj;
+    qx[a] = L > 0.0 ? round((xi - xj) / L) : 0.0;
+    qy[a] = L > 0.0 ? round((yi - yj) / L) : 0.0;
+    qz[a] = L > 0.0 ? round((zi - zj) / L) : 0.0;
   }
-  const double cij[3] = {(xi - xj) - L * nxij, (yi - yj) - L * nyij, (zi - zj) - L * nzij};
-  const long out = mode ? D.off3s[s] : 0;
+  __syncthreads();
+  const long npair = (long)m * (m - 1) / 2;
+  const long out = mode ? D.off3s[i] : 0;
   int count = 0;
-  for (int bb = a + 1; bb < n; bb += 64) {
-    const int q = bb + lane;
+  for (long t0 = 0; t0 < npair; t0 += 64) {
+    const long t = t0 + lane;
     bool ok = false;
-    int k = 0;
-    double cik[3] = {0.0, 0.0, 0.0}, cjk[3] = {0.0, 0.0, 0.0};
-    if (q < n) {
-      k = nb[q];
-      const double xk = D.cx[k], yk = D.cy[k], zk = D.cz[k];
-      double xjk = xj - xk, yjk = yj - yk, zjk = zj - zk;
-      double nxik = 0.0, nyik = 0.0, nzik = 0.0;
+    int r = 0, s = 0;
+    double xjk = 0.0, yjk = 0.0, zjk = 0.0;
+    if (t < npair) {
+      pair_of(t, m, r, s);
+      xjk = px[r] - px[s];
+      yjk = py[r] - py[s];
+      zjk = pz[r] - pz[s];
       if (L > 0.0) {
-        nxik = round((xi - xk) / L);
-        nyik = round((yi - yk) / L);
-        nzik = round((zi - zk) / L);
-        xjk = xjk + L * nxij - L * nxik;
-        yjk = yjk + L * nyij - L * nyik;
-        zjk = zjk + L * nzij - L * nzik;
+        xjk = xjk + L * qx[r] - L * qx[s];
+        yjk = yjk + L * qy[r] - L * qy[s];
+        zjk = zjk + L * qz[r] - L * qz[s];
       }
       ok = sq3(xjk, yjk, zjk) < K.rc2_3;
-      cik[0] = (xi - xk) - L * nxik;
-      cik[1] = (yi - yk) - L * nyik;
-      cik[2] = (zi - zk) - L * nzik;
-      cjk[0] = xjk;
-      cjk[1] = yjk;
-      cjk[2] = zjk;
     }
-    const uint64_t m = __ballot(ok);
+    const uint64_t msk = __ballot(ok);
     if (mode == 1 && ok) {
-      const long p = out + count + __popcll(m & lanemask_lt());
+      const long p = out + count + __popcll(msk & lanemask_lt());
       if (p < D.cap3) {
-        D.tri[p] = make_int4(i, j, k, 0);
+        D.tri[p] = make_int4(i, nb[r], nb[s], 0);
+        const double cij[3] = {(xi - px[r]) - L * qx[r], (yi - py[r]) - L * qy[r], (zi - pz[r]) - L * qz[r]};
+        const double cik[3] = {(xi - px[s]) - L * qx[s], (yi - py[s]) - L * qy[s], (zi - pz[s]) - L * qz[s]};
+        const double cjk[3] = {xjk, yjk, zjk};
         TriGeo g;
-        tri_geometry(cij, cik, cjk, K, i, j, k, g);
+        tri_geometry(cij, cik, cjk, K, i, nb[r], nb[s], g);
         D.geo3[p] = g;
         atomicAdd(shist + (g.nreg == 2 ? 1 : 2 * g.reg0), 1);
       }
     }
-    count += __popcll(m);
+    count += __popcll(msk);
   }
-  if (mode == 0 && lane == 0) D.cnt3s[s] = count;
-  }
-  // one global atomic per block and region (same-address atomics from every
-  // triplet would serialise)
+  if (mode == 0 && lane == 0) D.cnt3s[i] = count;
+  // one global atomic per molecule and region
   __syncthreads();
-  if (mode && threadIdx.x < 3 && shist[threadIdx.x] > 0) atomicAdd(D.totals + T_HIST + threadIdx.x, shist[threadIdx.x]);
+  if (mode && lane < 3 && shist[lane] > 0) atomicAdd(D.totals + T_HIST + lane, shist[lane]);
 }
 template __global__ void k_fs3_list<0>(Dev, Consts);
 template __global__ void k_fs3_list<1>(Dev, Consts);
@@ -313,14 +326,14 @@ __device__ void block_exclusive_scan(const int* cnt, int* off, long n, int* tota
 __device__ void box_ranges(const Dev& D, int do3, int do2) {
   for (int b = threadIdx.x; b <= D.nbox; b += blockDim.x) {
     const int m = D.box_start[b];
-    const int t3 = do3 ? (m < D.nm ? D.off3s[D.sl3_off[m]] : D.totals[T_N3]) : 0;
+    const int t3 = do3 ? (m < D.nm ? D.off3s[m] : D.totals[T_N3]) : 0;
     const int t2 = do2 ? m * D.fs2_parts : 0;
     D.box_t3[b] = t3;
     D.box_t2[b] = t2;
     int c3 = 0, c2 = 0;
     if (b < D.nbox) {
       const int m1 = D.box_start[b + 1];
-      const int e3 = do3 ? (m1 < D.nm ? D.off3s[D.sl3_off[m1]] : D.totals[T_N3]) : 0;
+      const int e3 = do3 ? (m1 < D.nm ? D.off3s[m1] : D.totals[T_N3]) : 0;
       const int e2 = do2 ? m1 * D.fs2_parts : 0;
       c3 = (e3 - t3 + RED_CHUNK - 1) / RED_CHUNK;
       c2 = (e2 - t2 + RED_CHUNK - 1) / RED_CHUNK;
@@ -331,19 +344,18 @@ __device__ void box_ranges(const Dev& D, int do3, int do2) {
   __syncthreads();
 }
 
-// phase 0: per-molecule counts -> sub-list and pair offsets
-// phase 1: per-sub-list candidate counts -> FS3 triplet offsets; then the
+// phase 0: per-molecule counts -> FS2 sub-list and pair offsets
+// phase 1: per-molecule FS3 triplet counts -> triplet offsets; then the
 //          per-box triplet ranges and the first reduction chunk of each box
 // phase 2: per-sub-list candidate counts -> FS2 item offsets (list export only)
 __global__ __launch_bounds__(1024) void k_scan(Dev D, int phase, int do3, int do2) {
   if (phase == 0) {
-    if (do3) block_exclusive_scan(D.sl3_cnt, D.sl3_off, D.nm, D.totals + T_SL3);
     if (do2) {
       block_exclusive_scan(D.sl2_cnt, D.sl2_off, D.nm, D.totals + T_SL2);
       block_exclusive_scan(D.pcnt2, D.poff2, D.nm, D.totals + T_NP2);
     }
   } else if (phase == 1) {
-    if (do3) block_exclusive_scan(D.cnt3s, D.off3s, D.totals[T_SL3], D.totals + T_N3);
+    if (do3) block_exclusive_scan(D.cnt3s, D.off3s, D.nm, D.totals + T_N3);
     box_ranges(D, do3, do2);
     block_exclusive_scan(D.box_c3, D.box_c3, D.nbox, D.totals + T_NCH3);
     block_exclusive_scan(D.box_c2, D.box_c2, D.nbox, D.totals + T_NCH2);

@@ -159,7 +159,6 @@ typedef struct {
                                  0: they read it through the vector cache (default) */
   int fs2_parts;              /* FS2 spectator-pass blocks per molecule, 1..8 (default 1) */
   int fs2_pair_parts;         /* FS2 pair-pass blocks per molecule, 1..8 (default 4) */
-  int list_waves_per_cu;      /* FS3 sub-list sweep wavefronts per CU, 1..1024 (default 8) */
 } ccpol3b_tuning;
 
 int ccpol3b_get_tuning(const ccpol3b_ctx *ctx, ccpol3b_tuning *tuning);

@@ -89,8 +89,8 @@ def test_autotune_keeps_results(params_fs2fs3, batch):
     assert torch.allclose(en1, en0, rtol=1e-12, atol=1e-12)
     assert torch.allclose(fs1, fs0, rtol=1e-10, atol=1e-10 * fs0.abs().max().item())
     # every launch shape gives the same answer
-    for shape in ({"fs3_lds_coef": 1, "fs2_parts": 1, "fs2_pair_parts": 1, "list_waves_per_cu": 4},
-                  {"fs3_lds_coef": 0, "fs2_parts": 8, "fs2_pair_parts": 8, "list_waves_per_cu": 512}):
+    for shape in ({"fs3_lds_coef": 1, "fs2_parts": 1, "fs2_pair_parts": 1},
+                  {"fs3_lds_coef": 0, "fs2_parts": 8, "fs2_pair_parts": 8}):
         tb.set_tuning(**shape)
         fs2, _, _, en2 = tb.eval_batch_device(ms, bl, com, site)
         assert torch.allclose(en2, en0, rtol=1e-12, atol=1e-12), shape
