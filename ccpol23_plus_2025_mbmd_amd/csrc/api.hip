@@ -21,7 +21,7 @@ __global__ void k_prep(int, int, const double*, const double*, const double*, co
 __global__ void k_neighbours(Dev, Consts);
 template <int mode> __global__ void k_fs3_list(Dev, Consts);
 __global__ void k_fs2_items(Dev, Consts, int);
-__global__ void k_scan(Dev, int, int, int);
+template <int NT> __global__ void k_scan(Dev, int, int, int);
 __global__ void k_fs3_eval_f64(Dev, Consts, const double4*, const double*);
 __global__ void k_fs3_eval_f32(Dev, Consts, const float4*, const float*);
 __global__ void k_fs3_eval_f64_lds(Dev, Consts, const double4*, const double*);
@@ -305,7 +305,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   const long nprep = std::max<long>((long)nm * ACC_W, T_COUNT);
   hipLaunchKernelGGL(k_prep, dim3((nprep + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
   hipLaunchKernelGGL(k_neighbours, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
+  hipLaunchKernelGGL(k_scan<1024>, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
   LAUNCHCHK(c, "neighbour lists");
   const size_t lds = fs2_lds_bytes(D.cap_nb), lds_p = fs2_pair_lds_bytes(D.cap_nb);
   const size_t lds3 = fs3_list_lds_bytes(D.cap_nb);
@@ -335,7 +335,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   EVREC(2);
   hipStream_t s3 = c->side;
   if (do3) hipLaunchKernelGGL(k_fs3_list<0>, dim3(nm), dim3(64), lds3, s3, D, c->K);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s3, D, 1, do3, do2);
+  hipLaunchKernelGGL(k_scan<256>, dim3(1), dim3(256), 0, s3, D, 1, do3, do2);
   if (do3) hipLaunchKernelGGL(k_fs3_list<1>, dim3(nm), dim3(64), lds3, s3, D, c->K);
   LAUNCHCHK(c, "FS3 lists");
   const long cap3 = D.cap3;
@@ -1006,7 +1006,7 @@ long ccpol3b_get_fs2_list(ccpol3b_ctx* c, int* kij, long cap) {
   const hipStream_t st = nullptr;
   const long nsl2 = c->h_totals[T_COUNT + T_SL2];
   if (nsl2 > 0) hipLaunchKernelGGL(k_fs2_items, dim3(waves_grid(nsl2)), dim3(256), 0, st, D, c->K, 0);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, D, 2, 0, 1);
+  hipLaunchKernelGGL(k_scan<1024>, dim3(1), dim3(1024), 0, st, D, 2, 0, 1);
   LAUNCHCHK(c, "FS2 list count");
   int ni = 0;
   HIPCHK(c, hipMemcpy(&ni, D.totals + T_NI2L, sizeof(int), hipMemcpyDeviceToHost));

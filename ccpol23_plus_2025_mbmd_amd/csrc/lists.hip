@@ -264,23 +264,24 @@ __global__ __launch_bounds__(256) void k_fs2_items(Dev D, Consts K, int mode) {
 }
 
 // ------------------------------------------------------------------ scans
-// Exclusive scan of n ints by one 1024-thread block, tile by tile through LDS
+// Exclusive scan of n ints by one block of NT threads, tile by tile through LDS
 // (coalesced loads, 8 consecutive elements per thread); off[n] and *total get
 // the sum.
 constexpr int SCAN_PER = 8;
-constexpr int SCAN_TILE = 1024 * SCAN_PER;
 
+template <int NT>
 __device__ void block_exclusive_scan(const int* cnt, int* off, long n, int* total) {
-  __shared__ int tile[SCAN_TILE];
-  __shared__ long wsum[16];
+  constexpr int TILE = NT * SCAN_PER;
+  __shared__ int tile[TILE];
+  __shared__ long wsum[NT / 64];
   __shared__ long carry_s;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (t == 0) carry_s = 0;
   __syncthreads();
-  for (long base = 0; base < n; base += SCAN_TILE) {
+  for (long base = 0; base < n; base += TILE) {
     for (int r = 0; r < SCAN_PER; ++r) {
-      const long q = base + r * 1024 + t;
-      tile[r * 1024 + t] = q < n ? cnt[q] : 0;
+      const long q = base + r * NT + t;
+      tile[r * NT + t] = q < n ? cnt[q] : 0;
     }
     __syncthreads();
     long loc[SCAN_PER];
@@ -303,11 +304,11 @@ __device__ void block_exclusive_scan(const int* cnt, int* off, long n, int* tota
     const long excl = carry + wbase + incl - s;
     __syncthreads();
     for (int r = 0; r < SCAN_PER; ++r) tile[t * SCAN_PER + r] = (int)(excl + loc[r]);
-    if (t == 1023) carry_s = excl + s;
+    if (t == NT - 1) carry_s = excl + s;
     __syncthreads();
     for (int r = 0; r < SCAN_PER; ++r) {
-      const long q = base + r * 1024 + t;
-      if (q < n) off[q] = tile[r * 1024 + t];
+      const long q = base + r * NT + t;
+      if (q < n) off[q] = tile[r * NT + t];
     }
     __syncthreads();
   }
@@ -348,20 +349,26 @@ __device__ void box_ranges(const Dev& D, int do3, int do2) {
 // phase 1: per-molecule FS3 triplet counts -> triplet offsets; then the
 //          per-box triplet ranges and the first reduction chunk of each box
 // phase 2: per-sub-list candidate counts -> FS2 item offsets (list export only)
-__global__ __launch_bounds__(1024) void k_scan(Dev D, int phase, int do3, int do2) {
+// One block of NT threads: 1024 before the FS2 / FS3 fork and for the export,
+// 256 for phase 1, which runs beside the FS2 passes and would otherwise wait
+// for a CU with 16 free wavefront slots.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_scan(Dev D, int phase, int do3, int do2) {
   if (phase == 0) {
     if (do2) {
-      block_exclusive_scan(D.sl2_cnt, D.sl2_off, D.nm, D.totals + T_SL2);
-      block_exclusive_scan(D.pcnt2, D.poff2, D.nm, D.totals + T_NP2);
+      block_exclusive_scan<NT>(D.sl2_cnt, D.sl2_off, D.nm, D.totals + T_SL2);
+      block_exclusive_scan<NT>(D.pcnt2, D.poff2, D.nm, D.totals + T_NP2);
     }
   } else if (phase == 1) {
-    if (do3) block_exclusive_scan(D.cnt3s, D.off3s, D.nm, D.totals + T_N3);
+    if (do3) block_exclusive_scan<NT>(D.cnt3s, D.off3s, D.nm, D.totals + T_N3);
     box_ranges(D, do3, do2);
-    block_exclusive_scan(D.box_c3, D.box_c3, D.nbox, D.totals + T_NCH3);
-    block_exclusive_scan(D.box_c2, D.box_c2, D.nbox, D.totals + T_NCH2);
+    block_exclusive_scan<NT>(D.box_c3, D.box_c3, D.nbox, D.totals + T_NCH3);
+    block_exclusive_scan<NT>(D.box_c2, D.box_c2, D.nbox, D.totals + T_NCH2);
   } else if (phase == 2) {
-    if (do2) block_exclusive_scan(D.cnt2s, D.off2s, D.totals[T_SL2], D.totals + T_NI2L);
+    if (do2) block_exclusive_scan<NT>(D.cnt2s, D.off2s, D.totals[T_SL2], D.totals + T_NI2L);
   }
 }
+template __global__ void k_scan<1024>(Dev, int, int, int);
+template __global__ void k_scan<256>(Dev, int, int, int);
 
 }  // namespace ccpol3b
