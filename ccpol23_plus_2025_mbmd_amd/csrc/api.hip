@@ -18,8 +18,8 @@ namespace ccpol3b {
 // kernels (lists.hip, eval.hip, fs2.hip)
 __global__ void k_prep(int, int, const double*, const double*, const double*, const double*,
                        const double*, const double*, Dev);
-__global__ void k_neighbours(Dev, Consts);
-template <int mode> __global__ void k_fs3_list(Dev, Consts);
+__global__ void k_neighbours(Dev, Consts, int);
+__global__ void k_fs3_fill(Dev, Consts);
 __global__ void k_fs2_items(Dev, Consts, int);
 template <int NT> __global__ void k_scan(Dev, int, int, int);
 __global__ void k_fs3_eval_f64(Dev, Consts, const double4*, const double*);
@@ -304,24 +304,23 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   EVREC(0);
   const long nprep = std::max<long>((long)nm * ACC_W, T_COUNT);
   hipLaunchKernelGGL(k_prep, dim3((nprep + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
-  hipLaunchKernelGGL(k_neighbours, dim3(waves_grid(nm)), dim3(256), 0, st, D, c->K);
-  hipLaunchKernelGGL(k_scan<1024>, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
-  LAUNCHCHK(c, "neighbour lists");
   const size_t lds = fs2_lds_bytes(D.cap_nb), lds_p = fs2_pair_lds_bytes(D.cap_nb);
   const size_t lds3 = fs3_list_lds_bytes(D.cap_nb);
   if (do2 && std::max(lds, lds_p) > 65536)
     return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the FS2 passes");
   if (do3 && lds3 > 65536)
     return fail(c, CCPOL3B_ECAPACITY, "FS3 neighbour count too large for the FS3 list build");
+  // neighbour lists with the per-molecule FS3 triplet counts, then every offset
+  hipLaunchKernelGGL(k_neighbours, dim3(nm), dim3(64), do3 ? lds3 : 0, st, D, c->K, do3);
+  hipLaunchKernelGGL(k_scan<1024>, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
+  LAUNCHCHK(c, "neighbour lists");
   EVREC(1);
-  // Two branches from here: the FS2 passes on `st`, and the FS3 list build
-  // (count, offsets with the per-box ranges, fill with the geometry) followed
-  // by the FS3 evaluation on the context's second stream.  Neither reads what
-  // the other writes (both add forces into acc with atomics); `st` waits for
-  // the FS3 branch before the epilogue.  Under capture the fork and join
-  // become graph edges.  (With the branches the other way round, the FS3
-  // kernel shares the chip with the FS2 passes for longer and a call of 512
-  // waters takes 3 % more.)
+  // Two branches from here: the FS2 passes on `st`, and the FS3 list fill
+  // (triplets in the reference order with their geometry) followed by the FS3
+  // evaluation on the context's second stream.  Neither reads what the other
+  // writes (both add forces into acc with atomics); `st` waits for the FS3
+  // branch before the epilogue.  Under capture the fork and join become graph
+  // edges.
 #define EVREC_SIDE(i) \
   if (!capture) HIPCHK(c, hipEventRecord(c->ev[i], c->side))
   HIPCHK(c, hipEventRecord(c->fork, st));
@@ -334,10 +333,8 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   }
   EVREC(2);
   hipStream_t s3 = c->side;
-  if (do3) hipLaunchKernelGGL(k_fs3_list<0>, dim3(nm), dim3(64), lds3, s3, D, c->K);
-  hipLaunchKernelGGL(k_scan<256>, dim3(1), dim3(256), 0, s3, D, 1, do3, do2);
-  if (do3) hipLaunchKernelGGL(k_fs3_list<1>, dim3(nm), dim3(64), lds3, s3, D, c->K);
-  LAUNCHCHK(c, "FS3 lists");
+  if (do3) hipLaunchKernelGGL(k_fs3_fill, dim3(nm), dim3(64), lds3, s3, D, c->K);
+  LAUNCHCHK(c, "FS3 list");
   const long cap3 = D.cap3;
   EVREC_SIDE(8);
   if (do3) {

@@ -6,8 +6,9 @@
 // zero, squares are summed left to right, and this file is compiled without
 // FMA contraction (the reference's x86-64 build has no FMA).
 //
-// Work decomposition: one wavefront per molecule for the neighbour sweep and
-// the FS3 triplets, one per directed neighbour pair for the exported FS2 items.
+// Work decomposition: one wavefront per molecule for the neighbour sweep (which
+// also counts the molecule's FS3 triplets) and the FS3 list fill, one per
+// directed neighbour pair for the exported FS2 items.
 // Lists come out in the reference's loop orders: FS3 (i,j,k) lexicographic
 // with i<j<k (:859-860), FS2 (k,i,j) with i<j (:297-301).
 #pragma clang fp contract(off)
@@ -58,80 +59,24 @@ __global__ void k_prep(int nm, int ld, const double* __restrict__ x, const doubl
   if (t < T_COUNT) D.totals[t] = 0;
 }
 
-// ------------------------------------------------------------------ neighbour lists
-// One wavefront per molecule i; every other molecule j of the same box is
-// tested with the reference's pair predicate (min image of x_i - x_j).  Both
-// cutoffs come out of one sweep, in ascending j.  Per-molecule counts of the
-// sub-list sweeps that follow are written too.
-__global__ __launch_bounds__(256) void k_neighbours(Dev D, Consts K) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= D.nm) return;
-  const int i = wave;
-  const int b = D.mol_box[i];
-  const double L = D.boxL[b];
-  const int j0 = D.box_start[b], j1 = D.box_start[b + 1];
-  const double xi = D.cx[i], yi = D.cy[i], zi = D.cz[i];
-  int n3 = 0, n2 = 0, u3 = 0, u2 = 0;
-  int* nb3 = D.nb3 + (long)i * D.cap_nb;
-  int* nb2 = D.nb2 + (long)i * D.cap_nb;
-  double* v2 = D.nbv2 + (long)i * 3 * D.cap_nb;
-  for (int base = j0; base < j1; base += 64) {
-    const int j = base + lane;
-    bool in3 = false, in2 = false;
-    double dx = 0.0, dy = 0.0, dz = 0.0;
-    if (j < j1 && j != i) {
-      dx = minimg(xi - D.cx[j], L);
-      dy = minimg(yi - D.cy[j], L);
-      dz = minimg(zi - D.cz[j], L);
-      const double d2 = sq3(dx, dy, dz);
-      in3 = d2 < K.rc2_3;
-      in2 = d2 < K.rc2_2;
-    }
-    const uint64_t m3 = __ballot(in3), m2 = __ballot(in2);
-    const uint64_t lo3 = __ballot(in3 && j < i), lo2 = __ballot(in2 && j < i);
-    if (in3) {
-      const int p = n3 + __popcll(m3 & lanemask_lt());
-      if (p < D.cap_nb) nb3[p] = j;
-    }
-    if (in2) {
-      const int p = n2 + __popcll(m2 & lanemask_lt());
-      if (p < D.cap_nb) {
-        nb2[p] = j;
-        v2[p] = dx;
-        v2[D.cap_nb + p] = dy;
-        v2[2 * D.cap_nb + p] = dz;
-      }
-    }
-    n3 += __popcll(m3);
-    n2 += __popcll(m2);
-    u3 += __popcll(lo3);
-    u2 += __popcll(lo2);
-  }
-  if (lane == 0) {
-    if (n3 > D.cap_nb || n2 > D.cap_nb) atomicMax(D.totals + T_NBOVER, n3 > n2 ? n3 : n2);
-    n3 = n3 < D.cap_nb ? n3 : D.cap_nb;
-    n2 = n2 < D.cap_nb ? n2 : D.cap_nb;
-    D.nb3_cnt[i] = n3;
-    D.nb2_cnt[i] = n2;
-    D.up3[i] = u3 < n3 ? u3 : n3;  // first upper position (entries < i come first)
-    D.up2[i] = u2 < n2 ? u2 : n2;
-    D.sl2_cnt[i] = n2;             // FS2 sub-lists (k; i) over all neighbours i
-    D.pcnt2[i] = n2 - D.up2[i];    // FS2 pairs i<j
-  }
-}
-
-// ------------------------------------------------------------------ FS3 triplets
-// One wavefront (one 64-thread block) per molecule i.  Its upper FS3
-// neighbours j > i (ascending, from the neighbour sweep) are staged in LDS with
-// their positions and integer images n_ij = nint((x_i - x_j)/L).  The wavefront
-// then takes the pairs (j, k), j < k, of those neighbours in the reference's
-// order (j, then k: :859-860), 64 at a time, and tests the derived vector
-// x_jk = dx_jk + L*n_ij - L*n_ik (:882-884) with the reference's arithmetic.
-// mode 0 counts per molecule into cnt3s[i]; mode 1 writes from off3s[i] in the
-// same order, with each triplet's geometry (TriGeo) and the SR / switch / LR
-// histogram.  Dynamic LDS: 6 doubles per neighbour slot.
+// ------------------------------------------------------------------ FS3 triplet sweep
+// The FS3 triplets of molecule i are the pairs (j, k), j < k, of its upper FS3
+// neighbours (j > i, ascending from the neighbour sweep), in the reference's
+// order (j, then k: :859-860), accepted when the derived vector
+// x_jk = dx_jk + L*n_ij - L*n_ik (:882-884) is inside the cutoff.  The upper
+// neighbours are staged in LDS, 6 doubles per slot: position and integer images
+// n_ij = nint((x_i - x_j)/L).
 size_t fs3_list_lds_bytes(int cap_nb) { return (size_t)cap_nb * 6 * sizeof(double); }
+
+__device__ __forceinline__ void stage_upper(double* st, int cap, int a, double xi, double yi, double zi, double xj,
+                                            double yj, double zj, double L) {
+  st[a] = xj;
+  st[cap + a] = yj;
+  st[2 * cap + a] = zj;
+  st[3 * cap + a] = L > 0.0 ? round((xi - xj) / L) : 0.0;
+  st[4 * cap + a] = L > 0.0 ? round((yi - yj) / L) : 0.0;
+  st[5 * cap + a] = L > 0.0 ? round((zi - zj) / L) : 0.0;
+}
 
 // pair (r, s), r < s < m, of flat index t in row-major order over r
 __device__ __forceinline__ void pair_of(long t, int m, int& r, int& s) {
@@ -144,38 +89,16 @@ __device__ __forceinline__ void pair_of(long t, int m, int& r, int& s) {
   s = q + 1 + (int)(t - ((long)q * m - (long)q * (q + 1) / 2));
 }
 
+// One wavefront sweeps the m staged neighbours of i, 64 pairs at a time.  mode 0
+// returns the number of triplets; mode 1 also writes them from `out` with their
+// geometry (TriGeo) and counts their regions in shist.
 template <int mode>
-__global__ __launch_bounds__(64) void k_fs3_list(Dev D, Consts K) {
-  extern __shared__ double lsm[];
-  __shared__ int shist[3];
-  const int lane = threadIdx.x;
-  const int i = blockIdx.x;
-  if (i >= D.nm) return;  // the whole block
-  if (lane < 3) shist[lane] = 0;
-  const int cap = D.cap_nb;
-  double* px = lsm;
-  double* py = px + cap;
-  double* pz = py + cap;
-  double* qx = pz + cap;  // n_ij (integer-valued)
-  double* qy = qx + cap;
-  double* qz = qy + cap;
-  const int n = D.nb3_cnt[i], up = D.up3[i], m = n - up;
-  const int* nb = D.nb3 + (long)i * cap + up;
-  const double L = D.boxL[D.mol_box[i]];
-  const double xi = D.cx[i], yi = D.cy[i], zi = D.cz[i];
-  for (int a = lane; a < m; a += 64) {
-    const int j = nb[a];
-    const double xj = D.cx[j], yj = D.cy[j], zj = D.cz[j];
-    px[a] = xj;
-    py[a] = yj;
-    pz[a] = zj;
-    qx[a] = L > 0.0 ? round((xi - xj) / L) : 0.0;
-    qy[a] = L > 0.0 ? round((yi - yj) / L) : 0.0;
-    qz[a] = L > 0.0 ? round((zi - zj) / L) : 0.0;
-  }
-  __syncthreads();
+__device__ int fs3_sweep(const Dev& D, const Consts& K, int i, const int* nb, int m, const double* st, int cap,
+                         double L, double xi, double yi, double zi, long out, int* shist) {
+  const int lane = threadIdx.x & 63;
+  const double *px = st, *py = st + cap, *pz = st + 2 * cap;
+  const double *qx = st + 3 * cap, *qy = st + 4 * cap, *qz = st + 5 * cap;
   const long npair = (long)m * (m - 1) / 2;
-  const long out = mode ? D.off3s[i] : 0;
   int count = 0;
   for (long t0 = 0; t0 < npair; t0 += 64) {
     const long t = t0 + lane;
@@ -210,13 +133,112 @@ __global__ __launch_bounds__(64) void k_fs3_list(Dev D, Consts K) {
     }
     count += __popcll(msk);
   }
-  if (mode == 0 && lane == 0) D.cnt3s[i] = count;
+  return count;
+}
+
+// ------------------------------------------------------------------ neighbour lists
+// One wavefront (one 64-thread block) per molecule i; every other molecule j of
+// the same box is tested with the reference's pair predicate (min image of
+// x_i - x_j).  Both cutoffs come out of one sweep, in ascending j.  With do3 the
+// upper FS3 neighbours are staged in LDS on the way and the wavefront then
+// counts i's FS3 triplets into cnt3s[i], so that the triplet offsets come out of
+// the scan that follows and the list can be filled in one more launch.
+__global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
+  extern __shared__ double lsm[];
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x;
+  if (i >= D.nm) return;  // the whole block
+  const int b = D.mol_box[i];
+  const double L = D.boxL[b];
+  const int j0 = D.box_start[b], j1 = D.box_start[b + 1];
+  const double xi = D.cx[i], yi = D.cy[i], zi = D.cz[i];
+  const int cap = D.cap_nb;
+  int n3 = 0, n2 = 0, u3 = 0, u2 = 0;
+  int* nb3 = D.nb3 + (long)i * cap;
+  int* nb2 = D.nb2 + (long)i * cap;
+  double* v2 = D.nbv2 + (long)i * 3 * cap;
+  for (int base = j0; base < j1; base += 64) {
+    const int j = base + lane;
+    bool in3 = false, in2 = false;
+    double xj = 0.0, yj = 0.0, zj = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+    if (j < j1 && j != i) {
+      xj = D.cx[j];
+      yj = D.cy[j];
+      zj = D.cz[j];
+      dx = minimg(xi - xj, L);
+      dy = minimg(yi - yj, L);
+      dz = minimg(zi - zj, L);
+      const double d2 = sq3(dx, dy, dz);
+      in3 = d2 < K.rc2_3;
+      in2 = d2 < K.rc2_2;
+    }
+    const uint64_t m3 = __ballot(in3), m2 = __ballot(in2);
+    const uint64_t lo3 = __ballot(in3 && j < i), lo2 = __ballot(in2 && j < i);
+    if (in3) {
+      const int p = n3 + __popcll(m3 & lanemask_lt());
+      if (p < cap) nb3[p] = j;
+      // upper entries (j > i) are staged at their position among the upper ones
+      const int pu = p - u3 - __popcll(lo3);
+      if (do3 && j > i && pu < cap) stage_upper(lsm, cap, pu, xi, yi, zi, xj, yj, zj, L);
+    }
+    if (in2) {
+      const int p = n2 + __popcll(m2 & lanemask_lt());
+      if (p < cap) {
+        nb2[p] = j;
+        v2[p] = dx;
+        v2[cap + p] = dy;
+        v2[2 * cap + p] = dz;
+      }
+    }
+    n3 += __popcll(m3);
+    n2 += __popcll(m2);
+    u3 += __popcll(lo3);
+    u2 += __popcll(lo2);
+  }
+  const int nn3 = n3 < cap ? n3 : cap, nn2 = n2 < cap ? n2 : cap;
+  const int up3 = u3 < nn3 ? u3 : nn3, up2 = u2 < nn2 ? u2 : nn2;
+  if (lane == 0) {
+    if (n3 > cap || n2 > cap) atomicMax(D.totals + T_NBOVER, n3 > n2 ? n3 : n2);
+    D.nb3_cnt[i] = nn3;
+    D.nb2_cnt[i] = nn2;
+    D.up3[i] = up3;  // first upper position (entries < i come first)
+    D.up2[i] = up2;
+    D.sl2_cnt[i] = nn2;        // FS2 sub-lists (k; i) over all neighbours i
+    D.pcnt2[i] = nn2 - up2;    // FS2 pairs i<j
+  }
+  if (do3) {
+    __syncthreads();  // the staged neighbours
+    const int count = fs3_sweep<0>(D, K, i, nullptr, nn3 - up3, lsm, cap, L, xi, yi, zi, 0, nullptr);
+    if (lane == 0) D.cnt3s[i] = count;
+  }
+}
+
+// ------------------------------------------------------------------ FS3 triplets
+// One wavefront (one 64-thread block) per molecule i: stages its upper FS3
+// neighbours again and writes its triplets from off3s[i] (fs3_sweep), with the
+// geometry of each triplet and the SR / switch / LR histogram.
+__global__ __launch_bounds__(64) void k_fs3_fill(Dev D, Consts K) {
+  extern __shared__ double lsm[];
+  __shared__ int shist[3];
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x;
+  if (i >= D.nm) return;  // the whole block
+  if (lane < 3) shist[lane] = 0;
+  const int cap = D.cap_nb;
+  const int n = D.nb3_cnt[i], up = D.up3[i], m = n - up;
+  const int* nb = D.nb3 + (long)i * cap + up;
+  const double L = D.boxL[D.mol_box[i]];
+  const double xi = D.cx[i], yi = D.cy[i], zi = D.cz[i];
+  for (int a = lane; a < m; a += 64) {
+    const int j = nb[a];
+    stage_upper(lsm, cap, a, xi, yi, zi, D.cx[j], D.cy[j], D.cz[j], L);
+  }
+  __syncthreads();
+  fs3_sweep<1>(D, K, i, nb, m, lsm, cap, L, xi, yi, zi, D.off3s[i], shist);
   // one global atomic per molecule and region
   __syncthreads();
-  if (mode && lane < 3 && shist[lane] > 0) atomicAdd(D.totals + T_HIST + lane, shist[lane]);
+  if (lane < 3 && shist[lane] > 0) atomicAdd(D.totals + T_HIST + lane, shist[lane]);
 }
-template __global__ void k_fs3_list<0>(Dev, Consts);
-template __global__ void k_fs3_list<1>(Dev, Consts);
 
 // ------------------------------------------------------------------ FS2 items
 // Ordered items (k; i<j): one wavefront per sub-list (k; i = nb2(k)[a]); the
@@ -345,13 +367,10 @@ __device__ void box_ranges(const Dev& D, int do3, int do2) {
   __syncthreads();
 }
 
-// phase 0: per-molecule counts -> FS2 sub-list and pair offsets
-// phase 1: per-molecule FS3 triplet counts -> triplet offsets; then the
-//          per-box triplet ranges and the first reduction chunk of each box
+// phase 0: per-molecule counts -> FS2 sub-list and pair offsets, FS3 triplet
+//          offsets; then the per-box triplet ranges and the first reduction
+//          chunk of each box
 // phase 2: per-sub-list candidate counts -> FS2 item offsets (list export only)
-// One block of NT threads: 1024 before the FS2 / FS3 fork and for the export,
-// 256 for phase 1, which runs beside the FS2 passes and would otherwise wait
-// for a CU with 16 free wavefront slots.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_scan(Dev D, int phase, int do3, int do2) {
   if (phase == 0) {
@@ -359,7 +378,6 @@ __global__ __launch_bounds__(NT) void k_scan(Dev D, int phase, int do3, int do2)
       block_exclusive_scan<NT>(D.sl2_cnt, D.sl2_off, D.nm, D.totals + T_SL2);
       block_exclusive_scan<NT>(D.pcnt2, D.poff2, D.nm, D.totals + T_NP2);
     }
-  } else if (phase == 1) {
     if (do3) block_exclusive_scan<NT>(D.cnt3s, D.off3s, D.nm, D.totals + T_N3);
     box_ranges(D, do3, do2);
     block_exclusive_scan<NT>(D.box_c3, D.box_c3, D.nbox, D.totals + T_NCH3);
@@ -369,6 +387,5 @@ __global__ __launch_bounds__(NT) void k_scan(Dev D, int phase, int do3, int do2)
   }
 }
 template __global__ void k_scan<1024>(Dev, int, int, int);
-template __global__ void k_scan<256>(Dev, int, int, int);
 
 }  // namespace ccpol3b
