@@ -158,7 +158,7 @@ typedef struct {
   int fs3_lds_coef;           /* 1: FS3 blocks stage the region's coefficient table in LDS;
                                  0: they read it through the vector cache (default) */
   int fs2_parts;              /* FS2 spectator-pass blocks per molecule, 1..8 (default 1) */
-  int fs2_pair_parts;         /* FS2 pair-pass blocks per molecule, 1..8 (default 4) */
+  int fs2_pair_parts;         /* FS2 pair-pass blocks per molecule, 1..8 (default 2) */
 } ccpol3b_tuning;
 
 int ccpol3b_get_tuning(const ccpol3b_ctx *ctx, ccpol3b_tuning *tuning);
