@@ -101,10 +101,14 @@ __device__ __forceinline__ double fs2_T(const Fs2Geo& G, const double* __restric
 
 // Accepted candidates queued per block before they are evaluated.
 constexpr int FS2_QCAP = 1024;
+// The spectator pass takes FS2_SU candidates per thread at a time (their
+// sub-list searches and loads in flight together) into a queue of FS2_SQCAP.
+constexpr int FS2_SU = 4;
+constexpr int FS2_SQCAP = 2048;
 
 // Dynamic LDS of the spectator pass: 6 doubles and 4 ints per neighbour slot, the queue.
 size_t fs2_lds_bytes(int cap_nb) {
-  return (size_t)cap_nb * (6 * sizeof(double) + 4 * sizeof(int)) + FS2_QCAP * sizeof(int) + 16;
+  return (size_t)cap_nb * (6 * sizeof(double) + 4 * sizeof(int)) + FS2_SQCAP * sizeof(int) + 16;
 }
 // Dynamic LDS of the pair pass: 6 doubles and 1 int per neighbour slot, 5 doubles
 // per pair slot, the queue.
@@ -284,9 +288,10 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
 // ---------------------------------------------------------------- spectator pass
 // D.fs2_parts blocks per molecule k, each taking an equal share of k's
 // candidates (k; i = nb2(k)[a], j = nb2(i)[q]) with q >= up2(i), flattened in
-// the reference's order (a, then q).  Accepted items are queued in that order
-// and every thread sums its queue entries in order, so the energy and virial
-// sums are the same run to run.
+// the reference's order (a, then q).  Each thread tests FS2_SU candidates at a
+// time, 256 apart, and appends the accepted ones in a fixed order (thread by
+// thread, each in its own candidate order); every thread then sums its queue
+// entries in order, so the energy and virial sums are the same run to run.
 __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   extern __shared__ double dsm[];
   __shared__ int wtot[4];
@@ -355,27 +360,62 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   for (int a = 0; a < NS2 * 3 + 3; ++a) Fk[a] = 0.0;
   double E = 0.0, W = 0.0;
   int nitems = 0, qn = 0;
-  for (int base = f0; base < f1; base += 256) {
-    const int f = base + tid;
-    bool ok = false;
-    int val = 0;
-    if (f < f1) {
-      int lo = 0, hi = n;  // off[lo] <= f < off[hi]
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (off[mid] <= f) lo = mid; else hi = mid;
-      }
-      const int a = lo, i = nbl[a];
-      const int qj = u0s[a] + (f - off[a]);
-      const long e = (long)i * cap + qj;
-      if (D.nb2[e] != k) {
-        const double* vi = D.nbv2 + (long)i * 3 * cap + qj;  // x_ij = x_i - x_j
-        ok = sq3_nc(sx[a] + vi[0], sy[a] + vi[cap], sz[a] + vi[2 * cap]) < K.rc2_2;
-        val = (a << 16) | qj;
+  // sub-list search steps: powers of two from the largest below n
+  int top = 0;
+  if (n > 1) {
+    top = 1;
+    while (2 * top <= n - 1) top *= 2;
+  }
+  for (int base = f0; base < f1; base += 256 * FS2_SU) {
+    // candidate f = base + 256 u + tid lies in sub-list lo[u]: the last a < n
+    // with off[a] <= f (off is non-decreasing and off[0] = 0)
+    int lo[FS2_SU];
+#pragma unroll
+    for (int u = 0; u < FS2_SU; ++u) lo[u] = 0;
+    for (int sstep = top; sstep > 0; sstep >>= 1) {
+#pragma unroll
+      for (int u = 0; u < FS2_SU; ++u) {
+        const int t = lo[u] + sstep;
+        if (t < n && off[t] <= base + 256 * u + tid) lo[u] = t;
       }
     }
-    qn += block_append(ok, val, queue, qn, wtot);
-    if (qn <= FS2_QCAP - 256 && base + 256 < f1) continue;  // block-uniform
+    bool ok[FS2_SU];
+    int val[FS2_SU];
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < FS2_SU; ++u) {
+      const int f = base + 256 * u + tid;
+      ok[u] = false;
+      val[u] = 0;
+      if (f < f1) {
+        const int a = lo[u], i = nbl[a];
+        const int qj = u0s[a] + (f - off[a]);
+        const int jn = D.nb2[(long)i * cap + qj];
+        const double* vi = D.nbv2 + (long)i * 3 * cap + qj;  // x_ij = x_i - x_j
+        const bool in = sq3_nc(sx[a] + vi[0], sy[a] + vi[cap], sz[a] + vi[2 * cap]) < K.rc2_2;
+        ok[u] = in & (jn != k);
+        val[u] = (a << 16) | qj;
+      }
+      cnt += ok[u] ? 1 : 0;
+    }
+    // block exclusive scan of the per-thread counts; each thread appends its
+    // accepted candidates in u order
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += v;
+    }
+    if (lane == 63) wtot[w] = incl;
+    __syncthreads();
+    int p = qn + incl - cnt;
+    for (int u = 0; u < w; ++u) p += wtot[u];
+#pragma unroll
+    for (int u = 0; u < FS2_SU; ++u)
+      if (ok[u]) queue[p++] = val[u];
+    qn += (wtot[0] + wtot[1]) + (wtot[2] + wtot[3]);
+    __syncthreads();
+    if (qn <= FS2_SQCAP - 256 * FS2_SU && base + 256 * FS2_SU < f1) continue;  // block-uniform
     for (int e = tid; e < qn; e += 256) {
       const int a = queue[e] >> 16, qj = queue[e] & 0xffff;
       const int i = nbl[a];

@@ -289,55 +289,85 @@ __global__ __launch_bounds__(256) void k_fs2_items(Dev D, Consts K, int mode) {
 // Exclusive scan of n ints by one block of NT threads, tile by tile through LDS
 // (coalesced loads, 8 consecutive elements per thread); off[n] and *total get
 // the sum.
-constexpr int SCAN_PER = 8;
+// Exclusive scans of K int arrays of length n together, by one block of NT
+// threads: off[k][q] = sum of cnt[k][0..q-1], and off[k][n] and *total[k]
+// receive the sums.  off may alias cnt (each tile is read before it is
+// written).  Tiles are staged in buf (SCAN_BUF ints shared by the K arrays).
+constexpr int SCAN_BUF = 8192;
 
-template <int NT>
-__device__ void block_exclusive_scan(const int* cnt, int* off, long n, int* total) {
-  constexpr int TILE = NT * SCAN_PER;
-  __shared__ int tile[TILE];
-  __shared__ long wsum[NT / 64];
-  __shared__ long carry_s;
+template <int NT, int K>
+__device__ void block_exclusive_scan(int* buf, const int* const (&cnt)[K], int* const (&off)[K], long n,
+                                     int* const (&total)[K]) {
+  constexpr int PER = SCAN_BUF / (K * NT);  // entries per thread and array
+  constexpr int TILE = NT * PER;
+  static_assert(PER >= 1, "scan buffer too small");
+  __shared__ long wsum[K][NT / 64];
+  __shared__ long carry_s[K];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (t == 0) carry_s = 0;
+  if (t < K) carry_s[t] = 0;
   __syncthreads();
   for (long base = 0; base < n; base += TILE) {
-    for (int r = 0; r < SCAN_PER; ++r) {
-      const long q = base + r * NT + t;
-      tile[r * NT + t] = q < n ? cnt[q] : 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        const long q = base + r * NT + t;
+        buf[k * TILE + r * NT + t] = q < n ? cnt[k][q] : 0;
+      }
+    __syncthreads();
+    long loc[K][PER], s[K], incl[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      s[k] = 0;
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        loc[k][r] = s[k];
+        s[k] += buf[k * TILE + t * PER + r];
+      }
+      incl[k] = s[k];
+    }
+    // inclusive scans of the per-thread sums: wavefront shuffles, then wavefront totals
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const long u = __shfl_up(incl[k], d, 64);
+        if (lane >= d) incl[k] += u;
+      }
+    if (lane == 63)
+#pragma unroll
+      for (int k = 0; k < K; ++k) wsum[k][w] = incl[k];
+    __syncthreads();
+    long excl[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      long wbase = 0;
+      for (int u = 0; u < w; ++u) wbase += wsum[k][u];
+      excl[k] = carry_s[k] + wbase + incl[k] - s[k];
     }
     __syncthreads();
-    long loc[SCAN_PER];
-    long s = 0;
-    for (int r = 0; r < SCAN_PER; ++r) {
-      loc[r] = s;
-      s += tile[t * SCAN_PER + r];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+      for (int r = 0; r < PER; ++r) buf[k * TILE + t * PER + r] = (int)(excl[k] + loc[k][r]);
+      if (t == NT - 1) carry_s[k] = excl[k] + s[k];
     }
-    // inclusive scan of the per-thread sums: wavefront shuffles, then wavefront totals
-    long incl = s;
-    for (int d = 1; d < 64; d <<= 1) {
-      const long u = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += u;
-    }
-    if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    long wbase = 0;
-    for (int u = 0; u < w; ++u) wbase += wsum[u];
-    const long carry = carry_s;
-    const long excl = carry + wbase + incl - s;
-    __syncthreads();
-    for (int r = 0; r < SCAN_PER; ++r) tile[t * SCAN_PER + r] = (int)(excl + loc[r]);
-    if (t == NT - 1) carry_s = excl + s;
-    __syncthreads();
-    for (int r = 0; r < SCAN_PER; ++r) {
-      const long q = base + r * NT + t;
-      if (q < n) off[q] = tile[r * NT + t];
-    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        const long q = base + r * NT + t;
+        if (q < n) off[k][q] = buf[k * TILE + r * NT + t];
+      }
     __syncthreads();
   }
-  if (t == 0) {
-    off[n] = (int)carry_s;
-    *total = (int)carry_s;
-  }
+  if (t == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      off[k][n] = (int)carry_s[k];
+      *total[k] = (int)carry_s[k];
+    }
   __syncthreads();
 }
 
@@ -373,17 +403,38 @@ __device__ void box_ranges(const Dev& D, int do3, int do2) {
 // phase 2: per-sub-list candidate counts -> FS2 item offsets (list export only)
 template <int NT>
 __global__ __launch_bounds__(NT) void k_scan(Dev D, int phase, int do3, int do2) {
+  __shared__ int buf[SCAN_BUF];
+  int* const T = D.totals;
   if (phase == 0) {
-    if (do2) {
-      block_exclusive_scan<NT>(D.sl2_cnt, D.sl2_off, D.nm, D.totals + T_SL2);
-      block_exclusive_scan<NT>(D.pcnt2, D.poff2, D.nm, D.totals + T_NP2);
+    // the per-molecule scans of both terms together
+    if (do2 && do3) {
+      const int* const c[3] = {D.sl2_cnt, D.pcnt2, D.cnt3s};
+      int* const o[3] = {D.sl2_off, D.poff2, D.off3s};
+      int* const tt[3] = {T + T_SL2, T + T_NP2, T + T_N3};
+      block_exclusive_scan<NT, 3>(buf, c, o, D.nm, tt);
+    } else if (do2) {
+      const int* const c[2] = {D.sl2_cnt, D.pcnt2};
+      int* const o[2] = {D.sl2_off, D.poff2};
+      int* const tt[2] = {T + T_SL2, T + T_NP2};
+      block_exclusive_scan<NT, 2>(buf, c, o, D.nm, tt);
+    } else if (do3) {
+      const int* const c[1] = {D.cnt3s};
+      int* const o[1] = {D.off3s};
+      int* const tt[1] = {T + T_N3};
+      block_exclusive_scan<NT, 1>(buf, c, o, D.nm, tt);
     }
-    if (do3) block_exclusive_scan<NT>(D.cnt3s, D.off3s, D.nm, D.totals + T_N3);
     box_ranges(D, do3, do2);
-    block_exclusive_scan<NT>(D.box_c3, D.box_c3, D.nbox, D.totals + T_NCH3);
-    block_exclusive_scan<NT>(D.box_c2, D.box_c2, D.nbox, D.totals + T_NCH2);
+    const int* const c[2] = {D.box_c3, D.box_c2};
+    int* const o[2] = {D.box_c3, D.box_c2};
+    int* const tt[2] = {T + T_NCH3, T + T_NCH2};
+    block_exclusive_scan<NT, 2>(buf, c, o, D.nbox, tt);
   } else if (phase == 2) {
-    if (do2) block_exclusive_scan<NT>(D.cnt2s, D.off2s, D.totals[T_SL2], D.totals + T_NI2L);
+    if (do2) {
+      const int* const c[1] = {D.cnt2s};
+      int* const o[1] = {D.off2s};
+      int* const tt[1] = {T + T_NI2L};
+      block_exclusive_scan<NT, 1>(buf, c, o, D.totals[T_SL2], tt);
+    }
   }
 }
 template __global__ void k_scan<1024>(Dev, int, int, int);
