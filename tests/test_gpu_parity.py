@@ -237,6 +237,52 @@ def test_batch_matches_single(params_fs2fs3):
         assert torch.equal(en[bi], en1)
 
 
+@pytest.mark.parametrize("terms", ["fs2fs3", "fs2", "fs3"])
+def test_large_batch_matches_single(params_fs2fs3, terms):
+    """6,100 molecules in six boxes: the offset scans of the list build then
+    take several tiles (2048 molecules per tile with both terms, 4096 with one
+    of FS2, 8192 with FS3 alone), against one box per call, where each takes
+    one tile."""
+    import torch
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    p = params_fs2fs3
+    if terms == "fs2":
+        p = p.replace(s3_term=False)
+    elif terms == "fs3":
+        p = p.replace(s2_term=False)
+    tb = ThreeBody(p, "fp64")
+    boxes = [lattice_box(n, seed=s) for n, s in ((700, 41), (1000, 42), (900, 43), (1200, 44), (800, 45),
+                                                 (1500, 46))]
+    print("large batch", terms, sum(b.nm for b in boxes), "molecules", flush=True)
+    dev = torch.device("cuda:0")
+
+    def pack(bs):
+        com = torch.tensor(np.concatenate([b.com for b in bs]).T.copy(), dtype=torch.float64, device=dev)
+        site = torch.tensor(np.concatenate([np.stack([b.xs.reshape(b.nm, b.ld), b.ys.reshape(b.nm, b.ld),
+                                                      b.zs.reshape(b.nm, b.ld)]) for b in bs], axis=1).copy(),
+                            dtype=torch.float64, device=dev)
+        return com.contiguous(), site.contiguous()
+
+    com, site = pack(boxes)
+    starts = np.cumsum([0] + [b.nm for b in boxes])
+    fs, c2, c3, en = tb.eval_batch_device(starts, [b.box for b in boxes], com, site)
+    torch.cuda.synchronize()
+    st = tb.stats()
+    n3 = n2 = 0
+    for bi, b in enumerate(boxes):
+        cb, sb = pack([b])
+        fs1, c21, c31, en1 = tb.eval_device(cb, sb, b.box)
+        torch.cuda.synchronize()
+        n3 += tb.stats()["n_fs3"]
+        n2 += tb.stats()["n_fs2_items"]
+        sl = slice(starts[bi], starts[bi + 1])
+        for a, b_ in ((fs[:, sl], fs1), (c2[:, sl], c21), (c3[:, sl], c31)):
+            assert torch.allclose(a, b_, rtol=0, atol=1e-12 * float(b_.abs().max()))
+        assert torch.equal(en[bi], en1)
+    assert (st["n_fs3"], st["n_fs2_items"]) == (n3, n2)
+    assert (n3 > 0) == (terms != "fs2") and (n2 > 0) == (terms != "fs3")
+
+
 def test_empty_and_tiny_inputs(tb_fs2fs3, oracle, params_fs2fs3):
     """nm = 0, 1, 2 and three far-apart molecules: no triplets or items, zero
     energies, accumulated outputs untouched, vir unchanged."""
