@@ -151,6 +151,10 @@ struct ccpol3b_ctx {
   size_t cap_in = 0, cap_out = 0;
   std::vector<int> h_molbox, h_boxstart;
   std::vector<double> h_boxL;
+  // the box description as last copied to the device (empty: none valid); a
+  // call whose description is the same copies nothing
+  std::vector<int> up_molbox, up_boxstart;
+  std::vector<double> up_boxL;
   hipEvent_t ev[9] = {};
   // the FS3 list build runs on a second stream beside the FS2 passes
   hipStream_t side = nullptr;
@@ -189,6 +193,12 @@ int dev_alloc(ccpol3b_ctx* c, P*& p, size_t n) {
   return CCPOL3B_OK;
 }
 
+void forget_upload(ccpol3b_ctx* c) {
+  c->up_molbox.clear();
+  c->up_boxstart.clear();
+  c->up_boxL.clear();
+}
+
 void free_dev(ccpol3b_ctx* c) {
   Dev& D = c->D;
   dev_free(D.cx); dev_free(D.cy); dev_free(D.cz); dev_free(D.site);
@@ -202,6 +212,7 @@ void free_dev(ccpol3b_ctx* c) {
   dev_free(D.box_t3); dev_free(D.box_t2); dev_free(D.totals);
   dev_free(D.box_c3); dev_free(D.box_c2); dev_free(D.red3); dev_free(D.red2); dev_free(D.s2pair);
   D.cap_red = 0;
+  forget_upload(c);
 }
 
 // Per-molecule arrays for nm molecules and nbox boxes.  Sub-list and pair
@@ -210,6 +221,7 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
   Dev& D = c->D;
   int rc;
   if (nm > c->cap_nm) {
+    forget_upload(c);
     const int n = std::max(nm, 16);
     const size_t nn = (size_t)n * D.cap_nb;
     if ((rc = dev_alloc(c, D.cx, n)) || (rc = dev_alloc(c, D.cy, n)) || (rc = dev_alloc(c, D.cz, n)) ||
@@ -228,6 +240,7 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
     c->cap_nm = n;
   }
   if (nbox > c->cap_box) {
+    forget_upload(c);
     if ((rc = dev_alloc(c, D.boxL, nbox)) || (rc = dev_alloc(c, D.box_start, nbox + 1)) ||
         (rc = dev_alloc(c, D.box_t3, nbox + 1)) || (rc = dev_alloc(c, D.box_t2, nbox + 1)) ||
         (rc = dev_alloc(c, D.box_c3, nbox + 1)) || (rc = dev_alloc(c, D.box_c2, nbox + 1)))
@@ -295,11 +308,15 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   }
   D.fs2_parts = c->tune.fs2_parts;
   D.fs2_pair_parts = c->tune.fs2_pair_parts;
-  if (!capture) {
+  if (!capture && (c->h_molbox != c->up_molbox || c->h_boxL != c->up_boxL || c->h_boxstart != c->up_boxstart)) {
+    forget_upload(c);
     HIPCHK(c, hipMemcpyAsync(D.mol_box, c->h_molbox.data(), sizeof(int) * nm, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(D.boxL, c->h_boxL.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(D.box_start, c->h_boxstart.data(), sizeof(int) * (nbox + 1),
                              hipMemcpyHostToDevice, st));
+    c->up_molbox = c->h_molbox;
+    c->up_boxL = c->h_boxL;
+    c->up_boxstart = c->h_boxstart;
   }
   EVREC(0);
   const long nprep = std::max<long>((long)nm * ACC_W, T_COUNT);

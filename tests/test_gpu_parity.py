@@ -168,6 +168,27 @@ def test_repeatable(tb_fs2fs3):
         np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-13 * np.abs(a[k]).max())
 
 
+def test_box_description_changes_between_calls(params_fs2fs3):
+    """One context, calls whose box description (box length, molecule count,
+    open boundary) changes or repeats: each result equals a fresh context's.
+    The context copies the box description to the device only when it differs
+    from the last copy."""
+    from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    a = lattice_box(64, seed=21)
+    b = WaterBox(a.com * 1.02, a.quats, a.box * 1.02)   # same molecules, longer box
+    c = lattice_box(100, seed=22)
+    d = WaterBox(a.com, a.quats, 0.0)                   # open boundary
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    for box in (a, b, a, c, d, a, a):
+        got = tb.evaluate(box)
+        ref = ThreeBody(params_fs2fs3, "fp64").evaluate(box)
+        for k in ("qfs2", "qfs3", "vir", "n_fs3", "n_fs2_items"):
+            assert got[k] == ref[k], k
+        for k in ("fxs", "fys", "fzs", "fxfs2", "fxfs3"):
+            np.testing.assert_allclose(got[k], ref[k], rtol=0, atol=1e-13 * np.abs(ref[k]).max())
+
+
 def test_mixed_precision_error_512(params_fs2fs3, oracle):
     """Mixed (FP32 exponential/polynomial work in the shifted pair-vector basis)
     vs the FP64 path at 512 waters: site forces, COM forces, and what an
