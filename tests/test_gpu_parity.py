@@ -88,6 +88,27 @@ def test_lists_bit_exact(tb_fs2fs3, oracle, params_fs2fs3, n, seed):
     np.testing.assert_array_equal(l2, oracle.fs2_list(params_fs2fs3, box.x, box.y, box.z, box.box))
 
 
+def test_fs2_dense_box(oracle, params_fs2fs3):
+    """FS2 alone on 216 waters compressed to 1.3x water density (~77 neighbours
+    within 7.5 A, ~3,000 candidates per spectator): the spectator pass then
+    tests its candidates in several rounds and evaluates its queue more than
+    once per molecule.  FS2 list bit-exact; energy, forces and virial against
+    the oracle."""
+    from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    p = params_fs2fs3.replace(s3_term=False)
+    base = lattice_box(216, seed=51)
+    s = (1 / 1.3) ** (1 / 3)
+    box = WaterBox(base.com * s, base.quats, base.box * s)
+    print("fs2 dense box 216 waters", flush=True)
+    tb = ThreeBody(p, "fp64")
+    got = tb.evaluate(box)
+    ref = oracle.threebody_term(p, box, nthreads=NTHREADS)
+    assert got["n_fs2_items"] > 216 * 1200
+    assert_parity(got, ref, s3=False)
+    np.testing.assert_array_equal(tb.fs2_list(), oracle.fs2_list(p, box.x, box.y, box.z, box.box))
+
+
 def test_fs3_variant_64_box(tb_fs3, oracle, params_fs3):
     """FS3-only variant (switches T T F) on the 64-water box, L = 12.42 A."""
     box = lattice_box(64, seed=1)
