@@ -31,7 +31,11 @@ __constant__ double c_q2[NS2] = {0.258504607653328389, 0.564050362569283648, 0.5
                                  -0.693302666395947953, -0.693302666395947953};
 constexpr double ETA3 = 0.48056977605478400;
 constexpr double BETA3 = 0.85836606579445000;
-constexpr double G3 = 0.400000000000000;
+// The exchange-charge shift g of :177 is declared `real(8), parameter :: g3 =
+// 0.400000000000000` without a kind suffix, so the literal is default REAL(4)
+// (source/Makefile builds with `mpifort -O3`, no -fdefault-real-8): g3 holds
+// the single-precision value of 0.4 widened to double, 0.4000000059604645.
+constexpr double G3 = (double)0.4f;
 
 struct Fs2Geo {
   double xij[3], xki[3], xkj[3];

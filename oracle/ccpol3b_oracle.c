@@ -35,7 +35,10 @@ static const int ntype3[ORC_NS3] = {1, 2, 2, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 6, 6,
 /* FS2 constants, threebody_potential.f90:175-177, 197-201 */
 static const double eta3 = 0.48056977605478400;
 static const double beta3 = 0.85836606579445000;
-static const double g3 = 0.400000000000000;
+/* g3 of :177 is written without a kind suffix (`0.400000000000000`), so it is
+ * a default REAL(4) literal widened to real(8): 0.4f = 0.4000000059604645
+ * (source/Makefile: mpifort -O3, no -fdefault-real-8). */
+static const double g3 = (double)0.4f;
 static const double charge3b[5] = {0.258504607653328389, 0.564050362569283648,
                                    0.564050362569283648, -0.693302666395947953,
                                    -0.693302666395947953};

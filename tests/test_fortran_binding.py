@@ -132,3 +132,79 @@ def test_fortran_md_loop_matches_python(tmp_path):
     np.testing.assert_allclose(rec_f, rec, rtol=1e-10, atol=1e-10)
     for got, key in ((com_f, "com"), (quat_f, "quat"), (vel_f, "vel"), (jb_f, "jbody")):
         np.testing.assert_allclose(got, st[key], rtol=1e-9, atol=1e-12 * np.abs(st[key]).max(), err_msg=key)
+
+
+# ------------------------------------------------ the INTEGRATION.md snippets at the reference's shapes
+REF_DRIVER = HERE / "fortran" / "_build" / "drive_ref"
+REF_SOURCE = HERE / "fortran" / "drive_ref.f90"
+INTEGRATION = HERE.parent / "INTEGRATION.md"
+
+
+def integration_snippets():
+    """The first two ```fortran blocks of INTEGRATION.md (context creation and
+    the new body of threebody_term)."""
+    text = INTEGRATION.read_text()
+    blocks, pos = [], 0
+    while len(blocks) < 2:
+        a = text.index("```fortran\n", pos) + len("```fortran\n")
+        b = text.index("```", a)
+        blocks.append([ln.strip() for ln in text[a:b].splitlines() if ln.strip()])
+        pos = b + 3
+    return blocks
+
+
+def test_integration_snippets_are_the_tested_program():
+    """Every line of the two snippets a maintainer copies from INTEGRATION.md
+    appears, in the same order, in tests/fortran/drive_ref.f90 (which the GPU
+    test below runs against the golden boxes); and the cutoff passed is rcut3
+    (A), not the squared rcutsd3."""
+    src = [ln.strip() for ln in REF_SOURCE.read_text().splitlines() if ln.strip()]
+    for block in integration_snippets():
+        at = 0
+        for line in block:
+            assert line in src[at:], f"INTEGRATION.md line not in drive_ref.f90 (in order): {line!r}"
+            at = src.index(line, at) + 1
+    flat = " ".join(" ".join(b) for b in integration_snippets())
+    assert "tb_par%rcut3 = rcut3" in flat and "rcutsd3" not in flat
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["box64_fs2fs3", "box64_fs3"])
+def test_reference_shapes_drop_in_matches_golden(tmp_path, name):
+    """The reference's globals at their declared shapes (nsite = 50, nom = 5000,
+    bS3_s(50,50) read with ld_pair = 50), the INTEGRATION.md snippets, one call
+    of threebody_term: the golden 64-water box at the FP64 parity bar."""
+    assert REF_DRIVER.exists(), "tests/fortran/_build/drive_ref is built by __graft_entry__.build()"
+    with np.load(HERE / "golden" / f"{name}.npz", allow_pickle=False) as f:
+        g = {k: f[k] for k in f.files}
+    variant = "fs3" if name.endswith("_fs3") else "fs2fs3"
+    p = load_variant(variant)
+    s2 = 0 if variant == "fs3" else int(p.s2_term)
+    nm = len(g["com"])
+    d = DATA_DIR / variant
+    fmt = lambda v: f"{float(v):.17g}"
+    inp, out = tmp_path / "in.txt", tmp_path / "out.txt"
+    with open(inp, "w") as fh:
+        fh.write(f"{nm} {fmt(g['box'])}\n")
+        fh.write(f"{fmt(p.alpha3)} {fmt(p.delta33)} {fmt(p.rcut3)} {s2} {int(p.s3_term)}\n")
+        fh.write(f"{d / 'data3b_full_SR_5p5'}\n{d / 'data3b_full_LR_4p5'}\n")
+        for row in g["com"]:
+            fh.write(" ".join(fmt(v) for v in row) + "\n")
+        for m in range(nm):
+            for s in range(17):
+                fh.write(" ".join(fmt(v) for v in g["site"][m, s]) + "\n")
+    print(f"fortran reference-shape drop-in: {name}", flush=True)
+    r = subprocess.run([str(REF_DRIVER), str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "ok" in r.stdout, (r.stdout, r.stderr)
+    vals = np.loadtxt(out, max_rows=1)
+    fsite = np.loadtxt(out, skiprows=1, max_rows=nm * 17).reshape(nm, 17, 3)
+    com = np.loadtxt(out, skiprows=1 + nm * 17)
+    for got, key in ((vals[0], "qfs2"), (vals[1], "qfs3")):
+        ref = float(g[key])
+        assert abs(got - ref) <= 1e-10 * abs(ref) + 2e-12, (key, got, ref)
+    assert abs(vals[2] - float(g["vir"])) <= 1e-9 * abs(float(g["vir"])) + 1e-9
+    rel = lambda a, b: np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
+    assert rel(fsite, g["fsite"]) <= 1e-9
+    if s2:
+        assert rel(com[:, :3], g["fcom2"]) <= 1e-7
+    assert rel(com[:, 3:], g["fcom3"]) <= 1e-7
