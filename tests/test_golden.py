@@ -115,9 +115,12 @@ def golden_tb():
 
 
 TOLS = {"fp64": dict(e_tol=1e-10, f_tol=1e-9, com_tol=1e-7),
-        # mixed: the bar is on the force output; energies and the COM forces
-        # derived from them cancel by ~1e5 and carry ~1e-3 in FP32
-        "mixed": dict(e_tol=3e-3, f_tol=1e-5, com_tol=1e-2)}
+        # mixed: 1e-5 holds for the site forces and the full force output; the
+        # energies and the COM forces derived from them (f_d' E + f_d w' E) do
+        # not meet it -- the short-range triplet energies cancel ~1e4-fold, so
+        # FP32 rounding costs ~1e-4 of them (tests/test_mixed_limits.py).  The
+        # energy and COM tolerances are the measured envelope on these boxes.
+        "mixed": dict(e_tol=2e-3, f_tol=1e-5, com_tol=5e-4)}
 
 
 @pytest.mark.gpu

@@ -235,19 +235,26 @@ def test_mixed_precision_error_512(params_fs2fs3, oracle):
 
     err_m = rel_rms(molecular(got), molecular(ref))
     err_t = rel_rms(torque(got), torque(ref))
+    vir_err = abs(got["vir"] - ref["vir"]) / abs(ref["vir"])
     print(f"mixed vs fp64: site force rms {err_f:.3e}, all forces {err_all:.3e}, FS3 COM {err_c:.3e}, "
-          f"molecular force {err_m:.3e}, torque {err_t:.3e}, FS3 energy {err_e:.3e}")
-    # the bar of the mixed path is on the force output
+          f"molecular force {err_m:.3e}, torque {err_t:.3e}, FS3 energy {err_e:.3e}, virial {vir_err:.3e}")
+    # The 1e-5 force bar holds for the site forces (and the full output vector,
+    # which they dominate).
     assert err_f <= 1e-5
     assert err_all <= 1e-5
-    # The FS3 energy cancels by ~2e5 per triplet even in the shifted basis
-    # (sum of |terms| over |E|), so the FP32 energy and the COM forces derived
-    # from it (f_d' E + f_d w' E) carry ~1e-3; the site-force errors of a
-    # molecule add up against a molecular force ~15x smaller than its site forces.
-    assert err_e <= 3e-3
-    assert err_c <= 1e-2
-    assert err_m <= 1e-3
-    assert err_t <= 5e-4
+    # It does NOT hold for the FS3 COM forces, the molecular forces and the
+    # energy: their envelopes below are measured (50 seeded 512-water boxes: COM
+    # 8e-5, molecular 1.6e-4, torque 7e-5, energy 7e-4), not a bar.  The
+    # short-range triplet energies cancel ~1e4-fold in the shifted basis, and
+    # FP32 rounding of the coefficients alone costs 6e-4 of them
+    # (tests/test_mixed_limits.py), so the benchmark's headline is the FP64 path.
+    assert err_e <= 2e-3
+    assert err_c <= 3e-4
+    assert err_m <= 4e-4
+    assert err_t <= 2e-4
+    # the virial sums r dE/dr over site pairs and COM vectors; in a periodic
+    # 512-water box its FP32 error stays at the energy's level
+    assert vir_err <= 5e-3
 
 
 def test_batch_matches_single(params_fs2fs3):
