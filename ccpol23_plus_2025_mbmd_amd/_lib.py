@@ -85,6 +85,8 @@ def load():
     L.ccpol3b_threebody_term.restype = C.c_int
     L.ccpol3b_eval_device.argtypes = [vp, C.c_int, C.c_int] + [vp] * 6 + [C.c_double] + [vp] * 6 + [vp]
     L.ccpol3b_eval_device.restype = C.c_int
+    L.ccpol3b_eval_device_noalloc.argtypes = [vp, C.c_int, C.c_int] + [vp] * 6 + [C.c_double] + [vp] * 6 + [vp]
+    L.ccpol3b_eval_device_noalloc.restype = C.c_int
     L.ccpol3b_eval_batch_device.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp, C.c_int] + [vp] * 12 + [vp]
     L.ccpol3b_eval_batch_device.restype = C.c_int
     L.ccpol3b_eval_batch_grad_device.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp, C.c_int] + [vp] * 13 + [vp]
@@ -143,6 +145,7 @@ EXPORTED = [
     "ccpol3b_get_fs2_list", "ccpol3b_read_data3b", "ccpol3b_orbit_index", "ccpol3b_status_string",
     "ccpol3b_last_error", "ccpol3b_eval_batch_grad_device", "ccpol3b_set_coefficients", "ccpol3b_gram_device",
     "ccpol3b_get_tuning", "ccpol3b_set_tuning", "ccpol3b_autotune", "ccpol3b_graph_create", "ccpol3b_graph_launch",
+    "ccpol3b_eval_device_noalloc",
     "ccpol3b_graph_status", "ccpol3b_graph_destroy", "ccpol3b_md_create", "ccpol3b_md_set_state",
     "ccpol3b_md_get_state", "ccpol3b_md_run", "ccpol3b_md_destroy",
 ]

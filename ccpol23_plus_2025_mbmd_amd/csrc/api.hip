@@ -8,7 +8,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ccpol3b.h"
@@ -159,6 +161,10 @@ struct ccpol3b_ctx {
   // the FS3 list build runs on a second stream beside the FS2 passes
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  // the arrays of the last allocation-free call (ccpol3b_eval_device_noalloc),
+  // when that was the last call
+  Dev nD{};
+  bool last_noalloc = false;
 };
 
 namespace {
@@ -287,28 +293,34 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
 // outgrows its capacity is flagged on the device; the kernels that write the
 // caller's outputs then write nothing, and run_checked() grows the capacity and
 // evaluates again.  Ends with an asynchronous copy of the counters to the host.
+// `fixed`: D is a caller-provided set of device arrays with its capacities and
+// box description already in place (the allocation-free entry point); nothing
+// is allocated or copied from the host, and c->D is not touched.
 int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double* y, const double* z,
         const double* xs, const double* ys, const double* zs, double* fxs, double* fys, double* fzs,
         double* fcom2, double* fcom3, double* energies, hipStream_t st, double* phi = nullptr,
-        bool capture = false) {
-  Dev& D = c->D;
+        bool capture = false, Dev* fixed = nullptr) {
+  Dev& D = fixed ? *fixed : c->D;
   int rc;
   const int do3 = c->s3, do2 = c->s2;
   // during graph capture nothing may allocate, copy from pageable memory or record timing events
 #define EVREC(i) \
   if (!capture) HIPCHK(c, hipEventRecord(c->ev[i], st))
-  if ((rc = ensure_molecules(c, nm, nbox))) return rc;
-  D.nm = nm;
-  D.nbox = nbox;  // before ensure_lists: the reduction arrays are sized by it
-  // triplet capacity: the last call's, or a bulk-water estimate
-  if ((rc = ensure_lists(c, std::max<long>(D.cap3, (long)nm * 8), 0))) return rc;
-  if (phi && D.cap3 > c->cap_phi) {
-    if ((rc = dev_alloc(c, c->d_phitrip, (size_t)D.cap3 * 2 * NLIN3))) return rc;
-    c->cap_phi = D.cap3;
+  if (!fixed) {
+    if ((rc = ensure_molecules(c, nm, nbox))) return rc;
+    D.nm = nm;
+    D.nbox = nbox;  // before ensure_lists: the reduction arrays are sized by it
+    // triplet capacity: the last call's, or a bulk-water estimate
+    if ((rc = ensure_lists(c, std::max<long>(D.cap3, (long)nm * 8), 0))) return rc;
+    if (phi && D.cap3 > c->cap_phi) {
+      if ((rc = dev_alloc(c, c->d_phitrip, (size_t)D.cap3 * 2 * NLIN3))) return rc;
+      c->cap_phi = D.cap3;
+    }
   }
   D.fs2_parts = c->tune.fs2_parts;
   D.fs2_pair_parts = c->tune.fs2_pair_parts;
-  if (!capture && (c->h_molbox != c->up_molbox || c->h_boxL != c->up_boxL || c->h_boxstart != c->up_boxstart)) {
+  if (!fixed && !capture &&
+      (c->h_molbox != c->up_molbox || c->h_boxL != c->up_boxL || c->h_boxstart != c->up_boxstart)) {
     forget_upload(c);
     HIPCHK(c, hipMemcpyAsync(D.mol_box, c->h_molbox.data(), sizeof(int) * nm, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(D.boxL, c->h_boxL.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
@@ -409,6 +421,7 @@ int run_checked(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const
                 const double* xs, const double* ys, const double* zs, double* fxs, double* fys, double* fzs,
                 double* fcom2, double* fcom3, double* energies, hipStream_t st, double* phi = nullptr) {
   Dev& D = c->D;
+  c->last_noalloc = false;
   for (int attempt = 0; attempt < 3; ++attempt) {
     int rc = run(c, nm, nbox, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st, phi);
     if (rc) return rc;
@@ -462,6 +475,7 @@ int eval_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box
                double* fys, double* fzs, double* fcom2, double* fcom3, double* energies, double* phi,
                void* stream);
 int check_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len, int ld);
+void* noalloc_scratch(int dev);
 void set_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len);
 int clone_ctx(const ccpol3b_ctx* c, ccpol3b_ctx** out);
 
@@ -583,6 +597,9 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return CCPOL3B_ENODEV;
   if (hipSetDevice(dev) != hipSuccess) return CCPOL3B_ENODEV;
 
+  // the allocation-free entry point's scratch array on this device (its
+  // address is resolved once per device; a failure surfaces at that entry point)
+  (void)noalloc_scratch(dev);
   ccpol3b_ctx* c = new ccpol3b_ctx();
   c->device = dev;
   c->precision = p->precision;
@@ -999,7 +1016,8 @@ long ccpol3b_get_fs3_list(ccpol3b_ctx* c, int* ijk, long cap) {
   const long m = std::min(n, cap);
   if (m > 0) {
     std::vector<int4> tmp(m);
-    if (hipMemcpy(tmp.data(), c->D.tri, sizeof(int4) * m, hipMemcpyDeviceToHost) != hipSuccess)
+    const int4* tri = c->last_noalloc ? c->nD.tri : c->D.tri;
+    if (hipMemcpy(tmp.data(), tri, sizeof(int4) * m, hipMemcpyDeviceToHost) != hipSuccess)
       return fail(c, CCPOL3B_EHIP, "copy of the FS3 list");
     for (long t = 0; t < m; ++t) {
       ijk[3 * t] = tmp[t].x;
@@ -1012,6 +1030,9 @@ long ccpol3b_get_fs3_list(ccpol3b_ctx* c, int* ijk, long cap) {
 
 long ccpol3b_get_fs2_list(ccpol3b_ctx* c, int* kij, long cap) {
   if (!c || (cap > 0 && !kij)) return CCPOL3B_EINVAL;
+  if (c->last_noalloc)
+    return fail(c, CCPOL3B_EINVAL, "the FS2 list is exported after ccpol3b_eval_device, not after the "
+                                   "allocation-free call");
   if (!c->s2 || c->D.nm == 0) return 0;
   // The evaluation does not materialise the FS2 items; build them from the
   // neighbour lists of the last call, in the reference's order.
@@ -1131,3 +1152,123 @@ int ccpol3b_graph_status(const ccpol3b_graph* g) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free entry point
+// ccpol3b_eval_device_noalloc evaluates one box with every per-call device
+// array carved from `scratch`, a file-scope device array of fixed size, instead
+// of the context's allocations: no hipMalloc/hipFree, no host-to-device copy.
+// Each GPU holds its own copy of the array; its address is resolved once per
+// device (hipGetSymbolAddress with that device current) and kept in
+// noalloc_ptr[].  Host code uses that address, never the array's name.  All
+// calls on one GPU share the array (it keeps the last call's contents; k_prep
+// clears the accumulators and counters each call), so calls that overlap on
+// one GPU race.
+namespace {
+
+constexpr int NA_NM_MAX = 1024;            // molecules of one call
+constexpr int NA_CAP_NB = 128;             // FS3 / FS2 neighbours per molecule
+constexpr long NA_CAP3 = 96L * NA_NM_MAX;  // FS3 triplets
+constexpr size_t NA_SCRATCH_BYTES = size_t(32) << 20;
+constexpr int NA_MAX_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+std::mutex noalloc_mutex;
+void* noalloc_ptr[NA_MAX_DEVICES] = {};
+
+// Lays the arrays of one evaluation (capacities NA_*) out from `base`, each
+// aligned to 256 bytes, and returns the bytes used; with base == nullptr it
+// only measures.
+size_t noalloc_carve(unsigned char* base, Dev& D) {
+  size_t off = 0;
+  auto take = [&](auto*& p, size_t n) {
+    using P = std::remove_reference_t<decltype(*p)>;
+    p = base ? reinterpret_cast<P*>(base + off) : nullptr;
+    off += (std::max<size_t>(n, 1) * sizeof(P) + 255) / 256 * 256;
+  };
+  const size_t n = NA_NM_MAX, nn = n * NA_CAP_NB, nbox = 1;
+  const long nred = std::max<long>(NA_CAP3, (long)NA_NM_MAX * FS2_PARTS_MAX) / RED_CHUNK + (long)nbox + 2;
+  D.cap_nb = NA_CAP_NB;
+  D.cap3 = NA_CAP3;
+  D.cap_pairs2 = D.cap_sl2 = (long)nn;
+  D.cap_items2 = 0;
+  D.cap_red = nred;
+  D.items2 = nullptr;
+  take(D.cx, n); take(D.cy, n); take(D.cz, n); take(D.site, n * NS3 * 3);
+  take(D.mol_box, n); take(D.boxL, nbox); take(D.box_start, nbox + 1);
+  take(D.nb3_cnt, n); take(D.nb3, nn); take(D.up3, n);
+  take(D.nb2_cnt, n); take(D.nb2, nn); take(D.up2, n); take(D.nbv2, 3 * nn);
+  take(D.sl2_cnt, n); take(D.sl2_off, n + 1);
+  take(D.cnt3s, n); take(D.off3s, n + 1);
+  take(D.cnt2s, nn); take(D.off2s, nn + 1);
+  take(D.tri, (size_t)NA_CAP3); take(D.geo3, (size_t)NA_CAP3);
+  take(D.pcnt2, n); take(D.poff2, n + 1);
+  take(D.acc, n * ACC_W); take(D.ew3, 2 * (size_t)NA_CAP3); take(D.ew2, n * FS2_PARTS_MAX * 2);
+  take(D.box_t3, nbox + 1); take(D.box_t2, nbox + 1); take(D.box_c3, nbox + 1); take(D.box_c2, nbox + 1);
+  take(D.red3, 2 * (size_t)nred); take(D.red2, 2 * (size_t)nred);
+  take(D.s2pair, 2 * nn);
+  take(D.totals, T_COUNT);
+  return off;
+}
+
+// The box description of a single box: every molecule in box 0 of length L (bohr).
+__global__ void k_single_box(Dev D, int nm, double L) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nm) D.mol_box[t] = 0;
+  if (t == 0) {
+    D.boxL[0] = L;
+    D.box_start[0] = 0;
+    D.box_start[1] = nm;
+  }
+}
+
+// Address of this device's copy of `scratch` (dev must be the current device);
+// resolved on first use and kept.  nullptr if it cannot be resolved.
+void* noalloc_scratch(int dev) {
+  if (dev < 0 || dev >= NA_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(noalloc_mutex);
+  if (!noalloc_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess && p) noalloc_ptr[dev] = p;
+  }
+  return noalloc_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" int ccpol3b_eval_device_noalloc(ccpol3b_ctx* c, int nm, int ld, const double* x, const double* y,
+                                           const double* z, const double* xs, const double* ys,
+                                           const double* zs, double box, double* fxs, double* fys, double* fzs,
+                                           double* fcom2, double* fcom3, double* energies, void* stream) {
+  if (!c || nm < 0 || ld < 17 || (nm > 0 && (!x || !y || !z || !xs || !ys || !zs || !fxs || !fys || !fzs ||
+                                             !fcom2 || !fcom3 || !energies)))
+    return fail(c, CCPOL3B_EINVAL, "invalid argument");
+  if (nm > NA_NM_MAX)
+    return fail(c, CCPOL3B_ECAPACITY, "the allocation-free entry point takes at most 1024 molecules");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, CCPOL3B_EHIP, "hipSetDevice");
+  unsigned char* base = static_cast<unsigned char*>(noalloc_scratch(c->device));
+  if (!base) return fail(c, CCPOL3B_EHIP, "address of the allocation-free scratch array");
+  Dev D{};
+  if (noalloc_carve(nullptr, D) > NA_SCRATCH_BYTES)
+    return fail(c, CCPOL3B_EHIP, "allocation-free scratch layout exceeds its array");
+  if (nm == 0) return CCPOL3B_OK;
+  noalloc_carve(base, D);
+  D.nm = nm;
+  D.nbox = 1;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_single_box, dim3((nm + 255) / 256), dim3(256), 0, st, D, nm,
+                     box > 0.0 ? box / BOHR2A : 0.0);
+  LAUNCHCHK(c, "box description");
+  int rc = run(c, nm, 1, ld, x, y, z, xs, ys, zs, fxs, fys, fzs, fcom2, fcom3, energies, st, nullptr, false, &D);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(st));
+  const int* tot = c->h_totals + T_COUNT;
+  if (tot[T_NBOVER] > 0)
+    return fail(c, CCPOL3B_ECAPACITY, "a molecule has more than 128 neighbours (allocation-free entry point)");
+  if (tot[T_N3] > D.cap3)
+    return fail(c, CCPOL3B_ECAPACITY, "more than 98304 FS3 triplets (allocation-free entry point)");
+  c->nD = D;
+  c->last_noalloc = true;
+  finish_stats(c);
+  return CCPOL3B_OK;
+}

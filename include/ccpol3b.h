@@ -113,6 +113,27 @@ int ccpol3b_eval_device(ccpol3b_ctx *ctx, int nm, int ld_site,
                         double *d_fcom2, double *d_fcom3, /* [3*nm] x..,y..,z.. blocks */
                         double *d_energies, void *stream);
 
+/* Allocation-free variant of ccpol3b_eval_device (same arguments, layouts,
+ * units and output semantics).  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory -- tight serving loops and
+ * harnesses that forbid hipMalloc.  Every per-call device array lives in one
+ * fixed 32 MiB device array of the library (one copy per GPU, held until the
+ * process ends), so the call makes no hipMalloc, hipFree or host-to-device
+ * copy; it reads the list sizes back once (one small device-to-host read) to
+ * check them.  Limits: at most 1024 molecules (else CCPOL3B_ECAPACITY before
+ * any device work), at most 128 FS3/FS2 neighbours per molecule and 98304 FS3
+ * triplets (else CCPOL3B_ECAPACITY, outputs untouched).  All calls on one GPU
+ * share that array: calls that overlap on one GPU (several host threads or
+ * streams) race.  ccpol3b_get_fs3_list reads its last list;
+ * ccpol3b_get_fs2_list is not available after it. */
+int ccpol3b_eval_device_noalloc(ccpol3b_ctx *ctx, int nm, int ld_site,
+                                const double *d_x, const double *d_y, const double *d_z,
+                                const double *d_xs, const double *d_ys, const double *d_zs,
+                                double box,
+                                double *d_fxs, double *d_fys, double *d_fzs,
+                                double *d_fcom2, double *d_fcom3,
+                                double *d_energies, void *stream);
+
 /* Batched evaluation of nbox independent boxes in one launch sequence.
  * Molecules of box b are rows mol_start[b] .. mol_start[b+1]-1 of the packed
  * device arrays; box_len[b] (A, host array) <= 0 means open boundary.
