@@ -165,6 +165,7 @@ struct ccpol3b_ctx {
   // when that was the last call
   Dev nD{};
   bool last_noalloc = false;
+  long version = 0;  // bumped by set_precision, set_coefficients, set_tuning
 };
 
 namespace {
@@ -713,6 +714,7 @@ void ccpol3b_destroy(ccpol3b_ctx* c) {
 int ccpol3b_set_precision(ccpol3b_ctx* c, int precision) {
   if (!c || (precision != CCPOL3B_FP64 && precision != CCPOL3B_MIXED)) return CCPOL3B_EINVAL;
   c->precision = precision;
+  ++c->version;
   return CCPOL3B_OK;
 }
 
@@ -765,6 +767,7 @@ int ccpol3b_set_coefficients(ccpol3b_ctx* c, const double* cs_sr, const double* 
   HIPCHK(c, hipDeviceSynchronize());
   c->h_cs[0].assign(cs_sr, cs_sr + NLIN3);
   c->h_cs[1].assign(cs_lr, cs_lr + NLIN3);
+  ++c->version;
   return upload_coefficients(c, cs_sr, cs_lr);
 }
 
@@ -780,6 +783,7 @@ int ccpol3b_set_tuning(ccpol3b_ctx* c, const ccpol3b_tuning* t) {
     return fail(c, CCPOL3B_EINVAL, "invalid tuning");
   c->tune = *t;
   c->tune.fs3_lds_coef = t->fs3_lds_coef ? 1 : 0;
+  ++c->version;
   return CCPOL3B_OK;
 }
 
@@ -825,6 +829,7 @@ int ccpol3b_autotune(ccpol3b_ctx* c, int nbox, const int* mol_start, const doubl
     return rc;
   }
   c->tune = win;
+  ++c->version;
   if (best) *best = win;
   if (best_ms) *best_ms = win_ms;
   return CCPOL3B_OK;
@@ -1143,6 +1148,18 @@ int ccpol3b_graph_launch(ccpol3b_graph* g, void* stream) {
   if (hipGraphLaunch(g->x, (hipStream_t)stream) != hipSuccess) return fail(g->c, CCPOL3B_EHIP, "hipGraphLaunch");
   return CCPOL3B_OK;
 }
+
+}  // extern "C"
+
+long ccpol3b::context_version(const ccpol3b_ctx* c) { return c ? c->version : -1; }
+
+const int* ccpol3b::graph_device_totals(const ccpol3b_graph* g, long* cap3) {
+  if (!g || !g->c) return nullptr;
+  if (cap3) *cap3 = g->c->D.cap3;
+  return g->c->D.totals;
+}
+
+extern "C" {
 
 int ccpol3b_graph_status(const ccpol3b_graph* g) {
   if (!g || !g->c) return CCPOL3B_EINVAL;

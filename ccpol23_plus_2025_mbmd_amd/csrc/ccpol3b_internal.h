@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct ccpol3b_graph;  // include/ccpol3b.h
+struct ccpol3b_ctx;
+
 namespace ccpol3b {
 
 constexpr int NS3 = 17;        // FS3 sites per molecule
@@ -109,5 +112,14 @@ struct Dev {
   double *s2pair;             // [cap_pairs2][2] S_ij and dS/dlambda of each FS2 pair
   int *totals;                // [T_COUNT]
 };
+
+// The device counters of a graph's private context (totals[T_COUNT], zeroed and
+// refilled by every replay) and the triplet capacity it was recorded with: a
+// replay overflowed when totals[T_NBOVER] > 0 or totals[T_N3] > cap3.
+const int* graph_device_totals(const ::ccpol3b_graph* g, long* cap3);
+
+// A counter that changes whenever the context's precision, coefficients or
+// tuning change (what a recorded graph holds a private copy of).
+long context_version(const ::ccpol3b_ctx* c);
 
 }  // namespace ccpol3b

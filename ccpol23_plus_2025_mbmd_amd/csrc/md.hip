@@ -1,12 +1,32 @@
-// Rigid-water NVE driver on the device: velocity Verlet for the centres of
-// mass and the NO_SQUISH symplectic rotation for the orientations, as
-// nve_1 (evolve.f90:621-1062) and nosquish (evolve.f90:8107-8204) integrate
-// them, with the forces of the three-body term evaluated by this library each
-// step.  State per molecule: COM (A), quaternion, COM velocity (A/ps) and
-// body-frame angular momentum (amu A^2/ps); forces in 10 J mol^-1 A^-1.
+// Rigid-water NVE driver on the device, for the three-body forces of this
+// library (the caller's two-body, Ewald and induction terms are not part of it).
+//
+// Scheme.  Centres of mass: velocity Verlet.  Orientations: the symplectic
+// NO_SQUISH splitting of Miller, Eleftheriou, Pattnaik, Ndirango, Newns and
+// Martyna (J. Chem. Phys. 116, 8649 (2002)), the integrator the reference's
+// nve_1 uses (evolve.f90:621-1062, 8107-8204).  In the paper's notation the
+// orientation is a unit quaternion q with conjugate momentum p, related to the
+// body-frame angular momentum j by p = 2 sum_k j_k P_k q and j_k = (P_k q).p / 2,
+// where P_1..P_3 are the signed permutations of the quaternion components
+// below.  The free-rotor flow splits into exact rotations about the principal
+// axes: rotating about axis k for a time t turns (q, p) in the planes of
+// (q, P_k q) and (p, P_k p) by the angle t (p . P_k q) / (4 I_k).  One
+// sub-step applies axes 3, 2, 1, 2, 3 for t/2, t/2, t, t/2, t/2; a time step
+// is NSUB such sub-steps.  Torques act on j directly: the half kick
+// p += dt/2 * 2 sum_k tau_k P_k q at fixed q is j += dt/2 * tau (body frame).
+//
+// Per step (one launch sequence, replayed from a HIP graph for the forces):
+//   k_md_stage(FIRST):  half kick of v and j, drift of x, NO_SQUISH rotation;
+//   k_md_sites:         lab-frame site offsets R(q) c_s, site forces zeroed;
+//   three-body forces:  the recorded evaluation (ccpol3b_graph_launch);
+//   k_md_stage(SECOND): half kick, COM wrapped into the box, kinetic energies;
+//   k_md_record:        E3, Kt, Kr and their sum for the step.
+// State per molecule: COM (A), quaternion, COM velocity (A/ps) and body-frame
+// angular momentum (amu A^2/ps); forces in 10 J mol^-1 A^-1.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "../../include/ccpol3b.h"
@@ -14,195 +34,246 @@
 
 namespace ccpol3b {
 
-struct MdDev {
-  int nm, ns, ld;
-  double dt, totm, pmi[3], box;
-  double *com;    // [3][nm]
-  double *q;      // [4][nm]
-  double *vel;    // [3][nm]
-  double *jb;     // [3][nm]
-  double *pc;     // [ns][3] principal-frame sites
-  double *site;   // [3][nm][ld] lab-frame offsets
-  double *fsite;  // [3][nm][ld]
-  double *fc2, *fc3;  // [3][nm]
-  double *en;     // [3] qfs2, qfs3, vir increment
-  double *ke;     // [2][nm] translational, rotational kinetic energy per molecule
-  double *rec;    // [nsteps][4] E3, Kt, Kr, E3 + Kt + Kr (kcal/mol)
+constexpr int NSUB = 10;  // NO_SQUISH sub-steps per time step
+
+// (P_k q)[a] = sign * q[src]: the three signed permutations of the paper, as
+// (source component, sign) per output component.
+struct SignedPerm {
+  int src[4];
+  double sign[4];
+};
+constexpr SignedPerm kPerm[3] = {
+    {{1, 0, 3, 2}, {-1.0, 1.0, 1.0, -1.0}},   // P_1
+    {{2, 3, 0, 1}, {-1.0, -1.0, 1.0, 1.0}},   // P_2
+    {{3, 2, 1, 0}, {-1.0, 1.0, -1.0, 1.0}},   // P_3
 };
 
-// rott(1..9) of quat_to_rot (molecular_sites.f90:95-111), row major
-__device__ __forceinline__ void md_rot(double q0, double q1, double q2, double q3, double* r) {
-  r[0] = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
-  r[1] = 2.0 * (q1 * q2 - q0 * q3);
-  r[2] = 2.0 * (q1 * q3 + q0 * q2);
-  r[3] = 2.0 * (q1 * q2 + q0 * q3);
-  r[4] = q0 * q0 - q1 * q1 + q2 * q2 - q3 * q3;
-  r[5] = 2.0 * (q2 * q3 - q0 * q1);
-  r[6] = 2.0 * (q1 * q3 - q0 * q2);
-  r[7] = 2.0 * (q2 * q3 + q0 * q1);
-  r[8] = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
+// The symmetric sequence of one sub-step: (axis, fraction of the sub-step).
+struct RotorStep {
+  int axis;
+  double frac;
+};
+constexpr RotorStep kRotorSeq[5] = {{2, 0.5}, {1, 0.5}, {0, 1.0}, {1, 0.5}, {2, 0.5}};
+
+__device__ __forceinline__ void permute(int k, const double* v, double* out) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) out[a] = kPerm[k].sign[a] * v[kPerm[k].src[a]];
 }
 
-// Lab-frame site offsets R(q) pc (transform, iway = 1) and zeroed site forces.
+__device__ __forceinline__ double dot4(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+
+// Exact free rotation about principal axis k for time t.
+__device__ __forceinline__ void free_rotation(int k, double t, double inertia, double* q, double* p) {
+  double pq[4], pp[4];
+  permute(k, q, pq);
+  permute(k, p, pp);
+  const double angle = t * dot4(p, pq) / (4.0 * inertia);
+  double s, c;
+  sincos(angle, &s, &c);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    q[a] = c * q[a] + s * pq[a];
+    p[a] = c * p[a] + s * pp[a];
+  }
+}
+
+// NO_SQUISH propagation of (q, j) over dt with principal moments pmi.
+__device__ void rotate_rigid_body(double dt, const double* pmi, double* q, double* j) {
+  double p[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double pq[4];
+    permute(k, q, pq);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) p[a] += 2.0 * j[k] * pq[a];
+  }
+  const double t = dt / NSUB;
+  for (int n = 0; n < NSUB; ++n) {
+#pragma unroll
+    for (int r = 0; r < 5; ++r) free_rotation(kRotorSeq[r].axis, kRotorSeq[r].frac * t, pmi[kRotorSeq[r].axis], q, p);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double pq[4];
+    permute(k, q, pq);
+    j[k] = 0.5 * dot4(p, pq);
+  }
+}
+
+// Rotation matrix of a unit quaternion (body -> lab), row major; the same
+// matrix as quat_to_rot (molecular_sites.f90:95-111).
+__device__ __forceinline__ void rotation_matrix(const double* q, double* R) {
+  const double w = q[0], x = q[1], y = q[2], z = q[3];
+  R[0] = w * w + x * x - y * y - z * z;
+  R[1] = 2.0 * (x * y - w * z);
+  R[2] = 2.0 * (x * z + w * y);
+  R[3] = 2.0 * (x * y + w * z);
+  R[4] = w * w - x * x + y * y - z * z;
+  R[5] = 2.0 * (y * z - w * x);
+  R[6] = 2.0 * (x * z - w * y);
+  R[7] = 2.0 * (y * z + w * x);
+  R[8] = w * w - x * x - y * y + z * z;
+}
+
+struct MdDev {
+  int nm, ns, ld;
+  double dt, mass, pmi[3], box;
+  double* com;    // [3][nm]
+  double* q;      // [4][nm]
+  double* vel;    // [3][nm]
+  double* jb;     // [3][nm]
+  double* pc;     // [ns][3] principal-frame sites
+  double* site;   // [3][nm][ld] lab-frame offsets
+  double* fsite;  // [3][nm][ld]
+  double* fc2;    // [3][nm]
+  double* fc3;    // [3][nm]
+  double* en;     // [3] qfs2, qfs3, vir increment
+  double* ke;     // [2][nm] translational, rotational kinetic energy per molecule
+  double* rec;    // [nsteps][4] E3, Kt, Kr, E3 + Kt + Kr (kcal/mol)
+  int* failed;    // set when a force evaluation outgrew its list capacities
+  const int* ftot;  // the force evaluation's counters (see graph_device_totals)
+  long fcap3;
+};
+
+// One rigid body in registers.
+struct Body {
+  double x[3], v[3], q[4], j[3];
+};
+
+__device__ __forceinline__ Body load_body(const MdDev& M, int m) {
+  Body b;
+  const int nm = M.nm;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    b.x[c] = M.com[c * nm + m];
+    b.v[c] = M.vel[c * nm + m];
+    b.j[c] = M.jb[c * nm + m];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) b.q[c] = M.q[c * nm + m];
+  return b;
+}
+
+__device__ __forceinline__ void store_body(const MdDev& M, int m, const Body& b) {
+  const int nm = M.nm;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    M.com[c * nm + m] = b.x[c];
+    M.vel[c * nm + m] = b.v[c];
+    M.jb[c * nm + m] = b.j[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) M.q[c * nm + m] = b.q[c];
+}
+
+// Molecular force (sites + both COM terms) and the body-frame torque about the COM.
+__device__ __forceinline__ void wrench(const MdDev& M, int m, const double* q, double* F, double* tb) {
+  const long plane = (long)M.nm * M.ld, base = (long)m * M.ld;
+  double tl[3] = {0.0, 0.0, 0.0};
+  for (int c = 0; c < 3; ++c) F[c] = M.fc2[c * M.nm + m] + M.fc3[c * M.nm + m];
+  for (int s = 0; s < M.ns; ++s) {
+    double f[3], r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      f[c] = M.fsite[c * plane + base + s];
+      r[c] = M.site[c * plane + base + s];
+      F[c] += f[c];
+    }
+    tl[0] += r[1] * f[2] - r[2] * f[1];
+    tl[1] += r[2] * f[0] - r[0] * f[2];
+    tl[2] += r[0] * f[1] - r[1] * f[0];
+  }
+  double R[9];
+  rotation_matrix(q, R);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) tb[a] = R[a] * tl[0] + R[3 + a] * tl[1] + R[6 + a] * tl[2];  // R^T tau
+}
+
+__device__ __forceinline__ bool evaluation_overflowed(const MdDev& M) {
+  return M.ftot[T_NBOVER] > 0 || M.ftot[T_N3] > M.fcap3;
+}
+
+enum { STAGE_FIRST = 1, STAGE_SECOND = 2 };
+
+// One half of the velocity-Verlet step for molecule m, with the forces of the
+// last evaluation.  Once an evaluation outgrew its capacities the state stays
+// put for the rest of the run (the host repeats the run with larger lists).
+__global__ void k_md_stage(MdDev M, int stage) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M.nm || *M.failed) return;
+  if (evaluation_overflowed(M)) {
+    if (m == 0) *M.failed = 1;
+    return;
+  }
+  Body b = load_body(M, m);
+  double F[3], tb[3];
+  wrench(M, m, b.q, F, tb);
+  const double h = 0.5 * M.dt;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    b.v[c] += h * F[c] / M.mass;
+    b.j[c] += h * tb[c];
+  }
+  if (stage == STAGE_FIRST) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b.x[c] += M.dt * b.v[c];
+    rotate_rigid_body(M.dt, M.pmi, b.q, b.j);
+  } else {
+    if (M.box > 0.0)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) b.x[c] -= M.box * round(b.x[c] / M.box);  // NINT: half away from zero
+    M.ke[m] = 0.5 * M.mass * (b.v[0] * b.v[0] + b.v[1] * b.v[1] + b.v[2] * b.v[2]);
+    M.ke[M.nm + m] = 0.5 * (b.j[0] * b.j[0] / M.pmi[0] + b.j[1] * b.j[1] / M.pmi[1] + b.j[2] * b.j[2] / M.pmi[2]);
+  }
+  store_body(M, m, b);
+}
+
+// Lab-frame site offsets R(q) c_s, and zeroed site forces (the evaluation adds to them).
 __global__ void k_md_sites(MdDev M) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)M.nm * M.ld) return;
+  if (t >= (long)M.nm * M.ld || *M.failed) return;
   const int m = (int)(t / M.ld), s = (int)(t - (long)m * M.ld);
   const long plane = (long)M.nm * M.ld;
-  double x = 0.0, y = 0.0, z = 0.0;
+  double lab[3] = {0.0, 0.0, 0.0};
   if (s < M.ns) {
-    double r[9];
-    md_rot(M.q[m], M.q[M.nm + m], M.q[2 * M.nm + m], M.q[3 * M.nm + m], r);
-    const double* p = M.pc + 3 * s;
-    x = p[0] * r[0] + p[1] * r[1] + p[2] * r[2];
-    y = p[0] * r[3] + p[1] * r[4] + p[2] * r[5];
-    z = p[0] * r[6] + p[1] * r[7] + p[2] * r[8];
+    const double q[4] = {M.q[m], M.q[M.nm + m], M.q[2 * M.nm + m], M.q[3 * M.nm + m]};
+    double R[9];
+    rotation_matrix(q, R);
+    const double* c = M.pc + 3 * s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) lab[a] = R[3 * a] * c[0] + R[3 * a + 1] * c[1] + R[3 * a + 2] * c[2];
   }
-  M.site[t] = x;
-  M.site[plane + t] = y;
-  M.site[2 * plane + t] = z;
-  M.fsite[t] = 0.0;
-  M.fsite[plane + t] = 0.0;
-  M.fsite[2 * plane + t] = 0.0;
-}
-
-// nosquish (evolve.f90:8107-8204): 10 sub-steps of the z/2, y/2, x, y/2, z/2
-// free-rotor splitting of the quaternion and its conjugate momentum.
-__device__ void md_nosquish(double tstep, const double* pmi, double* qq, double* pp) {
-  const int mrot = 10;
-  const double ov4 = 0.25, ov8 = 0.125;
-  const double tr = tstep / (double)mrot;
-  for (int m = 0; m < mrot; ++m) {
-    double cs, sn, z;
-    double qn1[4], pq2[4], qn2[4], pq3[4], qn3[4], pq4[4];
-    z = ov8 / pmi[2] * tr * (-pp[0] * qq[3] + pp[1] * qq[2] - pp[2] * qq[1] + pp[3] * qq[0]);
-    cs = cos(z);
-    sn = sin(z);
-    qn1[0] = cs * qq[0] - sn * qq[3]; qn1[1] = cs * qq[1] + sn * qq[2];
-    qn1[2] = cs * qq[2] - sn * qq[1]; qn1[3] = cs * qq[3] + sn * qq[0];
-    pq2[0] = cs * pp[0] - sn * pp[3]; pq2[1] = cs * pp[1] + sn * pp[2];
-    pq2[2] = cs * pp[2] - sn * pp[1]; pq2[3] = cs * pp[3] + sn * pp[0];
-    z = ov8 / pmi[1] * tr * (-pq2[0] * qn1[2] - pq2[1] * qn1[3] + pq2[2] * qn1[0] + pq2[3] * qn1[1]);
-    cs = cos(z);
-    sn = sin(z);
-    qn2[0] = cs * qn1[0] - sn * qn1[2]; qn2[1] = cs * qn1[1] - sn * qn1[3];
-    qn2[2] = cs * qn1[2] + sn * qn1[0]; qn2[3] = cs * qn1[3] + sn * qn1[1];
-    pq3[0] = cs * pq2[0] - sn * pq2[2]; pq3[1] = cs * pq2[1] - sn * pq2[3];
-    pq3[2] = cs * pq2[2] + sn * pq2[0]; pq3[3] = cs * pq2[3] + sn * pq2[1];
-    z = ov4 / pmi[0] * tr * (-pq3[0] * qn2[1] + pq3[1] * qn2[0] + pq3[2] * qn2[3] - pq3[3] * qn2[2]);
-    cs = cos(z);
-    sn = sin(z);
-    qn3[0] = cs * qn2[0] - sn * qn2[1]; qn3[1] = cs * qn2[1] + sn * qn2[0];
-    qn3[2] = cs * qn2[2] + sn * qn2[3]; qn3[3] = cs * qn2[3] - sn * qn2[2];
-    pq4[0] = cs * pq3[0] - sn * pq3[1]; pq4[1] = cs * pq3[1] + sn * pq3[0];
-    pq4[2] = cs * pq3[2] + sn * pq3[3]; pq4[3] = cs * pq3[3] - sn * pq3[2];
-    z = ov8 / pmi[1] * tr * (-pq4[0] * qn3[2] - pq4[1] * qn3[3] + pq4[2] * qn3[0] + pq4[3] * qn3[1]);
-    cs = cos(z);
-    sn = sin(z);
-    qn2[0] = cs * qn3[0] - sn * qn3[2]; qn2[1] = cs * qn3[1] - sn * qn3[3];
-    qn2[2] = cs * qn3[2] + sn * qn3[0]; qn2[3] = cs * qn3[3] + sn * qn3[1];
-    pq3[0] = cs * pq4[0] - sn * pq4[2]; pq3[1] = cs * pq4[1] - sn * pq4[3];
-    pq3[2] = cs * pq4[2] + sn * pq4[0]; pq3[3] = cs * pq4[3] + sn * pq4[1];
-    z = ov8 / pmi[2] * tr * (-pq3[0] * qn2[3] + pq3[1] * qn2[2] - pq3[2] * qn2[1] + pq3[3] * qn2[0]);
-    cs = cos(z);
-    sn = sin(z);
-    qq[0] = cs * qn2[0] - sn * qn2[3]; qq[1] = cs * qn2[1] + sn * qn2[2];
-    qq[2] = cs * qn2[2] - sn * qn2[1]; qq[3] = cs * qn2[3] + sn * qn2[0];
-    pp[0] = cs * pq3[0] - sn * pq3[3]; pp[1] = cs * pq3[1] + sn * pq3[2];
-    pp[2] = cs * pq3[2] - sn * pq3[1]; pp[3] = cs * pq3[3] + sn * pq3[0];
-  }
-}
-
-// One half of the velocity-Verlet step for molecule m (evolve.f90:659-781
-// for stage 1, :889-1017 for stage 2): molecular force and torque from the
-// site and COM forces, half kicks of the COM velocity and of the quaternion
-// momentum; stage 1 then drifts the COM and rotates with NO_SQUISH, stage 2
-// wraps the COM into the box and records the kinetic energies.
-__global__ void k_md_kick(MdDev M, int stage) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= M.nm) return;
-  const int nm = M.nm;
-  const long plane = (long)nm * M.ld, b = (long)m * M.ld;
-  double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
-  for (int s = 0; s < M.ns; ++s) {
-    const double fx = M.fsite[b + s], fy = M.fsite[plane + b + s], fz = M.fsite[2 * plane + b + s];
-    const double x = M.site[b + s], y = M.site[plane + b + s], z = M.site[2 * plane + b + s];
-    F[0] += fx;
-    F[1] += fy;
-    F[2] += fz;
-    T[0] += y * fz - z * fy;
-    T[1] += z * fx - x * fz;
-    T[2] += x * fy - y * fx;
-  }
-  for (int c = 0; c < 3; ++c) F[c] = F[c] + M.fc2[c * nm + m] + M.fc3[c * nm + m];
-  double q[4] = {M.q[m], M.q[nm + m], M.q[2 * nm + m], M.q[3 * nm + m]};
-  double r[9];
-  md_rot(q[0], q[1], q[2], q[3], r);
-  const double tqx = T[0] * r[0] + T[1] * r[3] + T[2] * r[6];
-  const double tqy = T[0] * r[1] + T[1] * r[4] + T[2] * r[7];
-  const double tqz = T[0] * r[2] + T[1] * r[5] + T[2] * r[8];
-  const double qt0 = 2.0 * (-q[1] * tqx - q[2] * tqy - q[3] * tqz);
-  const double qt1 = 2.0 * (q[0] * tqx - q[3] * tqy + q[2] * tqz);
-  const double qt2 = 2.0 * (q[3] * tqx + q[0] * tqy - q[1] * tqz);
-  const double qt3 = 2.0 * (-q[2] * tqx + q[1] * tqy + q[0] * tqz);
-  const double opx = M.jb[m], opy = M.jb[nm + m], opz = M.jb[2 * nm + m];
-  double p[4];
-  p[0] = 2.0 * (-q[1] * opx - q[2] * opy - q[3] * opz);
-  p[1] = 2.0 * (q[0] * opx - q[3] * opy + q[2] * opz);
-  p[2] = 2.0 * (q[3] * opx + q[0] * opy - q[1] * opz);
-  p[3] = 2.0 * (-q[2] * opx + q[1] * opy + q[0] * opz);
-  p[0] = p[0] + qt0 * 0.5 * M.dt;
-  p[1] = p[1] + qt1 * 0.5 * M.dt;
-  p[2] = p[2] + qt2 * 0.5 * M.dt;
-  p[3] = p[3] + qt3 * 0.5 * M.dt;
-  double v[3];
-  for (int c = 0; c < 3; ++c) v[c] = M.vel[c * nm + m] + 0.5 * F[c] * M.dt / M.totm;
-  double x[3] = {M.com[m], M.com[nm + m], M.com[2 * nm + m]};
-  if (stage == 1) {
-    for (int c = 0; c < 3; ++c) x[c] = x[c] + M.dt * v[c];
-    md_nosquish(M.dt, M.pmi, q, p);
-  } else if (M.box > 0.0) {
-    for (int c = 0; c < 3; ++c) x[c] = x[c] - M.box * round(x[c] / M.box);
-  }
-  const double jx = 0.5 * (-q[1] * p[0] + q[0] * p[1] + q[3] * p[2] - q[2] * p[3]);
-  const double jy = 0.5 * (-q[2] * p[0] - q[3] * p[1] + q[0] * p[2] + q[1] * p[3]);
-  const double jz = 0.5 * (-q[3] * p[0] + q[2] * p[1] - q[1] * p[2] + q[0] * p[3]);
-  for (int c = 0; c < 3; ++c) {
-    M.com[c * nm + m] = x[c];
-    M.vel[c * nm + m] = v[c];
-  }
-  for (int c = 0; c < 4; ++c) M.q[c * nm + m] = q[c];
-  M.jb[m] = jx;
-  M.jb[nm + m] = jy;
-  M.jb[2 * nm + m] = jz;
-  if (stage == 2) {
-    M.ke[m] = 0.5 * M.totm * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);  // tkinetic_eng (utility.f90:164-179)
-    M.ke[nm + m] = 0.5 * (jx * jx / M.pmi[0] + jy * jy / M.pmi[1] + jz * jz / M.pmi[2]);  // rkinetic_eng
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    M.site[a * plane + t] = lab[a];
+    M.fsite[a * plane + t] = 0.0;
   }
 }
 
 // Step record: E3 = qfs2 + qfs3, Kt, Kr (kcal/mol) and their sum; one block,
 // fixed-order sums.
 __global__ __launch_bounds__(256) void k_md_record(MdDev M, int step) {
-  __shared__ double red[2][256];
+  __shared__ double part[2][256];
   double kt = 0.0, kr = 0.0;
   for (int m = threadIdx.x; m < M.nm; m += 256) {
     kt += M.ke[m];
     kr += M.ke[M.nm + m];
   }
-  red[0][threadIdx.x] = kt;
-  red[1][threadIdx.x] = kr;
+  part[0][threadIdx.x] = kt;
+  part[1][threadIdx.x] = kr;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + w];
-      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+      part[0][threadIdx.x] += part[0][threadIdx.x + w];
+      part[1][threadIdx.x] += part[1][threadIdx.x + w];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     const double e3 = M.en[0] + M.en[1];
-    const double t = red[0][0] / 418.4, r = red[1][0] / 418.4;
+    const double t = part[0][0] / 418.4, r = part[1][0] / 418.4;
     double* o = M.rec + 4 * (long)step;
     o[0] = e3;
     o[1] = t;
@@ -220,6 +291,13 @@ struct ccpol3b_md {
   MdDev M{};
   int cap_steps = 0;
   bool forces_current = false;
+  // the force evaluation, recorded from the context into a HIP graph over the
+  // driver's arrays (recorded again when the context's precision, coefficients
+  // or tuning change, or a replay outgrew its list capacities)
+  ccpol3b_graph* graph = nullptr;
+  long graph_version = -1;  // context_version() when the graph was recorded
+  double headroom = 0.25;
+  double* saved = nullptr;  // [13][nm]: com, q, vel, jb at the start of a run
 };
 
 namespace {
@@ -239,18 +317,52 @@ void md_free(P*& p) {
   p = nullptr;
 }
 
-// site offsets, then the three-body forces at the current state
-int md_forces(ccpol3b_md* md) {
+// Records the force evaluation over the driver's arrays (sites must be current);
+// the recording's own checked evaluation leaves valid forces behind.
+int record_forces(ccpol3b_md* md, long version) {
   MdDev& M = md->M;
-  const long ns_tot = (long)M.nm * M.ld;
-  hipLaunchKernelGGL(k_md_sites, dim3((ns_tot + 255) / 256), dim3(256), 0, nullptr, M);
-  if (hipGetLastError() != hipSuccess) return CCPOL3B_EHIP;
+  if (md->graph) {
+    ccpol3b_graph_destroy(md->graph);
+    md->graph = nullptr;
+  }
   const long plane = (long)M.nm * M.ld;
-  int rc = ccpol3b_eval_device(md->ctx, M.nm, M.ld, M.com, M.com + M.nm, M.com + 2 * M.nm, M.site, M.site + plane,
-                               M.site + 2 * plane, M.box, M.fsite, M.fsite + plane, M.fsite + 2 * plane, M.fc2,
-                               M.fc3, M.en, nullptr);
-  if (rc == CCPOL3B_OK) md->forces_current = true;
-  return rc;
+  const int ms[2] = {0, M.nm};
+  const double bl[1] = {M.box};
+  int rc = ccpol3b_graph_create(md->ctx, 1, ms, bl, M.ld, M.com, M.com + M.nm, M.com + 2 * M.nm, M.site,
+                                M.site + plane, M.site + 2 * plane, M.fsite, M.fsite + plane, M.fsite + 2 * plane,
+                                M.fc2, M.fc3, M.en, nullptr, md->headroom, nullptr, &md->graph);
+  if (rc) return rc;
+  md->graph_version = version;
+  M.ftot = graph_device_totals(md->graph, &M.fcap3);
+  return M.ftot ? CCPOL3B_OK : CCPOL3B_EHIP;
+}
+
+int launch_sites(const MdDev& M) {
+  const long n = (long)M.nm * M.ld;
+  hipLaunchKernelGGL(k_md_sites, dim3((n + 255) / 256), dim3(256), 0, nullptr, M);
+  return hipGetLastError() == hipSuccess ? CCPOL3B_OK : CCPOL3B_EHIP;
+}
+
+// The arrays of the state, in the order they are saved.
+void state_arrays(MdDev& M, double* a[4], size_t n[4]) {
+  a[0] = M.com; n[0] = 3 * (size_t)M.nm;
+  a[1] = M.q; n[1] = 4 * (size_t)M.nm;
+  a[2] = M.vel; n[2] = 3 * (size_t)M.nm;
+  a[3] = M.jb; n[3] = 3 * (size_t)M.nm;
+}
+
+int copy_state(ccpol3b_md* md, bool save) {
+  double* a[4];
+  size_t n[4], off = 0;
+  state_arrays(md->M, a, n);
+  for (int k = 0; k < 4; ++k) {
+    double* dst = save ? md->saved + off : a[k];
+    const double* src = save ? a[k] : md->saved + off;
+    if (hipMemcpyAsync(dst, src, n[k] * sizeof(double), hipMemcpyDeviceToDevice, nullptr) != hipSuccess)
+      return CCPOL3B_EHIP;
+    off += n[k];
+  }
+  return CCPOL3B_OK;
 }
 
 }  // namespace
@@ -259,9 +371,11 @@ extern "C" {
 
 void ccpol3b_md_destroy(ccpol3b_md* md) {
   if (!md) return;
+  if (md->graph) ccpol3b_graph_destroy(md->graph);
   MdDev& M = md->M;
   md_free(M.com); md_free(M.q); md_free(M.vel); md_free(M.jb); md_free(M.pc); md_free(M.site);
   md_free(M.fsite); md_free(M.fc2); md_free(M.fc3); md_free(M.en); md_free(M.ke); md_free(M.rec);
+  md_free(M.failed); md_free(md->saved);
   delete md;
 }
 
@@ -278,18 +392,19 @@ int ccpol3b_md_create(ccpol3b_ctx* ctx, int nm, double box, int ns, const double
   M.ns = ns;
   M.ld = ns;
   M.box = box;
-  M.totm = mass;
+  M.mass = mass;
   for (int a = 0; a < 3; ++a) M.pmi[a] = pmi[a];
   const size_t n3 = (size_t)3 * nm, nsite = (size_t)3 * nm * M.ld;
   if (md_alloc(M.com, n3) || md_alloc(M.q, (size_t)4 * nm) || md_alloc(M.vel, n3) || md_alloc(M.jb, n3) ||
       md_alloc(M.pc, (size_t)3 * ns) || md_alloc(M.site, nsite) || md_alloc(M.fsite, nsite) ||
       md_alloc(M.fc2, n3) || md_alloc(M.fc3, n3) || md_alloc(M.en, 3) || md_alloc(M.ke, (size_t)2 * nm) ||
+      md_alloc(M.failed, 1) || md_alloc(md->saved, (size_t)13 * nm) ||
       hipMemcpy(M.pc, pcoord, sizeof(double) * 3 * ns, hipMemcpyHostToDevice) != hipSuccess) {
     ccpol3b_md_destroy(md);
     return CCPOL3B_ENOMEM;
   }
-  // an inactive term leaves its COM force untouched (as the reference does), and
-  // the kick always adds both: they start at zero
+  // an inactive term leaves its COM force untouched (as the reference does),
+  // and the kicks always add both: they start at zero
   if (hipMemset(M.fc2, 0, sizeof(double) * n3) != hipSuccess ||
       hipMemset(M.fc3, 0, sizeof(double) * n3) != hipSuccess) {
     ccpol3b_md_destroy(md);
@@ -345,6 +460,11 @@ int ccpol3b_md_get_state(ccpol3b_md* md, double* com, double* quat, double* vel,
   return CCPOL3B_OK;
 }
 
+// nsteps steps of dt with no host synchronisation inside the loop: the force
+// evaluation is a HIP graph replay.  The run ends with one wait; if a replay
+// outgrew the recorded list capacities, the state was frozen from that step
+// on, so the run is repeated from its start with the lists recorded again
+// (more headroom).
 int ccpol3b_md_run(ccpol3b_md* md, int nsteps, double dt, double* records) {
   if (!md || nsteps < 0 || !(dt > 0.0)) return CCPOL3B_EINVAL;
   MdDev& M = md->M;
@@ -355,20 +475,43 @@ int ccpol3b_md_run(ccpol3b_md* md, int nsteps, double dt, double* records) {
     if ((rc = md_alloc(M.rec, (size_t)4 * nsteps))) return rc;
     md->cap_steps = nsteps;
   }
-  if (!md->forces_current && (rc = md_forces(md))) return rc;
-  const dim3 g((M.nm + 255) / 256), b(256);
-  for (int s = 0; s < nsteps; ++s) {
-    hipLaunchKernelGGL(k_md_kick, g, b, 0, nullptr, M, 1);
-    if ((rc = md_forces(md))) return rc;
-    hipLaunchKernelGGL(k_md_kick, g, b, 0, nullptr, M, 2);
-    hipLaunchKernelGGL(k_md_record, dim3(1), dim3(256), 0, nullptr, M, s);
-    if (hipGetLastError() != hipSuccess) return CCPOL3B_EHIP;
+  const long version = context_version(md->ctx);
+  if ((rc = copy_state(md, true))) return rc;
+  const dim3 grid((M.nm + 255) / 256), block(256);
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    if (hipMemsetAsync(M.failed, 0, sizeof(int), nullptr) != hipSuccess) return CCPOL3B_EHIP;
+    if (!md->graph || md->graph_version != version) {
+      // recording evaluates the forces at the current state
+      if ((rc = launch_sites(M)) || (rc = record_forces(md, version))) return rc;
+    } else if (!md->forces_current) {
+      if ((rc = launch_sites(M)) || (rc = ccpol3b_graph_launch(md->graph, nullptr))) return rc;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+      hipLaunchKernelGGL(k_md_stage, grid, block, 0, nullptr, M, (int)STAGE_FIRST);
+      if ((rc = launch_sites(M)) || (rc = ccpol3b_graph_launch(md->graph, nullptr))) return rc;
+      hipLaunchKernelGGL(k_md_stage, grid, block, 0, nullptr, M, (int)STAGE_SECOND);
+      hipLaunchKernelGGL(k_md_record, dim3(1), dim3(256), 0, nullptr, M, s);
+      if (hipGetLastError() != hipSuccess) return CCPOL3B_EHIP;
+    }
+    int failed = 0;
+    if (hipMemcpy(&failed, M.failed, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return CCPOL3B_EHIP;
+    // the last replay's overflow (no later stage kernel read it)
+    if (!failed && ccpol3b_graph_status(md->graph) != CCPOL3B_OK) failed = 1;
+    if (!failed) {
+      md->forces_current = true;
+      if (records && nsteps > 0 &&
+          hipMemcpy(records, M.rec, sizeof(double) * 4 * nsteps, hipMemcpyDeviceToHost) != hipSuccess)
+        return CCPOL3B_EHIP;
+      return CCPOL3B_OK;
+    }
+    // repeat from the start with larger lists
+    if ((rc = copy_state(md, false))) return rc;
+    md->headroom *= 2.0;
+    ccpol3b_graph_destroy(md->graph);
+    md->graph = nullptr;
+    md->forces_current = false;
   }
-  if (records && nsteps > 0 &&
-      hipMemcpy(records, M.rec, sizeof(double) * 4 * nsteps, hipMemcpyDeviceToHost) != hipSuccess)
-    return CCPOL3B_EHIP;
-  if (hipDeviceSynchronize() != hipSuccess) return CCPOL3B_EHIP;
-  return CCPOL3B_OK;
+  return CCPOL3B_ECAPACITY;
 }
 
 }  // extern "C"
