@@ -241,7 +241,7 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
         (rc = dev_alloc(c, D.cnt3s, n)) || (rc = dev_alloc(c, D.off3s, n + 1)) ||
         (rc = dev_alloc(c, D.cnt2s, nn)) || (rc = dev_alloc(c, D.off2s, nn + 1)) ||
         (rc = dev_alloc(c, D.s2pair, 2 * nn)) || (rc = dev_alloc(c, D.ew2, (size_t)n * FS2_PARTS_MAX * 2)) ||
-        (rc = dev_alloc(c, D.acc, (size_t)n * ACC_W)))
+        (rc = dev_alloc(c, D.acc, (size_t)n * ACC_W * 2)))
       return rc;
     D.cap_sl2 = D.cap_pairs2 = (long)nn;
     c->cap_nm = n;
@@ -416,6 +416,10 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
 #undef EVREC
 }
 
+const char* const FXOVER_MSG =
+    "a force contribution of 64 hartree/bohr or more, or not a number: the geometry is outside the range of the "
+    "fixed-point force sums (overlapping sites?)";
+
 // run() until no list outgrew its capacity (at most twice), waiting for the
 // stream at the end of each attempt.
 int run_checked(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double* y, const double* z,
@@ -428,6 +432,7 @@ int run_checked(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(st));
     const int* tot = c->h_totals + T_COUNT;
+    if (tot[T_FXOVER]) return fail(c, CCPOL3B_EINVAL, FXOVER_MSG);
     if (tot[T_NBOVER] > 0) {
       // a neighbour list overflowed: grow its capacity (all per-molecule arrays)
       if (tot[T_NBOVER] > 65536) return fail(c, CCPOL3B_ECAPACITY, "neighbour list capacity exceeded");
@@ -1164,6 +1169,7 @@ extern "C" {
 int ccpol3b_graph_status(const ccpol3b_graph* g) {
   if (!g || !g->c) return CCPOL3B_EINVAL;
   const int* tot = g->c->h_totals + T_COUNT;
+  if (tot[T_FXOVER]) return CCPOL3B_EINVAL;
   if (tot[T_NBOVER] > 0 || tot[T_N3] > g->c->D.cap3) return CCPOL3B_ECAPACITY;
   return CCPOL3B_OK;
 }
@@ -1220,7 +1226,7 @@ size_t noalloc_carve(unsigned char* base, Dev& D) {
   take(D.cnt2s, nn); take(D.off2s, nn + 1);
   take(D.tri, (size_t)NA_CAP3); take(D.geo3, (size_t)NA_CAP3);
   take(D.pcnt2, n); take(D.poff2, n + 1);
-  take(D.acc, n * ACC_W); take(D.ew3, 2 * (size_t)NA_CAP3); take(D.ew2, n * FS2_PARTS_MAX * 2);
+  take(D.acc, n * ACC_W * 2); take(D.ew3, 2 * (size_t)NA_CAP3); take(D.ew2, n * FS2_PARTS_MAX * 2);
   take(D.box_t3, nbox + 1); take(D.box_t2, nbox + 1); take(D.box_c3, nbox + 1); take(D.box_c2, nbox + 1);
   take(D.red3, 2 * (size_t)nred); take(D.red2, 2 * (size_t)nred);
   take(D.s2pair, 2 * nn);
@@ -1280,6 +1286,7 @@ extern "C" int ccpol3b_eval_device_noalloc(ccpol3b_ctx* c, int nm, int ld, const
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(st));
   const int* tot = c->h_totals + T_COUNT;
+  if (tot[T_FXOVER]) return fail(c, CCPOL3B_EINVAL, FXOVER_MSG);
   if (tot[T_NBOVER] > 0)
     return fail(c, CCPOL3B_ECAPACITY, "a molecule has more than 128 neighbours (allocation-free entry point)");
   if (tot[T_N3] > D.cap3)

@@ -30,6 +30,7 @@ constexpr int T_SL2 = 8;     // FS2 sub-lists (k; i)
 constexpr int T_NI2L = 9;    // FS2 items of an exported list
 constexpr int T_NCH3 = 10;   // FS3 / FS2 energy reduction chunks
 constexpr int T_NCH2 = 11;
+constexpr int T_FXOVER = 12; // a force contribution fell outside the fixed-point range
 constexpr int T_COUNT = 16;
 constexpr int FS2_PARTS_MAX = 8; // spectator-pass blocks per molecule, at most
 constexpr int RED_CHUNK = 2048;  // entries per block in the first level of the energy reduction
@@ -102,7 +103,7 @@ struct Dev {
   int *pcnt2, *poff2;         // [nm], [nm+1]
   int4 *items2;               // [cap_items2]
   // results (hartree, bohr)
-  double *acc;                // [nm][ACC_W] force accumulators
+  unsigned long long *acc;    // [nm][ACC_W][2] force accumulators, 128-bit fixed point (devcommon.h)
   double *ew3;                // [cap3][2] per-triplet energy, virial
   double *ew2;                // [nm][fs2_parts][2] FS2 energy, virial per spectator-pass block
   int *box_t3, *box_t2;       // [nbox+1] triplet / FS2 energy-entry ranges of each box

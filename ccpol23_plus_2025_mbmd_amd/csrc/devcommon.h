@@ -26,10 +26,62 @@ __device__ __forceinline__ double sq3_nc(double x, double y, double z) {
   return x * x + y * y + z * z;
 }
 
-// A list outgrew its capacity in this call: the caller's outputs are not
-// written and the call is repeated with larger capacities.
+// A list outgrew its capacity in this call (the call is repeated with larger
+// capacities), or a force fell outside the fixed-point range (the call fails):
+// either way the caller's outputs are not written.
 __device__ __forceinline__ bool lists_overflowed(const Dev& D) {
-  return D.totals[T_NBOVER] > 0 || D.totals[T_N3] > D.cap3;
+  return D.totals[T_NBOVER] > 0 || D.totals[T_N3] > D.cap3 || D.totals[T_FXOVER] != 0;
+}
+
+// Forces are summed in 128-bit fixed point, in units of 2^-96 hartree/bohr,
+// each accumulator a pair of 64-bit words (lo, hi) with value
+// hi * 2^-32 + lo * 2^-96.  A contribution is added to lo with one integer
+// atomic, and hi receives its own part plus the carry out of lo.  Integer
+// addition is associative and the number of carries out of lo depends only on
+// the sum, not on the order of the additions, so every sum is exact and the
+// same bits come out whatever order blocks add in: every run, and a box
+// evaluated alone or inside a batch.  Contributions are rounded down to
+// 2^-96 hartree/bohr (1.3e-29) and must stay below 2^30 in magnitude; a NaN
+// or an infinite contribution (overlapping sites) flags the call instead.
+constexpr double FX_LIMIT = 1073741824.0;  // 2^30
+
+struct Fx128 {
+  unsigned long long lo;
+  unsigned long long hi;  // two's complement
+};
+
+__device__ __forceinline__ Fx128 fx_split(double v) {
+  const double a = v * 4294967296.0;  // in units of 2^-32
+  const double h = floor(a);
+  Fx128 r;
+  r.hi = (unsigned long long)(long long)h;
+  r.lo = (unsigned long long)((a - h) * 18446744073709551616.0);  // the fraction in units of 2^-96
+  return r;
+}
+
+// w[0..1] += x (global or LDS words), exactly.
+__device__ __forceinline__ void fx_atomic_add(unsigned long long* w, Fx128 x) {
+  const unsigned long long old = atomicAdd(w, x.lo);
+  const unsigned long long carry = old + x.lo < old ? 1ull : 0ull;
+  atomicAdd(w + 1, x.hi + carry);
+}
+
+__device__ __forceinline__ double fx_decode(const unsigned long long* w) {
+  return (double)(long long)w[1] * (1.0 / 4294967296.0) + (double)w[0] * 0x1p-96;
+}
+
+// v in fixed point; a NaN or out-of-range v flags the call and counts as zero.
+__device__ __forceinline__ Fx128 fx_checked(const Dev& D, double v) {
+  if (!(fabs(v) < FX_LIMIT)) {
+    atomicOr(D.totals + T_FXOVER, 1);
+    return Fx128{0ull, 0ull};
+  }
+  return fx_split(v);
+}
+
+// accumulator idx += v (hartree/bohr).
+__device__ __forceinline__ void fx_add(const Dev& D, long idx, double v) {
+  fx_atomic_add(D.acc + 2 * idx, fx_checked(D, v));
 }
 
 // owner of flat index s in an exclusive-offset array off[0..n]

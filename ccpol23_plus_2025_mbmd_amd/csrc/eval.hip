@@ -396,14 +396,14 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     if (role == 0) F = g3 * (uij + uik);
     else if (role == 1) F = g3 * (-uij + ujk);
     else F = g3 * (-uik - ujk);
-    unsafeAtomicAdd(D.acc + (long)g.m[role] * ACC_W + ACC_C3 + c, F);
+    fx_add(D, (long)g.m[role] * ACC_W + ACC_C3 + c, F);
   }
   __syncthreads();
-  // one FP64 atomic per site-force component, consecutive lanes on consecutive
-  // accumulator entries (51 per molecule)
+  // one fixed-point atomic per site-force component, consecutive lanes on
+  // consecutive accumulator entries (51 per molecule)
   if (tid < 3 * NS3 * 3) {
     const int role = tid / (NS3 * 3);
-    unsafeAtomicAdd(D.acc + (long)g.m[role] * ACC_W + (tid - role * NS3 * 3), sF[tid]);
+    fx_add(D, (long)g.m[role] * ACC_W + (tid - role * NS3 * 3), sF[tid]);
   }
   if (tid == 0) {
     D.ew3[2 * t] = Et;
@@ -419,7 +419,7 @@ __device__ __forceinline__ void finish_one(const Dev& D, long t, int ld, int nsi
   if (t >= (long)D.nm * ACC_W) return;
   const int m = (int)(t / ACC_W), a = (int)(t - (long)m * ACC_W);
   const double conv = H2KCAL * 418.4 / BOHR2A;
-  const double v = D.acc[t] * conv;
+  const double v = fx_decode(D.acc + 2 * t) * conv;
   if (a < NS3 * 3) {
     const int s = a / 3, c = a - s * 3;
     if (s < nsite) {

@@ -114,10 +114,10 @@ constexpr int FS2_SQCAP = 2048;
 size_t fs2_lds_bytes(int cap_nb) {
   return (size_t)cap_nb * (6 * sizeof(double) + 4 * sizeof(int)) + FS2_SQCAP * sizeof(int) + 16;
 }
-// Dynamic LDS of the pair pass: 6 doubles and 1 int per neighbour slot, 5 doubles
-// per pair slot, the queue.
+// Dynamic LDS of the pair pass: 6 doubles and 1 int per neighbour slot; per pair
+// slot S_ij and 8 fixed-point words (f f T, the COM force on j); the queue.
 size_t fs2_pair_lds_bytes(int cap_nb) {
-  return (size_t)cap_nb * (11 * sizeof(double) + sizeof(int)) + FS2_QCAP * sizeof(int) + 16;
+  return (size_t)cap_nb * (15 * sizeof(double) + sizeof(int)) + FS2_QCAP * sizeof(int) + 16;
 }
 
 // ---------------------------------------------------------------- pair pass
@@ -131,7 +131,7 @@ size_t fs2_pair_lds_bytes(int cap_nb) {
 __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
   extern __shared__ double dsm[];
   __shared__ int wcnt[4];
-  __shared__ double sFsi[9];
+  __shared__ unsigned long long sFsi[9][2];  // exchange-site forces on i, fixed point
   __shared__ double wred[4][3];
   const int parts = D.fs2_pair_parts;
   const int i = blockIdx.x / parts, part = blockIdx.x - i * parts;
@@ -150,9 +150,12 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
   double* sF = sR + cap;   // f(R_ki)
   double* sFp = sF + cap;  // f'(R_ki)
   double* sS = sFp + cap;  // S_ij of pair slot ql = q - q0
-  double* sWT = sS + cap;  // sum over accepted k of f(R_ki) f(R_kj) T_k
-  double* sFj = sWT + cap; // [cap][3] COM force on j
-  int* snb = reinterpret_cast<int*>(sFj + 3 * cap);
+  // per pair slot, summed over its accepted spectators k in fixed point (so
+  // the order of the LDS atomics does not matter): f(R_ki) f(R_kj) T_k, and
+  // the COM force on j
+  unsigned long long* sWT = reinterpret_cast<unsigned long long*>(sS + cap);  // [cap][2]
+  unsigned long long* sFj = sWT + 2 * cap;                                     // [cap][3][2]
+  int* snb = reinterpret_cast<int*>(sFj + 6 * cap);
   int* queue = snb + cap;
   const int* nb = D.nb2 + (long)i * cap;
   const double* vx = D.nbv2 + (long)i * 3 * cap;
@@ -169,7 +172,7 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
     sR[a] = R;
     fs2_damp(R, K, sF[a], sFp[a]);
   }
-  if (tid < 9) sFsi[tid] = 0.0;
+  if (tid < 9) sFsi[tid][0] = sFsi[tid][1] = 0ull;
   __syncthreads();
   // pair slots: S_ij and dS/dlambda (nine O,H,H site pairs, summed in order)
   for (int ql = tid; ql < npb; ql += 256) {
@@ -187,8 +190,8 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
         Sl += -BETA3 * r * e;
       }
     sS[ql] = S;
-    sWT[ql] = 0.0;
-    sFj[3 * ql] = sFj[3 * ql + 1] = sFj[3 * ql + 2] = 0.0;
+    for (int w = 0; w < 2; ++w) sWT[2 * ql + w] = 0ull;
+    for (int w = 0; w < 6; ++w) sFj[6 * ql + w] = 0ull;
     const long p = (long)D.poff2[i] + (q - up);
     D.s2pair[2 * p] = S;
     D.s2pair[2 * p + 1] = Sl;
@@ -241,9 +244,9 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
       const double ci = S * T * fpki * fkj * rcp_nr(G.Rki), cj = S * T * fki * fpkj * iRkj;
       for (int c = 0; c < 3; ++c) {
         Fi[c] += -ffS * dTdRi[c] + ci * G.xki[c];
-        atomicAdd(sFj + 3 * ql + c, -ffS * dTdRj[c] + cj * G.xkj[c]);
+        fx_atomic_add(sFj + 6 * ql + 2 * c, fx_checked(D, -ffS * dTdRj[c] + cj * G.xkj[c]));
       }
-      atomicAdd(sWT + ql, ff * T);
+      fx_atomic_add(sWT + 2 * ql, fx_checked(D, ff * T));
     }
     qn = 0;
     __syncthreads();  // the queue is refilled
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
     const int q = q0 + ql, j = snb[q];
     const double* sj = D.site + (long)j * NS3 * 3;
     const double xij[3] = {-sx[q], -sy[q], -sz[q]};
-    const double WTb = sWT[ql] * BETA3;
+    const double WTb = fx_decode(sWT + 2 * ql) * BETA3;
     const int own = role < 3 ? role : role - 3;
     double f[3] = {0.0, 0.0, 0.0};
     for (int o = 0; o < 3; ++o) {
@@ -269,11 +272,12 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
       for (int c = 0; c < 3; ++c) f[c] += g * v[c];
     }
     if (role < 3) {
-      double* accj = D.acc + (long)j * ACC_W;
-      for (int c = 0; c < 3; ++c) unsafeAtomicAdd(accj + own * 3 + c, -f[c]);
-      unsafeAtomicAdd(accj + ACC_C2 + own, sFj[3 * ql + own]);
+      const long accj = (long)j * ACC_W;
+      for (int c = 0; c < 3; ++c) fx_add(D, accj + own * 3 + c, -f[c]);
+      const unsigned long long* fj = sFj + 6 * ql + 2 * own;  // already fixed point
+      fx_atomic_add(D.acc + 2 * (accj + ACC_C2 + own), Fx128{fj[0], fj[1]});
     } else {
-      for (int c = 0; c < 3; ++c) atomicAdd(sFsi + own * 3 + c, f[c]);
+      for (int c = 0; c < 3; ++c) fx_atomic_add(sFsi[own * 3 + c], fx_checked(D, f[c]));
     }
   }
   for (int c = 0; c < 3; ++c) {
@@ -281,11 +285,11 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
     if (lane == 0) wred[w][c] = v;
   }
   __syncthreads();
-  double* acci = D.acc + (long)i * ACC_W;
-  if (tid < 9) unsafeAtomicAdd(acci + tid, sFsi[tid]);
+  const long acci = (long)i * ACC_W;
+  if (tid < 9) fx_atomic_add(D.acc + 2 * (acci + tid), Fx128{sFsi[tid][0], sFsi[tid][1]});
   else if (tid < 12) {
     const int c = tid - 9;
-    unsafeAtomicAdd(acci + ACC_C2 + c, (wred[0][c] + wred[1][c]) + (wred[2][c] + wred[3][c]));
+    fx_add(D, acci + ACC_C2 + c, (wred[0][c] + wred[1][c]) + (wred[2][c] + wred[3][c]));
   }
 }
 
@@ -475,10 +479,10 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   __syncthreads();
   if (tid < NV) {
     const double v = (wred[0][tid] + wred[1][tid]) + (wred[2][tid] + wred[3][tid]);
-    double* acck = D.acc + (long)k * ACC_W;
+    const long acck = (long)k * ACC_W;
     const long e = (long)k * parts + part;
-    if (tid < NS2 * 3) unsafeAtomicAdd(acck + tid, v);
-    else if (tid < NS2 * 3 + 3) unsafeAtomicAdd(acck + ACC_C2 + (tid - NS2 * 3), v);
+    if (tid < NS2 * 3) fx_add(D, acck + tid, v);
+    else if (tid < NS2 * 3 + 3) fx_add(D, acck + ACC_C2 + (tid - NS2 * 3), v);
     else if (tid == NS2 * 3 + 3) D.ew2[2 * e] = v;
     else if (tid == NS2 * 3 + 4) D.ew2[2 * e + 1] = v;
     else if (v > 0.0) atomicAdd(D.totals + T_NI2, (int)v);

@@ -55,7 +55,7 @@ __global__ void k_prep(int nm, int ld, const double* __restrict__ x, const doubl
     dst[1] = ys[src] / BOHR2A;
     dst[2] = zs[src] / BOHR2A;
   }
-  if (t < nm * ACC_W) D.acc[t] = 0.0;
+  if (t < nm * ACC_W) D.acc[2 * t] = D.acc[2 * t + 1] = 0ull;
   if (t < T_COUNT) D.totals[t] = 0;
 }
 

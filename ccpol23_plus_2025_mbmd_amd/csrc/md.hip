@@ -132,7 +132,7 @@ struct MdDev {
   double* en;     // [3] qfs2, qfs3, vir increment
   double* ke;     // [2][nm] translational, rotational kinetic energy per molecule
   double* rec;    // [nsteps][4] E3, Kt, Kr, E3 + Kt + Kr (kcal/mol)
-  int* failed;    // set when a force evaluation outgrew its list capacities
+  int* failed;    // set when a force evaluation wrote nothing (FAIL_LISTS, FAIL_FORCE)
   const int* ftot;  // the force evaluation's counters (see graph_device_totals)
   long fcap3;
 };
@@ -191,8 +191,12 @@ __device__ __forceinline__ void wrench(const MdDev& M, int m, const double* q, d
   for (int a = 0; a < 3; ++a) tb[a] = R[a] * tl[0] + R[3 + a] * tl[1] + R[6 + a] * tl[2];  // R^T tau
 }
 
-__device__ __forceinline__ bool evaluation_overflowed(const MdDev& M) {
-  return M.ftot[T_NBOVER] > 0 || M.ftot[T_N3] > M.fcap3;
+// Why the last force evaluation wrote nothing: 0 it did, FAIL_LISTS a list
+// outgrew its capacity, FAIL_FORCE a force left the fixed-point range.
+enum { FAIL_LISTS = 1, FAIL_FORCE = 2 };
+__device__ __forceinline__ int evaluation_failure(const MdDev& M) {
+  if (M.ftot[T_FXOVER]) return FAIL_FORCE;
+  return (M.ftot[T_NBOVER] > 0 || M.ftot[T_N3] > M.fcap3) ? FAIL_LISTS : 0;
 }
 
 enum { STAGE_FIRST = 1, STAGE_SECOND = 2 };
@@ -203,8 +207,8 @@ enum { STAGE_FIRST = 1, STAGE_SECOND = 2 };
 __global__ void k_md_stage(MdDev M, int stage) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M.nm || *M.failed) return;
-  if (evaluation_overflowed(M)) {
-    if (m == 0) *M.failed = 1;
+  if (const int why = evaluation_failure(M)) {
+    if (m == 0) *M.failed = why;
     return;
   }
   Body b = load_body(M, m);
@@ -495,8 +499,10 @@ int ccpol3b_md_run(ccpol3b_md* md, int nsteps, double dt, double* records) {
     }
     int failed = 0;
     if (hipMemcpy(&failed, M.failed, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return CCPOL3B_EHIP;
-    // the last replay's overflow (no later stage kernel read it)
-    if (!failed && ccpol3b_graph_status(md->graph) != CCPOL3B_OK) failed = 1;
+    // the last replay's outcome (no later stage kernel read it)
+    const int last = ccpol3b_graph_status(md->graph);
+    if (failed == FAIL_FORCE || last == CCPOL3B_EINVAL) return CCPOL3B_EINVAL;  // larger lists do not help
+    if (!failed && last != CCPOL3B_OK) failed = FAIL_LISTS;
     if (!failed) {
       md->forces_current = true;
       if (records && nsteps > 0 &&

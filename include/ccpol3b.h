@@ -19,6 +19,10 @@
  *   - box <= 0 selects an open boundary (isolated clusters).
  * Every entry point returns 0 on success and a negative CCPOL3B_E* code on
  * failure; on failure the caller's output arrays are left untouched.
+ * Forces are summed in 64-bit fixed point (2^-56 hartree/bohr), so they are
+ * the same bits run to run and whatever the batch a box is evaluated in; a
+ * force contribution of 64 hartree/bohr or more (overlapping sites) makes the
+ * call fail with CCPOL3B_EINVAL instead.
  * One context is used by one host thread at a time.
  */
 #ifndef CCPOL3B_H
@@ -201,7 +205,8 @@ int ccpol3b_autotune(ccpol3b_ctx *ctx, int nbox, const int *mol_start, const dou
  * `headroom` (e.g. 0.25 = 25 %) and records the launch sequence.  A replay
  * never waits for the host; after waiting for the stream the caller reads
  * ccpol3b_graph_status, which is CCPOL3B_ECAPACITY when the geometry outgrew
- * the recorded capacities (that replay wrote none of the outputs).  Outputs
+ * the recorded capacities, or CCPOL3B_EINVAL when a force left the
+ * fixed-point range (that replay wrote none of the outputs).  Outputs
  * follow ccpol3b_eval_batch_device (site forces accumulated); d_phi may be NULL. */
 typedef struct ccpol3b_graph ccpol3b_graph;
 int ccpol3b_graph_create(ccpol3b_ctx *ctx, int nbox, const int *mol_start, const double *box_len, int ld_site,

@@ -118,7 +118,7 @@ def golden_tb():
     return get
 
 
-TOLS = {"fp64": dict(e_tol=1e-10, f_tol=1e-9, com_tol=1e-7),
+TOLS = {"fp64": dict(e_tol=1e-10, f_tol=1e-9, com_tol=1e-9),
         # mixed: 1e-5 holds for the site forces and the full force output; the
         # energies and the COM forces derived from them (f_d' E + f_d w' E) do
         # not meet it -- the short-range triplet energies cancel ~1e4-fold, so

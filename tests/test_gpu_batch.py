@@ -40,9 +40,10 @@ def test_graph_replays_match_eager(params_fs2fs3, batch, precision):
         g.launch()
         torch.cuda.synchronize()
         assert g.status() == 0
+        # fixed-order energy sums and fixed-point force sums: the same bits
         assert torch.equal(g.energies, en), rep
-        assert torch.allclose(g.fsite, fs, rtol=1e-12, atol=1e-12 * fs.abs().max().item()), rep
-        assert torch.allclose(g.fcom3, c3, rtol=1e-12, atol=1e-12 * c3.abs().max().item()), rep
+        assert torch.equal(g.fsite, fs), rep
+        assert torch.equal(g.fcom2, c2) and torch.equal(g.fcom3, c3), rep
         # eager work between replays
         tb.eval_batch_device(ms, bl, com, site)
         torch.cuda.synchronize()
@@ -88,7 +89,8 @@ def test_autotune_keeps_results(params_fs2fs3, batch):
     fs1, _, _, en1 = tb.eval_batch_device(ms, bl, com, site)
     assert torch.allclose(en1, en0, rtol=1e-12, atol=1e-12)
     assert torch.allclose(fs1, fs0, rtol=1e-10, atol=1e-10 * fs0.abs().max().item())
-    # every launch shape gives the same answer
+    # every launch shape gives the same answer (to round-off: the FS2 block
+    # split decides how a spectator's items are grouped before they are summed)
     for shape in ({"fs3_lds_coef": 1, "fs2_parts": 1, "fs2_pair_parts": 1},
                   {"fs3_lds_coef": 0, "fs2_parts": 8, "fs2_pair_parts": 8}):
         tb.set_tuning(**shape)

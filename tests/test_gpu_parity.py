@@ -51,11 +51,23 @@ def all_forces(r, s2, s3):
     return np.concatenate(parts)
 
 
-# COM-force arrays alone: the FS3 COM force is f_d' E + f_d w' E over the region
-# energies, which cancel; the oracle's own FP64 result moves by up to 1.5e-8
-# relative when only the molecule order of a trimer is permuted, so these small
-# arrays are held to 1e-7 while the full force output is held to 1e-9.
-COM_TOL = 1e-7
+# COM-force arrays alone are held to 1e-9 too, except where FP64 itself cannot
+# fix them that well: the FS3 COM force is f_d' E + f_d w' E over region
+# energies that cancel up to ~1e6-fold, so for some isolated trimers the
+# oracle's own result moves by more than 1e-9 when only the order of the
+# molecules changes (tests/test_oracle_physics.py::test_fs3_com_conditioning).
+# There the bar is four times that self-variation (oracle_com_floor).
+COM_TOL = 1e-9
+
+
+def oracle_com_floor(oracle, params, box):
+    """Relative RMS change of the oracle's FS3 and FS2 COM forces when the
+    molecules of `box` are evaluated in reverse order."""
+    from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox
+    ref = oracle.threebody_term(params, box)
+    perm = np.arange(box.nm)[::-1]
+    rp = oracle.threebody_term(params, WaterBox(box.com[perm], box.quats[perm], box.box))
+    return max(rel_rms(com_forces(rp, t)[:, np.argsort(perm)], com_forces(ref, t)) for t in ("fs2", "fs3"))
 # Energies: 1e-10 relative plus an absolute floor.  The oracle's FS3 trimer
 # energy itself moves by up to 8.4e-13 kcal/mol (3.4e-10 relative for a
 # 2.4e-3 kcal/mol trimer) when only the molecule order is permuted, so the floor
@@ -63,15 +75,15 @@ COM_TOL = 1e-7
 E_FLOOR = 2e-12
 
 
-def assert_parity(got, ref, s2=True, s3=True, e_tol=1e-10, f_tol=1e-9):
+def assert_parity(got, ref, s2=True, s3=True, e_tol=1e-10, f_tol=1e-9, com_tol=COM_TOL):
     if s3:
         assert got["n_fs3"] == ref["n_fs3"]
         assert abs(got["qfs3"] - ref["qfs3"]) <= e_tol * abs(ref["qfs3"]) + E_FLOOR, (got["qfs3"], ref["qfs3"])
-        assert rel_rms(com_forces(got, "fs3"), com_forces(ref, "fs3")) <= COM_TOL
+        assert rel_rms(com_forces(got, "fs3"), com_forces(ref, "fs3")) <= com_tol
     if s2:
         assert got["n_fs2_items"] == ref["n_fs2"]
         assert abs(got["qfs2"] - ref["qfs2"]) <= e_tol * abs(ref["qfs2"]) + E_FLOOR, (got["qfs2"], ref["qfs2"])
-        assert rel_rms(com_forces(got, "fs2"), com_forces(ref, "fs2")) <= COM_TOL
+        assert rel_rms(com_forces(got, "fs2"), com_forces(ref, "fs2")) <= com_tol
     assert rel_rms(all_forces(got, s2, s3), all_forces(ref, s2, s3)) <= f_tol
     assert rel_rms(site_forces(got), site_forces(ref)) <= f_tol
     assert abs(got["vir"] - ref["vir"]) <= 10 * e_tol * abs(ref["vir"]) + 418.4 * E_FLOOR, (got["vir"], ref["vir"])
@@ -146,7 +158,7 @@ def test_trimers_all_regions(tb_fs2fs3, oracle, params_fs2fs3):
     for t in trimers:
         got = tb_fs2fs3.evaluate(t)
         ref = oracle.threebody_term(params_fs2fs3, t)
-        assert_parity(got, ref)
+        assert_parity(got, ref, com_tol=max(COM_TOL, 4 * oracle_com_floor(oracle, params_fs2fs3, t)))
         for key in ("n_fs3_sr", "n_fs3_switch", "n_fs3_lr"):
             if got[key]:
                 seen.add(key)
@@ -177,16 +189,43 @@ def test_drop_in_semantics(tb_fs2fs3, params_fs2fs3):
     assert q2 == base["qfs2"] and q3 == base["qfs3"]
 
 
-def test_repeatable(tb_fs2fs3):
-    """Energies and virial use fixed reduction trees: bitwise repeatable.  Forces
-    are scattered with FP64 atomics: repeatable to round-off."""
-    box = lattice_box(64, seed=9)
-    a = tb_fs2fs3.evaluate(box)
-    b = tb_fs2fs3.evaluate(box)
-    for k in ("qfs2", "qfs3", "vir"):
-        assert a[k] == b[k]
-    for k in ("fxs", "fys", "fzs", "fxfs2", "fxfs3"):
-        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-13 * np.abs(a[k]).max())
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_repeatable(params_fs2fs3, precision):
+    """Energies and virial use fixed reduction trees, forces fixed-point sums:
+    every output is the same bits run to run (the reference merges its forces
+    in a fixed order, threebody_potential.f90:1964-1977)."""
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    tb = ThreeBody(params_fs2fs3, precision)
+    box = lattice_box(256, seed=9)
+    a = tb.evaluate(box)
+    for _ in range(3):
+        b = tb.evaluate(box)
+        for k in ("qfs2", "qfs3", "vir"):
+            assert a[k] == b[k]
+        for k in ("fxs", "fys", "fzs", "fxfs2", "fyfs2", "fzfs2", "fxfs3", "fyfs3", "fzfs3"):
+            np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_overlapping_sites_fail_cleanly(params_fs2fs3):
+    """A spectator's oxygen exactly on a pair member's centre of mass makes the
+    FS2 exchange-charge term 1/|r_ks - R_i| infinite (the reference divides by
+    zero there).  The call fails with CCPOL3B_EINVAL, names the cause, and
+    leaves every output untouched."""
+    from ccpol23_plus_2025_mbmd_amd import _lib
+    from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    base = lattice_box(8, seed=2)
+    com = base.com.copy()
+    com[2] = com[0] - base.site[2, 0]  # O site of molecule 2 on the COM of molecule 0
+    wb = WaterBox(com, base.quats, 0.0)
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    nm, ld = wb.nm, wb.ld
+    outs = [np.full(nm * ld, 3.0) for _ in range(3)] + [np.full(nm, 5.0) for _ in range(6)]
+    keep = [o.copy() for o in outs]
+    with pytest.raises(_lib.CcpolError, match="fixed-point"):
+        tb.threebody_term(wb.x, wb.y, wb.z, wb.xs, wb.ys, wb.zs, 0.0, *outs, vir=1.0, ld=ld)
+    for o, k in zip(outs, keep):
+        np.testing.assert_array_equal(o, k)
 
 
 def test_box_description_changes_between_calls(params_fs2fs3):
@@ -281,9 +320,9 @@ def test_batch_matches_single(params_fs2fs3):
         fs1, c21, c31, en1 = tb.eval_device(cb, sb, b.box)
         torch.cuda.synchronize()
         sl = slice(starts[bi], starts[bi + 1])
-        # forces: FP64 atomics, equal to round-off; energies: fixed trees, bitwise
+        # forces: fixed-point sums; energies: fixed trees -- both bitwise
         for a, b_ in ((fs[:, sl], fs1), (c2[:, sl], c21), (c3[:, sl], c31)):
-            assert torch.allclose(a, b_, rtol=0, atol=1e-12 * float(b_.abs().max()))
+            assert torch.equal(a, b_)
         assert torch.equal(en[bi], en1)
 
 
@@ -327,7 +366,7 @@ def test_large_batch_matches_single(params_fs2fs3, terms):
         n2 += tb.stats()["n_fs2_items"]
         sl = slice(starts[bi], starts[bi + 1])
         for a, b_ in ((fs[:, sl], fs1), (c2[:, sl], c21), (c3[:, sl], c31)):
-            assert torch.allclose(a, b_, rtol=0, atol=1e-12 * float(b_.abs().max()))
+            assert torch.equal(a, b_)
         assert torch.equal(en[bi], en1)
     assert (st["n_fs3"], st["n_fs2_items"]) == (n3, n2)
     assert (n3 > 0) == (terms != "fs2") and (n2 > 0) == (terms != "fs3")

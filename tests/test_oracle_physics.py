@@ -244,3 +244,31 @@ def test_periodic_shift_box(oracle, params_fs2fs3):
     assert abs(energy(r1) - energy(r0)) <= 1e-10 * abs(energy(r0))
     np.testing.assert_allclose(molecular_forces(r1, 17), molecular_forces(r0, 17), rtol=1e-8,
                                atol=1e-10 * np.abs(molecular_forces(r0, 17)).max())
+
+
+def test_fs3_com_conditioning(oracle, params_fs2fs3):
+    """The FS3 COM force of a trimer is f_d' E + f_d w' E over region energies
+    that cancel up to ~1e6-fold, so FP64 fixes it only to ~1e-16 times that
+    cancellation.  Measured on the oracle itself: reversing the order of the
+    molecules (the same physics, another summation order) moves the FS3 COM
+    forces of the 40 trimers of the GPU parity set by more than 1e-9 relative
+    for some of them, while their site forces stay within 1e-12.  This is the
+    floor the GPU COM-force parity bar allows for (tests/test_gpu_parity.py)."""
+    from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox, trimer_set
+    worst_com, worst_site = 0.0, 0.0
+    for t in trimer_set(40, seed=17):
+        ref = oracle.threebody_term(params_fs2fs3, t)
+        perm = np.array([2, 1, 0])
+        rp = oracle.threebody_term(params_fs2fs3, WaterBox(t.com[perm], t.quats[perm], t.box))
+        inv = np.argsort(perm)
+        c = np.stack([ref[k] for k in ("fxfs3", "fyfs3", "fzfs3")])
+        cp = np.stack([rp[k] for k in ("fxfs3", "fyfs3", "fzfs3")])[:, inv]
+        if np.abs(c).max() > 0:
+            worst_com = max(worst_com, np.sqrt(((cp - c) ** 2).sum() / (c ** 2).sum()))
+        f = np.stack([ref[k][:, :17] for k in ("fxs", "fys", "fzs")])
+        fp = np.stack([rp[k][:, :17] for k in ("fxs", "fys", "fzs")])[:, inv]
+        if np.abs(f).max() > 0:
+            worst_site = max(worst_site, np.sqrt(((fp - f) ** 2).sum() / (f ** 2).sum()))
+    print(f"oracle self-variation under molecule reversal: FS3 COM {worst_com:.2e}, site {worst_site:.2e}")
+    assert worst_com > 1e-9
+    assert worst_site < 1e-11
