@@ -218,4 +218,5 @@ template <typename T> __device__ __forceinline__ T t_exp(T x);
 template <> __device__ __forceinline__ double t_exp<double>(double x) { return exp(x); }
 template <> __device__ __forceinline__ float t_exp<float>(float x) { return __expf(x); }
 
+
 }  // namespace ccpol3b

@@ -22,11 +22,181 @@
 
 namespace ccpol3b {
 
-// 2x2 site tiles of one molecule's 17 sites, aligned to the type blocks
-// (O | H H | D1 D1 | N1 N1 | N1 N1 | N2 N2 | N2 N2 | N3 N3 | N3 N3): tile b >= 1
-// holds sites 2b-1 and 2b; tile 0 holds O (its second row is padding).
-__device__ __forceinline__ int tile_first(int b) { return b == 0 ? 0 : 2 * b - 1; }
-__device__ __forceinline__ int tile_type(int b) { return b < 3 ? b : (b + 3) >> 1; }
+// First site of each type block (O | H H | D1 D1 | N1 x4 | N2 x4 | N3 x4).
+constexpr int kTypeStart[NTYPE + 1] = {0, 1, 3, 5, 9, 13, 17};
+
+// Operand layout of pass P (0: X=i,Y=j,Z=k; 1: X=i,Y=k,Z=j; 2: X=j,Y=k,Z=i) in
+// the site-pair blocks (0 ij, 1 ik, 2 jk, each [first][second] row major):
+// v_xz is A = block ablk, row of x at x*arow, next z at +az; v_yz is B = block
+// bblk, row of y at y*brow, next z at +bz.
+//   P=0: v_xz = ik[x][z], v_yz = jk[y][z]
+//   P=1: v_xz = ij[x][z], v_yz = jk[z][y]
+//   P=2: v_xz = ij[z][x], v_yz = ik[z][y]
+struct PassLayout {
+  int ablk, arow, az, bblk, brow, bz;
+};
+__host__ __device__ constexpr PassLayout pass_layout(int P) {
+  return P == 0 ? PassLayout{1, NS3, 1, 2, NS3, 1}
+       : P == 1 ? PassLayout{0, NS3, 1, 2, 1, NS3}
+                : PassLayout{0, 1, NS3, 1, 1, NS3};
+}
+
+// One tile of NU x NW site pairs of a pass: NU sites u of one molecule (rows
+// U + a URS), NW sites w of the other (rows W + b WRS), contracted over the 17
+// sites z of the third molecule (steps UZS, WZS) and the 6 type blocks tz:
+//   M[a][b][p][q] = sum_{z in tz} v_wz[p] v_uz[q],
+//   G[a][b][s]   += sum_pq C[tu][tw][tz][p][q][s] M[a][b][p][q].
+// Both loops are unrolled, so with compile-time strides every LDS read is a
+// base register plus an immediate offset.  With dc (PHI), also
+//   dE/dC[tu][tw][tz][p][q][s] += sum_ab vown[a][b][s] M[a][b][p][q].
+template <typename T, int NU, int NW, bool PHI>
+__device__ __forceinline__ void contract_tile(const typename Pair2<T>::type* U, int URS, int UZS,
+                                              const typename Pair2<T>::type* W, int WRS, int WZS,
+                                              const T* __restrict__ cp, T (&G)[NU][NW][2], T* dc,
+                                              const T (&vown)[NU][NW][2]) {
+  using T2 = typename Pair2<T>::type;
+#pragma unroll
+  for (int a = 0; a < NU; ++a)
+#pragma unroll
+    for (int b = 0; b < NW; ++b) G[a][b][0] = G[a][b][1] = T(0);
+#pragma unroll 1
+  for (int tz = 0; tz < NTYPE; ++tz) {
+    // M over the type block's sites; every block holds at least one site, so
+    // the first one initialises M
+    T M[NU][NW][2][2];
+    T uv[NU][2], wv[NW][2];
+    auto load = [&]() {
+#pragma unroll
+      for (int a = 0; a < NU; ++a) {
+        const T2 v = U[a * URS];
+        uv[a][0] = v.x;
+        uv[a][1] = v.y;
+      }
+#pragma unroll
+      for (int b = 0; b < NW; ++b) {
+        const T2 v = W[b * WRS];
+        wv[b][0] = v.x;
+        wv[b][1] = v.y;
+      }
+      U += UZS;
+      W += WZS;
+    };
+    load();
+#pragma unroll
+    for (int a = 0; a < NU; ++a)
+#pragma unroll
+      for (int b = 0; b < NW; ++b)
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) M[a][b][p][q] = wv[b][p] * uv[a][q];
+    const int nz = kTypeStart[tz + 1] - kTypeStart[tz];
+#pragma unroll 1
+    for (int z = 1; z < nz; ++z) {
+      load();
+#pragma unroll
+      for (int a = 0; a < NU; ++a)
+#pragma unroll
+        for (int b = 0; b < NW; ++b)
+#pragma unroll
+          for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) M[a][b][p][q] += wv[b][p] * uv[a][q];
+    }
+    if constexpr (PHI) {
+      if (dc) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+              T d = T(0);
+#pragma unroll
+              for (int a = 0; a < NU; ++a)
+#pragma unroll
+                for (int b = 0; b < NW; ++b) d += vown[a][b][s] * M[a][b][p][q];
+              atomicAdd(dc + tz * 8 + (p * 2 + q) * 2 + s, d);
+            }
+      }
+    }
+    T cc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cc[k] = cp[tz * 8 + k];
+#pragma unroll
+    for (int a = 0; a < NU; ++a)
+#pragma unroll
+      for (int b = 0; b < NW; ++b)
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) G[a][b][s] += cc[(p * 2 + q) * 2 + s] * M[a][b][p][q];
+  }
+}
+
+// The triple-stage work of one thread.  Waves 0-2 own passes 0-2 on the site
+// pairs of sites 1-16 (an 8 x 8 grid of 2 x 2 tiles aligned to the type
+// blocks, strides known at compile time).  Wave 3 owns the O row and the O
+// column of all three passes: lane 17 P + r, r < 8 the O row pairs (0, 2r+1)
+// and (0, 2r+2); 8 <= r < 16 the O column pairs (2r-15, 0) and (2r-14, 0),
+// evaluated with the roles of the two molecules swapped (U the O site of Y, W
+// two sites of X: C is symmetric under that swap); r = 16 the pair (0, 0).
+// own[a][b] is the site-pair index (in the pass's own block, ij / ik / jk) of
+// tile pair (a, b), -1 when unused.
+struct TileSpec {
+  int P, own[2][2];
+  int u_off, u_rs, u_zs, w_off, w_rs, w_zs, tu, tw;
+  bool used;
+};
+
+__device__ __forceinline__ TileSpec tile_spec(int tid) {
+  TileSpec t{};
+  const int wave = tid >> 6, lane = tid & 63;
+  if (wave < 3) {
+    const PassLayout L = pass_layout(wave);
+    const int x0 = 1 + 2 * (lane >> 3), y0 = 1 + 2 * (lane & 7);
+    t.P = wave;
+    t.used = true;
+    t.u_off = L.ablk * NPAIR + x0 * L.arow; t.u_rs = L.arow; t.u_zs = L.az;
+    t.w_off = L.bblk * NPAIR + y0 * L.brow; t.w_rs = L.brow; t.w_zs = L.bz;
+    t.tu = site_type(x0);
+    t.tw = site_type(y0);
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) t.own[a][b] = wave * NPAIR + (x0 + a) * NS3 + (y0 + b);
+    return t;
+  }
+  t.used = lane < 51;
+  const int P = t.used ? lane / 17 : 0, r = lane - P * 17;
+  const PassLayout L = pass_layout(P);
+  t.P = P;
+  t.own[0][0] = t.own[0][1] = t.own[1][0] = t.own[1][1] = -1;
+  if (r < 8) {  // O row: x = 0, y = 2r+1, 2r+2
+    const int y0 = 2 * r + 1;
+    t.u_off = L.ablk * NPAIR; t.u_zs = L.az;
+    t.w_off = L.bblk * NPAIR + y0 * L.brow; t.w_rs = L.brow; t.w_zs = L.bz;
+    t.tu = 0;
+    t.tw = site_type(y0);
+    t.own[0][0] = P * NPAIR + y0;
+    t.own[0][1] = P * NPAIR + y0 + 1;
+  } else if (r < 16) {  // O column: x = 2r-15, 2r-14, y = 0, roles swapped
+    const int x0 = 2 * r - 15;
+    t.u_off = L.bblk * NPAIR; t.u_zs = L.bz;
+    t.w_off = L.ablk * NPAIR + x0 * L.arow; t.w_rs = L.arow; t.w_zs = L.az;
+    t.tu = 0;
+    t.tw = site_type(x0);
+    t.own[0][0] = P * NPAIR + x0 * NS3;
+    t.own[0][1] = P * NPAIR + (x0 + 1) * NS3;
+  } else {  // (0, 0); the tile's second w row repeats the first and is unused
+    t.u_off = L.ablk * NPAIR; t.u_zs = L.az;
+    t.w_off = L.bblk * NPAIR; t.w_rs = 0; t.w_zs = L.bz;
+    t.tu = t.tw = 0;
+    t.own[0][0] = P * NPAIR;
+  }
+  t.u_rs = 0;
+  return t;
+}
 
 // ====================================================================== FS3
 // Distance and reciprocal of the pair stage: FP32 hardware square root and
@@ -54,9 +224,6 @@ int set_orbit_tables(const short* off, const short* mem) {
   if (hipMemcpyToSymbol(HIP_SYMBOL(c_orb_mem), mem, sizeof(short) * NCOEF) != hipSuccess) return -1;
   return 0;
 }
-
-// sites of each type block (O, H, D1, N1, N2, N3)
-__constant__ int c_nz[NTYPE] = {1, 2, 2, 4, 4, 4};
 
 // One 256-thread block per triplet:
 //  pair stage:   e, e r and d ln e/dr of the 867 site pairs of a region -> LDS;
@@ -109,24 +276,6 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     (&sP[0][0][0])[tid] = D.site[(long)m * NS3 * 3 + rem] - shift;
   }
 
-  // tile of this thread (threads 0..242): pass P, 2x2 site tile (x0,x0+1) x
-  // (y0,y0+1); the O tile's second row (xv1 / yv1 false) is the first H row,
-  // read and discarded.  Rows of v_xz and v_yz:
-  //   P=0 (X=i,Y=j,Z=k): v_xz = ik[x][z], v_yz = jk[y][z]
-  //   P=1 (X=i,Y=k,Z=j): v_xz = ij[x][z], v_yz = jk[z][y]
-  //   P=2 (X=j,Y=k,Z=i): v_xz = ij[z][x], v_yz = ik[z][y]
-  const bool tile = tid < 243;
-  const int P = tid / 81, trem = tid - P * 81;
-  const int xb = tile ? trem / 9 : 0, yb = tile ? trem - (trem / 9) * 9 : 0;
-  const int x0 = tile_first(xb), tx = tile_type(xb);
-  const int y0 = tile_first(yb), ty = tile_type(yb);
-  const bool xv1 = xb > 0, yv1 = yb > 0;
-  const int po00 = P * NPAIR + x0 * NS3 + y0;  // site pair (x0, y0) of block P
-  const bool acol = P == 2, bcol = P != 0;
-  const int aoff = (P == 0 ? NPAIR : 0) + (acol ? x0 : x0 * NS3);
-  const int boff = (P == 2 ? NPAIR : 2 * NPAIR) + (bcol ? y0 : y0 * NS3);
-  const int ao2 = acol ? 1 : NS3, as = acol ? NS3 : 1;
-  const int bo2 = bcol ? 1 : NS3, bs = bcol ? NS3 : 1;
   T gacc[2][2] = {{T(0), T(0)}, {T(0), T(0)}};  // fd w dE/dr of the tile's site pairs
   double eacc0 = 0.0, eacc1 = 0.0;
   for (int rr = 0; rr < nreg; ++rr) {
@@ -141,7 +290,8 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     // ---------------- pair stage: e, e*r and d ln e / dr for the 867 site pairs.
     // Thread tid takes p = tid + 256 k.  In the mixed kernels (blk, x, y) of p
     // advance incrementally (256 = 15 * 17 + 1), and the distances of the first
-    // region are reused from sR in the second.
+    // region are reused from sR in the second; the FP64 kernels divide and
+    // recompute (fewer live registers).
     {
       int blk = 0, x = tid / NS3, y = tid - (tid / NS3) * NS3;
       for (int p = tid; p < NPAIR3; p += 256) {
@@ -185,114 +335,73 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
       }
     }
     __syncthreads();
-    // ---------------- site-triple stage
-    if (tile) {
-      const T2* ap = sV + aoff;
-      const T2* bp = sV + boff;
-      const T* cp = (CSM ? sC : coef + X * NCOEF) + (tx * NTYPE + ty) * NTYPE * 8;
-      T G[2][2][2];
-#pragma unroll
-      for (int xs = 0; xs < 2; ++xs)
-#pragma unroll
-        for (int ys = 0; ys < 2; ++ys) G[xs][ys][0] = G[xs][ys][1] = T(0);
-      // PHI (pass 0 only): the pair vectors v_xy of the tile's i-j site pairs,
-      // zero for the padding row of the O tile
-      T vxy[2][2][2];
-      if constexpr (PHI) {
-#pragma unroll
-        for (int xs = 0; xs < 2; ++xs)
-#pragma unroll
-          for (int ys = 0; ys < 2; ++ys) {
-            const bool ok = P == 0 && (!xs || xv1) && (!ys || yv1);
-            const T2 v = sV[ok ? po00 + xs * NS3 + ys : 0];
-            vxy[xs][ys][0] = ok ? v.x : T(0);
-            vxy[xs][ys][1] = ok ? v.y : T(0);
-          }
-      }
-#pragma unroll 1
-      for (int tz = 0; tz < NTYPE; ++tz) {
-        // M[xs][ys][p][q] = sum_{z in type tz} v_yz[p] v_xz[q]; every type block
-        // holds at least one site, so the first term initialises M
-        T M[2][2][2][2];
-        const int nz = c_nz[tz];
-        {
-          const T2 a0 = ap[0], a1 = ap[ao2], b0 = bp[0], b1 = bp[bo2];
-          const T av[2][2] = {{a0.x, a0.y}, {a1.x, a1.y}};
-          const T bv[2][2] = {{b0.x, b0.y}, {b1.x, b1.y}};
-#pragma unroll
-          for (int xs = 0; xs < 2; ++xs)
-#pragma unroll
-            for (int ys = 0; ys < 2; ++ys)
-#pragma unroll
-              for (int p = 0; p < 2; ++p)
-#pragma unroll
-                for (int q = 0; q < 2; ++q) M[xs][ys][p][q] = bv[ys][p] * av[xs][q];
-          ap += as;
-          bp += bs;
-        }
-#pragma unroll 1
-        for (int z = 1; z < nz; ++z) {
-          const T2 a0 = ap[0], a1 = ap[ao2], b0 = bp[0], b1 = bp[bo2];
-          const T av[2][2] = {{a0.x, a0.y}, {a1.x, a1.y}};
-          const T bv[2][2] = {{b0.x, b0.y}, {b1.x, b1.y}};
-#pragma unroll
-          for (int xs = 0; xs < 2; ++xs)
-#pragma unroll
-            for (int ys = 0; ys < 2; ++ys)
-#pragma unroll
-              for (int p = 0; p < 2; ++p)
-#pragma unroll
-                for (int q = 0; q < 2; ++q) M[xs][ys][p][q] += bv[ys][p] * av[xs][q];
-          ap += as;
-          bp += bs;
-        }
-        if constexpr (PHI) {
-          // dE_X/dC[tx][ty][tz][p][q][s] = sum over the tile's (x, y) of v_xy[s] M[x,y,p,q]
-          if (P == 0) {
-            T* dc = sDC + ((tx * NTYPE + ty) * NTYPE + tz) * 8;
-#pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-              for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                  T d = T(0);
-#pragma unroll
-                  for (int xs = 0; xs < 2; ++xs)
-#pragma unroll
-                    for (int ys = 0; ys < 2; ++ys) d += vxy[xs][ys][s] * M[xs][ys][p][q];
-                  atomicAdd(dc + (p * 2 + q) * 2 + s, d);
-                }
-          }
-        }
-        T cc[8];
-#pragma unroll
-        for (int a = 0; a < 8; ++a) cc[a] = cp[tz * 8 + a];
-#pragma unroll
-        for (int xs = 0; xs < 2; ++xs)
-#pragma unroll
-          for (int ys = 0; ys < 2; ++ys)
-#pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-              for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int s = 0; s < 2; ++s) G[xs][ys][s] += cc[(p * 2 + q) * 2 + s] * M[xs][ys][p][q];
-      }
+    // ---------------- site-triple stage (the tile is rebuilt from an opaque
+    // copy of tid: cheaper than keeping it live across the region loop)
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
+    const TileSpec ts = tile_spec(tq);
+    if (ts.used) {
+      const T* cp = (CSM ? sC : coef + X * NCOEF) + (ts.tu * NTYPE + ts.tw) * NTYPE * 8;
+      T* dc = nullptr;
+      if constexpr (PHI)
+        if (ts.P == 0) dc = sDC + (ts.tu * NTYPE + ts.tw) * NTYPE * 8;
       const T fdw = T(g.fd * g.w[rr]);
+      // M, G and the gacc update of the tile; the energy is sum_xy v_xy G_xy
+      // over pass 0's tiles
+      auto finish = [&](auto& G, int nu) {
 #pragma unroll
-      for (int xs = 0; xs < 2; ++xs) {
+        for (int a = 0; a < 2; ++a) {
+          if (a >= nu) break;
 #pragma unroll
-        for (int ys = 0; ys < 2; ++ys) {
-          if ((xs && !xv1) || (ys && !yv1)) continue;
-          const int po = po00 + xs * NS3 + ys;
-          const T2 v = sV[po];
-          const T ev = v.x * G[xs][ys][0] + v.y * G[xs][ys][1];
-          gacc[xs][ys] += fdw * (sDer[po] * ev + v.x * G[xs][ys][1]);
-          if (P == 0) {
-            if (rr == 0) eacc0 += double(ev); else eacc1 += double(ev);
+          for (int b2 = 0; b2 < 2; ++b2) {
+            const int po = ts.own[a][b2];
+            if (po < 0) continue;
+            const T2 v = sV[po];
+            const T ev = v.x * G[a][b2][0] + v.y * G[a][b2][1];
+            gacc[a][b2] += fdw * (sDer[po] * ev + v.x * G[a][b2][1]);
+            if (ts.P == 0) {
+              if (rr == 0) eacc0 += double(ev); else eacc1 += double(ev);
+            }
           }
         }
+      };
+      if (tid < 192) {
+        T G[2][2][2], vown[2][2][2] = {};
+        if constexpr (PHI)
+          if (ts.P == 0)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int b2 = 0; b2 < 2; ++b2) {
+                const T2 v = sV[ts.own[a][b2]];
+                vown[a][b2][0] = v.x;
+                vown[a][b2][1] = v.y;
+              }
+        // one pass per wave, its strides as constants
+        if (ts.P == 0) {
+          constexpr PassLayout L = pass_layout(0);
+          contract_tile<T, 2, 2, PHI>(sV + ts.u_off, L.arow, L.az, sV + ts.w_off, L.brow, L.bz, cp, G, dc, vown);
+        } else if (ts.P == 1) {
+          constexpr PassLayout L = pass_layout(1);
+          contract_tile<T, 2, 2, PHI>(sV + ts.u_off, L.arow, L.az, sV + ts.w_off, L.brow, L.bz, cp, G, dc, vown);
+        } else {
+          constexpr PassLayout L = pass_layout(2);
+          contract_tile<T, 2, 2, PHI>(sV + ts.u_off, L.arow, L.az, sV + ts.w_off, L.brow, L.bz, cp, G, dc, vown);
+        }
+        finish(G, 2);
+      } else {
+        T G[1][2][2], vown[1][2][2] = {};
+        if constexpr (PHI)
+          if (ts.P == 0)
+#pragma unroll
+            for (int b2 = 0; b2 < 2; ++b2) {
+              const int po = ts.own[0][b2];
+              const T2 v = sV[po < 0 ? 0 : po];
+              vown[0][b2][0] = po < 0 ? T(0) : v.x;
+              vown[0][b2][1] = po < 0 ? T(0) : v.y;
+            }
+        contract_tile<T, 1, 2, PHI>(sV + ts.u_off, 0, ts.u_zs, sV + ts.w_off, ts.w_rs, ts.w_zs, cp, G, dc, vown);
+        finish(G, 1);
       }
     }
     __syncthreads();
@@ -321,25 +430,21 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     }
   }
   // the tile owner turns dE/dr into h = (dE/dr)/r and the site virial sum r dE/dr
-  // (the tile's sites are recomputed from an opaque copy of tid: cheaper than
-  // keeping them live across the region loop)
   double wsum = 0.0;
-  if (tile) {
-    int tq = tid;
-    asm volatile("" : "+v"(tq));
-    const int Pq = tq / 81, rq = tq - Pq * 81, xq = rq / 9, yq = rq - (rq / 9) * 9;
-    const int pq00 = Pq * NPAIR + tile_first(xq) * NS3 + tile_first(yq);
+  int tq = tid;
+  asm volatile("" : "+v"(tq));
+  const TileSpec ts = tile_spec(tq);
+  if (ts.used) {
 #pragma unroll
-    for (int xs = 0; xs < 2; ++xs) {
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int ys = 0; ys < 2; ++ys) {
-        if ((xs && xq == 0) || (ys && yq == 0)) continue;
-        const int po = pq00 + xs * NS3 + ys;
+      for (int b2 = 0; b2 < 2; ++b2) {
+        const int po = ts.own[a][b2];
+        if (po < 0) continue;
         const T r = sR[po];
-        wsum += double(gacc[xs][ys]) * double(r);
-        sR[po] = gacc[xs][ys] * t_rcp<T>(r);
+        wsum += double(gacc[a][b2]) * double(r);
+        sR[po] = gacc[a][b2] * t_rcp<T>(r);
       }
-    }
   }
   // region energies and the site virial (FP64, fixed reduction trees); the
   // first reduction's barrier also publishes h
