@@ -36,7 +36,7 @@ class Stats(C.Structure):
         ("n_fs2_items", C.c_long), ("n_fs2_pairs", C.c_long),
         ("ms_total", C.c_double), ("ms_lists", C.c_double), ("ms_fs2", C.c_double), ("ms_fs3", C.c_double),
         ("ms_reduce", C.c_double), ("ms_h2d", C.c_double), ("ms_d2h", C.c_double),
-        ("ms_fs3_eval", C.c_double),
+        ("ms_fs3_eval", C.c_double), ("n_cells", C.c_long),
     ]
 
     def as_dict(self):
@@ -44,7 +44,8 @@ class Stats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("fs3_lds_coef", C.c_int), ("fs2_parts", C.c_int), ("fs2_pair_parts", C.c_int)]
+    _fields_ = [("fs3_lds_coef", C.c_int), ("fs2_parts", C.c_int), ("fs2_pair_parts", C.c_int),
+                ("neighbour_cells", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -21,6 +21,8 @@ namespace ccpol3b {
 __global__ void k_prep(int, int, const double*, const double*, const double*, const double*,
                        const double*, const double*, Dev);
 __global__ void k_neighbours(Dev, Consts, int);
+__global__ void k_cells_assign(Dev);
+template <int NT> __global__ void k_cells_build(Dev);
 __global__ void k_fs3_fill(Dev, Consts);
 __global__ void k_fs2_items(Dev, Consts, int);
 template <int NT> __global__ void k_scan(Dev, int, int, int);
@@ -135,7 +137,7 @@ struct ccpol3b_ctx {
   float* d_coef32s = nullptr;    // [2][NCOEF] FP32, shifted pair-vector basis of the mixed kernels
   double* d_phitrip = nullptr;   // [cap_phi][728] per-triplet coefficient-gradient rows
   long cap_phi = 0;
-  ccpol3b_tuning tune{0, 1, 2};
+  ccpol3b_tuning tune{0, 1, 2, 1};
   // creation parameters (Fortran-ordered 17x17 tables, current coefficients),
   // kept so a graph can own a private copy of the context
   std::vector<double> h_tab[6], h_cs[2];
@@ -157,6 +159,9 @@ struct ccpol3b_ctx {
   // call whose description is the same copies nothing
   std::vector<int> up_molbox, up_boxstart;
   std::vector<double> up_boxL;
+  // cell grids of the uploaded box description (cells a side, first cell)
+  std::vector<int> h_cellnc, h_cell0;
+  long cap_cells = 0;
   hipEvent_t ev[9] = {};
   // the FS3 list build runs on a second stream beside the FS2 passes
   hipStream_t side = nullptr;
@@ -210,6 +215,10 @@ void free_dev(ccpol3b_ctx* c) {
   Dev& D = c->D;
   dev_free(D.cx); dev_free(D.cy); dev_free(D.cz); dev_free(D.site);
   dev_free(D.mol_box); dev_free(D.boxL); dev_free(D.box_start);
+  dev_free(D.cell_nc); dev_free(D.cell0); dev_free(D.cell_cnt); dev_free(D.cell_start);
+  dev_free(D.cell_mol); dev_free(D.mol_cell);
+  D.ncells = 0;
+  c->cap_cells = 0;
   dev_free(D.nb3_cnt); dev_free(D.nb3); dev_free(D.up3);
   dev_free(D.nb2_cnt); dev_free(D.nb2); dev_free(D.up2); dev_free(D.nbv2);
   dev_free(D.sl2_cnt); dev_free(D.sl2_off);
@@ -233,6 +242,7 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
     const size_t nn = (size_t)n * D.cap_nb;
     if ((rc = dev_alloc(c, D.cx, n)) || (rc = dev_alloc(c, D.cy, n)) || (rc = dev_alloc(c, D.cz, n)) ||
         (rc = dev_alloc(c, D.site, (size_t)n * NS3 * 3)) || (rc = dev_alloc(c, D.mol_box, n)) ||
+        (rc = dev_alloc(c, D.cell_mol, n)) || (rc = dev_alloc(c, D.mol_cell, n)) ||
         (rc = dev_alloc(c, D.nb3_cnt, n)) || (rc = dev_alloc(c, D.nb2_cnt, n)) ||
         (rc = dev_alloc(c, D.up3, n)) || (rc = dev_alloc(c, D.up2, n)) ||
         (rc = dev_alloc(c, D.nb3, nn)) || (rc = dev_alloc(c, D.nb2, nn)) || (rc = dev_alloc(c, D.nbv2, 3 * nn)) ||
@@ -249,6 +259,7 @@ int ensure_molecules(ccpol3b_ctx* c, int nm, int nbox) {
   if (nbox > c->cap_box) {
     forget_upload(c);
     if ((rc = dev_alloc(c, D.boxL, nbox)) || (rc = dev_alloc(c, D.box_start, nbox + 1)) ||
+        (rc = dev_alloc(c, D.cell_nc, nbox)) || (rc = dev_alloc(c, D.cell0, nbox)) ||
         (rc = dev_alloc(c, D.box_t3, nbox + 1)) || (rc = dev_alloc(c, D.box_t2, nbox + 1)) ||
         (rc = dev_alloc(c, D.box_c3, nbox + 1)) || (rc = dev_alloc(c, D.box_c2, nbox + 1)))
       return rc;
@@ -286,6 +297,39 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
   return ms;
+}
+
+// Cell grids of the box description in c->h_*: a periodic box gets nc cells a
+// side, nc = floor(L / (r_max (1 + 1e-9))) with r_max the larger of the two
+// cutoffs, when that is at least CELL_MIN (with fewer the 27-cell stencil
+// covers the whole box and the sweep is cheaper), at most cbrt(8 n_box) (a few
+// molecules per cell or more) and at most 1024.  Allocates the count arrays.
+int setup_cells(ccpol3b_ctx* c, int nbox) {
+  Dev& D = c->D;
+  const double rmax = std::sqrt(std::max(c->K.rc2_3, c->K.rc2_2));
+  c->h_cellnc.assign(nbox, 0);
+  c->h_cell0.assign(nbox, 0);
+  long ncells = 0;
+  for (int b = 0; b < nbox; ++b) {
+    const double L = c->h_boxL[b];
+    const int nmb = c->h_boxstart[b + 1] - c->h_boxstart[b];
+    int nc = 0;
+    if (c->tune.neighbour_cells && L > 0.0 && rmax > 0.0 && nmb > 0) {
+      const double f = std::min({std::floor(L / (rmax * (1.0 + 1e-9))), std::floor(std::cbrt(8.0 * nmb)), 1024.0});
+      if (f >= CELL_MIN) nc = (int)f;
+    }
+    c->h_cellnc[b] = nc;
+    c->h_cell0[b] = (int)ncells;
+    ncells += (long)nc * nc * nc;
+    if (ncells > (1L << 28)) return fail(c, CCPOL3B_ECAPACITY, "too many neighbour-search cells");
+  }
+  if (ncells > c->cap_cells) {
+    int rc;
+    if ((rc = dev_alloc(c, D.cell_cnt, ncells)) || (rc = dev_alloc(c, D.cell_start, ncells + 1))) return rc;
+    c->cap_cells = ncells;
+  }
+  D.ncells = (int)ncells;
+  return CCPOL3B_OK;
 }
 
 // The evaluation proper, on device arrays already in place.  Enqueues the whole
@@ -327,21 +371,32 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     HIPCHK(c, hipMemcpyAsync(D.boxL, c->h_boxL.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(D.box_start, c->h_boxstart.data(), sizeof(int) * (nbox + 1),
                              hipMemcpyHostToDevice, st));
+    if ((rc = setup_cells(c, nbox))) return rc;
+    HIPCHK(c, hipMemcpyAsync(D.cell_nc, c->h_cellnc.data(), sizeof(int) * nbox, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(D.cell0, c->h_cell0.data(), sizeof(int) * nbox, hipMemcpyHostToDevice, st));
     c->up_molbox = c->h_molbox;
     c->up_boxL = c->h_boxL;
     c->up_boxstart = c->h_boxstart;
   }
   EVREC(0);
-  const long nprep = std::max<long>((long)nm * ACC_W, T_COUNT);
+  const long nprep = std::max<long>({(long)nm * ACC_W, (long)T_COUNT, (long)D.ncells});
   hipLaunchKernelGGL(k_prep, dim3((nprep + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
   const size_t lds = fs2_lds_bytes(D.cap_nb), lds_p = fs2_pair_lds_bytes(D.cap_nb);
   const size_t lds3 = fs3_list_lds_bytes(D.cap_nb);
+  // the neighbour search: FS3 staging, then the cell path's accepted entries
+  const size_t ldsn = (do3 ? lds3 : 0) + (D.ncells > 0 ? 2 * sizeof(int) * (size_t)D.cap_nb : 0);
   if (do2 && std::max(lds, lds_p) > 65536)
     return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the FS2 passes");
-  if (do3 && lds3 > 65536)
-    return fail(c, CCPOL3B_ECAPACITY, "FS3 neighbour count too large for the FS3 list build");
+  if (std::max(do3 ? lds3 : 0, ldsn) > 65536)
+    return fail(c, CCPOL3B_ECAPACITY, "neighbour count too large for the neighbour search and the FS3 list build");
+  if (D.ncells > 0) {
+    // cell grids: each molecule's cell, then the cells' molecule ranges
+    hipLaunchKernelGGL(k_cells_assign, dim3((nm + 255) / 256), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_cells_build<1024>, dim3(1), dim3(1024), 0, st, D);
+    LAUNCHCHK(c, "cell grids");
+  }
   // neighbour lists with the per-molecule FS3 triplet counts, then every offset
-  hipLaunchKernelGGL(k_neighbours, dim3(nm), dim3(64), do3 ? lds3 : 0, st, D, c->K, do3);
+  hipLaunchKernelGGL(k_neighbours, dim3(nm), dim3(64), ldsn, st, D, c->K, do3);
   hipLaunchKernelGGL(k_scan<1024>, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
   LAUNCHCHK(c, "neighbour lists");
   EVREC(1);
@@ -468,6 +523,7 @@ void finish_stats(ccpol3b_ctx* c) {
   c->stats.ms_reduce = std::min(event_ms(c->ev[2], c->ev[4]), event_ms(c->ev[3], c->ev[4]));
   c->stats.ms_total = event_ms(c->ev[0], c->ev[4]);
   c->stats.ms_fs3_eval = c->stats.n_fs3 > 0 ? event_ms(c->ev[8], c->ev[3]) : 0.0;
+  c->stats.n_cells = c->last_noalloc ? 0 : c->D.ncells;
 }
 
 void set_single_box(ccpol3b_ctx* c, int nm, double box) {
@@ -786,8 +842,11 @@ int ccpol3b_set_tuning(ccpol3b_ctx* c, const ccpol3b_tuning* t) {
   if (!c || !t || t->fs2_parts < 1 || t->fs2_parts > FS2_PARTS_MAX || t->fs2_pair_parts < 1 ||
       t->fs2_pair_parts > FS2_PARTS_MAX)
     return fail(c, CCPOL3B_EINVAL, "invalid tuning");
+  const int cells = t->neighbour_cells ? 1 : 0;
+  if (cells != c->tune.neighbour_cells) forget_upload(c);  // the cell grids follow the box upload
   c->tune = *t;
   c->tune.fs3_lds_coef = t->fs3_lds_coef ? 1 : 0;
+  c->tune.neighbour_cells = cells;
   ++c->version;
   return CCPOL3B_OK;
 }
@@ -816,7 +875,7 @@ int ccpol3b_autotune(ccpol3b_ctx* c, int nbox, const int* mol_start, const doubl
     for (int pp : parts)
       for (int pw : pparts) {
         if (rc) break;
-        c->tune = ccpol3b_tuning{lds, pp, pw};
+        c->tune = ccpol3b_tuning{lds, pp, pw, keep.neighbour_cells};
         double tot = 0.0;
         for (int r = 0; r <= reps && rc == CCPOL3B_OK; ++r) {
           rc = run_checked(c, nm, nbox, ld, x, y, z, xs, ys, zs, fs, fs + (size_t)ld * nm,
@@ -1219,6 +1278,7 @@ size_t noalloc_carve(unsigned char* base, Dev& D) {
   D.items2 = nullptr;
   take(D.cx, n); take(D.cy, n); take(D.cz, n); take(D.site, n * NS3 * 3);
   take(D.mol_box, n); take(D.boxL, nbox); take(D.box_start, nbox + 1);
+  take(D.cell_nc, nbox);  // 0: the sweep (no cell grid)
   take(D.nb3_cnt, n); take(D.nb3, nn); take(D.up3, n);
   take(D.nb2_cnt, n); take(D.nb2, nn); take(D.up2, n); take(D.nbv2, 3 * nn);
   take(D.sl2_cnt, n); take(D.sl2_off, n + 1);
@@ -1242,6 +1302,7 @@ __global__ void k_single_box(Dev D, int nm, double L) {
     D.boxL[0] = L;
     D.box_start[0] = 0;
     D.box_start[1] = nm;
+    D.cell_nc[0] = 0;
   }
 }
 

@@ -75,6 +75,10 @@ struct TriGeo {
   int pad;
 };
 
+// Smallest cell grid used: with 3 cells a side the 27-cell stencil is the
+// whole box, so the sweep over the box does the same with less work.
+constexpr int CELL_MIN = 4;
+
 // Device arrays of one evaluation (sized by the context; see api.hip).
 struct Dev {
   int nm, nbox, cap_nb;
@@ -87,6 +91,15 @@ struct Dev {
   int *mol_box;           // [nm]
   double *boxL;           // [nbox] (bohr, <= 0 open)
   int *box_start;         // [nbox+1]
+  // cell grids of the periodic boxes wide enough for one (nc >= CELL_MIN cells
+  // a side, each side at least the larger cutoff); the other boxes are swept
+  int ncells;             // cells of all boxes
+  int *cell_nc;           // [nbox] cells a side, 0: no grid
+  int *cell0;             // [nbox] first cell of the box
+  int *cell_cnt;          // [ncells] molecules per cell (zeroed by k_prep, back to 0 after the fill)
+  int *cell_start;        // [ncells+1] first entry of each cell in cell_mol
+  int *cell_mol;          // [nm] molecules grouped by cell
+  int *mol_cell;          // [nm] cell of each molecule (-1: no grid)
   // neighbour lists (ascending molecule index) and their first upper entries
   int *nb3_cnt, *nb3, *up3;   // FS3 cutoff: [nm], [nm][cap_nb], [nm]
   int *nb2_cnt, *nb2, *up2;   // FS2 cutoff

@@ -253,7 +253,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   __shared__ T2 sV[NPAIR3];
   __shared__ T sDer[NPAIR3];
   __shared__ T sR[NPAIR3];
-  __shared__ double sRed[4];
+  __shared__ double sRed[4][3];
   __shared__ TriGeo g;  // block-uniform geometry, read with lane-varying indices
   // PHI: dE_X/dC of the dense table [tx][ty][tz][p][q][s] of the current region
   __shared__ T sDC[PHI ? NCOEF : 1];
@@ -446,11 +446,21 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
         sR[po] = gacc[a][b2] * t_rcp<T>(r);
       }
   }
-  // region energies and the site virial (FP64, fixed reduction trees); the
-  // first reduction's barrier also publishes h
-  const double EX0 = block_sum256(eacc0, sRed);
-  const double EX1 = nreg == 2 ? block_sum256(eacc1, sRed) : 0.0;
-  const double Wsites = block_sum256(wsum, sRed);
+  // region energies and the site virial (FP64, fixed reduction trees: each
+  // wavefront's butterfly, then the four wavefront sums); the barrier also
+  // publishes h
+  {
+    const double s0 = wave_sum(eacc0), s1 = wave_sum(eacc1), s2 = wave_sum(wsum);
+    if ((tid & 63) == 0) {
+      sRed[tid >> 6][0] = s0;
+      sRed[tid >> 6][1] = s1;
+      sRed[tid >> 6][2] = s2;
+    }
+  }
+  __syncthreads();
+  const double EX0 = (sRed[0][0] + sRed[1][0]) + (sRed[2][0] + sRed[3][0]);
+  const double EX1 = (sRed[0][1] + sRed[1][1]) + (sRed[2][1] + sRed[3][1]);
+  const double Wsites = (sRed[0][2] + sRed[1][2]) + (sRed[2][2] + sRed[3][2]);
   const double Eb = g.w[0] * EX0 + g.w[1] * EX1;
   const double Et = g.fd * Eb;
   const double dEdR = g.dfd * Eb + g.fd * (g.wp[0] * EX0 + g.wp[1] * EX1);  // dE/dR_avg

@@ -6,9 +6,13 @@
 // zero, squares are summed left to right, and this file is compiled without
 // FMA contraction (the reference's x86-64 build has no FMA).
 //
-// Work decomposition: one wavefront per molecule for the neighbour sweep (which
+// Work decomposition: one wavefront per molecule for the neighbour search (which
 // also counts the molecule's FS3 triplets) and the FS3 list fill, one per
-// directed neighbour pair for the exported FS2 items.
+// directed neighbour pair for the exported FS2 items.  The neighbour search
+// tests the molecules of the 27 surrounding cells of a cell grid when the box
+// has one (periodic, at least CELL_MIN cells a side of at least the larger
+// cutoff), and every molecule of the box otherwise; both produce the same
+// lists, in ascending molecule index.
 // Lists come out in the reference's loop orders: FS3 (i,j,k) lexicographic
 // with i<j<k (:859-860), FS2 (k,i,j) with i<j (:297-301).
 #pragma clang fp contract(off)
@@ -57,6 +61,35 @@ __global__ void k_prep(int nm, int ld, const double* __restrict__ x, const doubl
   }
   if (t < nm * ACC_W) D.acc[2 * t] = D.acc[2 * t + 1] = 0ull;
   if (t < T_COUNT) D.totals[t] = 0;
+  if (t < D.ncells) D.cell_cnt[t] = 0;
+}
+
+// ------------------------------------------------------------------ cell grids
+// Cell of coordinate x (bohr) along one side: the position wrapped into [0, L)
+// over the cell side L/nc.  Rounding can only move a molecule lying on a cell
+// face into the adjacent cell, and the cell side exceeds the cutoff by a
+// relative 1e-9, so two molecules within the cutoff (minimum image) are
+// always in the same or adjacent cells.
+__device__ __forceinline__ int cell_coord(double x, double L, int nc) {
+  const double u = x - L * floor(x / L);
+  const int q = (int)(u / (L / nc));
+  return q < 0 ? 0 : (q >= nc ? nc - 1 : q);
+}
+
+// Thread per molecule: its cell (or -1 when its box has no grid) and the cell
+// occupancy counts.
+__global__ void k_cells_assign(Dev D) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= D.nm) return;
+  const int b = D.mol_box[m], nc = D.cell_nc[b];
+  int cc = -1;
+  if (nc >= CELL_MIN) {
+    const double L = D.boxL[b];
+    cc = D.cell0[b] + (cell_coord(D.cx[m], L, nc) * nc + cell_coord(D.cy[m], L, nc)) * nc +
+         cell_coord(D.cz[m], L, nc);
+    atomicAdd(D.cell_cnt + cc, 1);
+  }
+  D.mol_cell[m] = cc;
 }
 
 // ------------------------------------------------------------------ FS3 triplet sweep
@@ -137,9 +170,14 @@ __device__ int fs3_sweep(const Dev& D, const Consts& K, int i, const int* nb, in
 }
 
 // ------------------------------------------------------------------ neighbour lists
-// One wavefront (one 64-thread block) per molecule i; every other molecule j of
-// the same box is tested with the reference's pair predicate (min image of
-// x_i - x_j).  Both cutoffs come out of one sweep, in ascending j.  With do3 the
+// One wavefront (one 64-thread block) per molecule i; the candidates j are the
+// molecules of the 27 cells around i's cell when the box has a cell grid
+// (:297-336 test every molecule; the cells leave out only molecules beyond
+// the cutoffs), every other molecule of the box otherwise.  Each is tested
+// with the reference's pair predicate (min image of x_i - x_j), both cutoffs
+// at once.  The sweep meets j in ascending order; the cell path ranks the
+// accepted j and writes each at its rank, so both give the same lists.  With
+// do3 the
 // upper FS3 neighbours are staged in LDS on the way and the wavefront then
 // counts i's FS3 triplets into cnt3s[i], so that the triplet offsets come out of
 // the scan that follows and the list can be filled in one more launch.
@@ -157,43 +195,125 @@ __global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
   int* nb3 = D.nb3 + (long)i * cap;
   int* nb2 = D.nb2 + (long)i * cap;
   double* v2 = D.nbv2 + (long)i * 3 * cap;
-  for (int base = j0; base < j1; base += 64) {
-    const int j = base + lane;
-    bool in3 = false, in2 = false;
-    double xj = 0.0, yj = 0.0, zj = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
-    if (j < j1 && j != i) {
-      xj = D.cx[j];
-      yj = D.cy[j];
-      zj = D.cz[j];
-      dx = minimg(xi - xj, L);
-      dy = minimg(yi - yj, L);
-      dz = minimg(zi - zj, L);
-      const double d2 = sq3(dx, dy, dz);
-      in3 = d2 < K.rc2_3;
-      in2 = d2 < K.rc2_2;
+  const int nc = D.cell_nc[b];
+  if (nc >= CELL_MIN) {
+    // candidates: the molecules of the 27 cells around i's cell (distinct, as
+    // nc >= 3), cell q's members at flat positions [spre[q], spre[q+1])
+    __shared__ int spre[28], sbeg[27];
+    int* A3 = reinterpret_cast<int*>(lsm + (do3 ? 6 * cap : 0));  // accepted, in candidate order
+    int* A2 = A3 + cap;
+    const int c0 = D.cell0[b], ci = D.mol_cell[i] - c0;
+    const int ix = ci / (nc * nc), iy = (ci / nc) % nc, iz = ci % nc;
+    int len = 0;
+    if (lane < 27) {
+      int jx = ix + lane / 9 - 1, jy = iy + (lane / 3) % 3 - 1, jz = iz + lane % 3 - 1;
+      jx += jx < 0 ? nc : (jx >= nc ? -nc : 0);
+      jy += jy < 0 ? nc : (jy >= nc ? -nc : 0);
+      jz += jz < 0 ? nc : (jz >= nc ? -nc : 0);
+      const int cc = c0 + (jx * nc + jy) * nc + jz;
+      sbeg[lane] = D.cell_start[cc];
+      len = D.cell_start[cc + 1] - sbeg[lane];
     }
-    const uint64_t m3 = __ballot(in3), m2 = __ballot(in2);
-    const uint64_t lo3 = __ballot(in3 && j < i), lo2 = __ballot(in2 && j < i);
-    if (in3) {
-      const int p = n3 + __popcll(m3 & lanemask_lt());
-      if (p < cap) nb3[p] = j;
-      // upper entries (j > i) are staged at their position among the upper ones
-      const int pu = p - u3 - __popcll(lo3);
-      if (do3 && j > i && pu < cap) stage_upper(lsm, cap, pu, xi, yi, zi, xj, yj, zj, L);
+    int pre = len;  // inclusive prefix over lanes 0..26
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+      const int u = __shfl_up(pre, d, 64);
+      if (lane >= d) pre += u;
     }
-    if (in2) {
-      const int p = n2 + __popcll(m2 & lanemask_lt());
-      if (p < cap) {
-        nb2[p] = j;
-        v2[p] = dx;
-        v2[cap + p] = dy;
-        v2[2 * cap + p] = dz;
+    if (lane < 27) spre[lane + 1] = pre;
+    if (lane == 0) spre[0] = 0;
+    const int total = __shfl(pre, 26, 64);
+    __syncthreads();
+    for (int f0 = 0; f0 < total; f0 += 64) {
+      const int f = f0 + lane;
+      bool in3 = false, in2 = false;
+      int j = -1;
+      if (f < total) {
+        int lo = 0, hi = 27;  // spre[lo] <= f < spre[hi]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (spre[mid] <= f) lo = mid; else hi = mid;
+        }
+        j = D.cell_mol[sbeg[lo] + f - spre[lo]];
+        if (j != i) {
+          const double d2 = sq3(minimg(xi - D.cx[j], L), minimg(yi - D.cy[j], L), minimg(zi - D.cz[j], L));
+          in3 = d2 < K.rc2_3;
+          in2 = d2 < K.rc2_2;
+        }
       }
+      const uint64_t m3 = __ballot(in3), m2 = __ballot(in2);
+      if (in3) {
+        const int p = n3 + __popcll(m3 & lanemask_lt());
+        if (p < cap) A3[p] = j;
+      }
+      if (in2) {
+        const int p = n2 + __popcll(m2 & lanemask_lt());
+        if (p < cap) A2[p] = j;
+      }
+      n3 += __popcll(m3);
+      n2 += __popcll(m2);
+      u3 += __popcll(__ballot(in3 && j < i));
+      u2 += __popcll(__ballot(in2 && j < i));
     }
-    n3 += __popcll(m3);
-    n2 += __popcll(m2);
-    u3 += __popcll(lo3);
-    u2 += __popcll(lo2);
+    __syncthreads();
+    // ascending order: each stored entry goes to its rank among the stored entries
+    const int s3 = n3 < cap ? n3 : cap, s2 = n2 < cap ? n2 : cap;
+    for (int e = lane; e < s3; e += 64) {
+      const int j = A3[e];
+      int r = 0;
+      for (int f = 0; f < s3; ++f) r += A3[f] < j ? 1 : 0;
+      nb3[r] = j;
+      const int pu = r - u3;  // position among the upper entries
+      if (do3 && j > i && pu >= 0 && pu < cap) stage_upper(lsm, cap, pu, xi, yi, zi, D.cx[j], D.cy[j], D.cz[j], L);
+    }
+    for (int e = lane; e < s2; e += 64) {
+      const int j = A2[e];
+      int r = 0;
+      for (int f = 0; f < s2; ++f) r += A2[f] < j ? 1 : 0;
+      nb2[r] = j;
+      v2[r] = minimg(xi - D.cx[j], L);
+      v2[cap + r] = minimg(yi - D.cy[j], L);
+      v2[2 * cap + r] = minimg(zi - D.cz[j], L);
+    }
+  } else {
+    for (int base = j0; base < j1; base += 64) {
+      const int j = base + lane;
+      bool in3 = false, in2 = false;
+      double xj = 0.0, yj = 0.0, zj = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+      if (j < j1 && j != i) {
+        xj = D.cx[j];
+        yj = D.cy[j];
+        zj = D.cz[j];
+        dx = minimg(xi - xj, L);
+        dy = minimg(yi - yj, L);
+        dz = minimg(zi - zj, L);
+        const double d2 = sq3(dx, dy, dz);
+        in3 = d2 < K.rc2_3;
+        in2 = d2 < K.rc2_2;
+      }
+      const uint64_t m3 = __ballot(in3), m2 = __ballot(in2);
+      const uint64_t lo3 = __ballot(in3 && j < i), lo2 = __ballot(in2 && j < i);
+      if (in3) {
+        const int p = n3 + __popcll(m3 & lanemask_lt());
+        if (p < cap) nb3[p] = j;
+        // upper entries (j > i) are staged at their position among the upper ones
+        const int pu = p - u3 - __popcll(lo3);
+        if (do3 && j > i && pu < cap) stage_upper(lsm, cap, pu, xi, yi, zi, xj, yj, zj, L);
+      }
+      if (in2) {
+        const int p = n2 + __popcll(m2 & lanemask_lt());
+        if (p < cap) {
+          nb2[p] = j;
+          v2[p] = dx;
+          v2[cap + p] = dy;
+          v2[2 * cap + p] = dz;
+        }
+      }
+      n3 += __popcll(m3);
+      n2 += __popcll(m2);
+      u3 += __popcll(lo3);
+      u2 += __popcll(lo2);
+    }
   }
   const int nn3 = n3 < cap ? n3 : cap, nn2 = n2 < cap ? n2 : cap;
   const int up3 = u3 < nn3 ? u3 : nn3, up2 = u2 < nn2 ? u2 : nn2;
@@ -370,6 +490,24 @@ __device__ void block_exclusive_scan(int* buf, const int* const (&cnt)[K], int* 
     }
   __syncthreads();
 }
+
+// One block: exclusive scan of the cell counts into cell_start, then every
+// molecule into its cell's range of cell_mol (order within a cell is not
+// used: the neighbour search ranks what it accepts).  Counting the cell down
+// leaves cell_cnt at zero.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_cells_build(Dev D) {
+  __shared__ int buf[SCAN_BUF];
+  const int* const c[1] = {D.cell_cnt};
+  int* const o[1] = {D.cell_start};
+  int* const tt[1] = {D.cell_start + D.ncells};
+  block_exclusive_scan<NT, 1>(buf, c, o, D.ncells, tt);
+  for (int m = threadIdx.x; m < D.nm; m += NT) {
+    const int cc = D.mol_cell[m];
+    if (cc >= 0) D.cell_mol[D.cell_start[cc] + atomicSub(D.cell_cnt + cc, 1) - 1] = m;
+  }
+}
+template __global__ void k_cells_build<1024>(Dev);
 
 // Triplet range of each box (the list is sorted by its first molecule, and
 // boxes are contiguous molecule ranges) and the number of reduction chunks of

@@ -81,6 +81,8 @@ typedef struct {
   double ms_reduce;                  /* from the later branch's end: write-back and energy sums */
   double ms_h2d, ms_d2h;             /* host-pointer entry point only */
   double ms_fs3_eval;                /* the FS3 evaluation kernel alone (inside ms_fs3) */
+  long n_cells;                      /* neighbour-search cells over the call's boxes (0: every box
+                                        swept molecule by molecule) */
 } ccpol3b_stats;
 
 /* Create / destroy a context holding device copies of the parameters.
@@ -184,6 +186,9 @@ typedef struct {
                                  0: they read it through the vector cache (default) */
   int fs2_parts;              /* FS2 spectator-pass blocks per molecule, 1..8 (default 1) */
   int fs2_pair_parts;         /* FS2 pair-pass blocks per molecule, 1..8 (default 2) */
+  int neighbour_cells;        /* 1: cell-grid neighbour search in periodic boxes of at least 4
+                                 cells a side (default); 0: test every molecule of the box.
+                                 The lists are the same either way. */
 } ccpol3b_tuning;
 
 int ccpol3b_get_tuning(const ccpol3b_ctx *ctx, ccpol3b_tuning *tuning);

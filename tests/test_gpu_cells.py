@@ -1,0 +1,64 @@
+"""Cell-grid neighbour search (periodic boxes of at least 4 cells a side, each
+side at least the FS2 cutoff of 7.5 A): it must give the lists of the sweep
+over the whole box, which are the reference's (threebody_potential.f90
+:297-336, 859-892), so every list, energy and force equals the sweep's bit for
+bit.  Cases: 1,024 waters (4 cells a side), 2,048 waters (5), 1,024 waters
+with the molecules moved by whole box lengths in and out of [0, L) (cell
+assignment wraps positions), and 256 waters at 1/4 density (4 cells a side)
+against the oracle's lists."""
+import numpy as np
+import pytest
+
+from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox, lattice_box
+
+pytestmark = pytest.mark.gpu
+
+
+def evaluate(params, box, cells):
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    tb = ThreeBody(params, "fp64")
+    tb.set_tuning(neighbour_cells=cells)
+    got = tb.evaluate(box)
+    return got, tb.stats(), tb.fs3_list(), tb.fs2_list()
+
+
+def cases():
+    b1024 = lattice_box(1024, seed=41)
+    rng = np.random.default_rng(42)
+    shift = rng.integers(-3, 4, size=(1024, 3)) * b1024.box
+    shifted = WaterBox(b1024.com + shift, b1024.quats, b1024.box)
+    b256 = lattice_box(256, seed=43)
+    s = 4.0 ** (1 / 3)
+    dilute = WaterBox(b256.com * s, b256.quats, b256.box * s)
+    return {"1024": (b1024, 4), "2048": (lattice_box(2048, seed=44), 5), "1024_shifted": (shifted, 4),
+            "256_dilute": (dilute, 4)}
+
+
+CASES = cases()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_cells_match_sweep(params_fs2fs3, oracle, name):
+    box, nc = CASES[name]
+    print(f"cell lists: {name} ({box.nm} waters, L = {box.box:.2f} A, {nc} cells a side)", flush=True)
+    got, st, l3, l2 = evaluate(params_fs2fs3, box, 1)
+    ref, st0, r3, r2 = evaluate(params_fs2fs3, box, 0)
+    assert st["n_cells"] == nc ** 3 and st0["n_cells"] == 0
+    print(f"lists: {st['ms_lists']:.3f} ms with cells, {st0['ms_lists']:.3f} ms sweeping; "
+          f"{st['n_fs3']} triplets, {st['n_fs2_items']} FS2 items", flush=True)
+    assert st["n_fs3"] == st0["n_fs3"] > 0 and st["n_fs2_items"] == st0["n_fs2_items"] > 0
+    np.testing.assert_array_equal(l3, r3)
+    np.testing.assert_array_equal(l2, r2)
+    for k in ("qfs2", "qfs3", "vir"):
+        assert got[k] == ref[k], k
+    for k in ("fxs", "fys", "fzs", "fxfs2", "fyfs2", "fzfs2", "fxfs3", "fyfs3", "fzfs3"):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    if box.nm <= 256:
+        np.testing.assert_array_equal(l3, oracle.fs3_list(params_fs2fs3, box.x, box.y, box.z, box.box))
+        np.testing.assert_array_equal(l2, oracle.fs2_list(params_fs2fs3, box.x, box.y, box.z, box.box))
+
+
+def test_cells_small_box_sweeps(params_fs2fs3):
+    """512 waters (L = 24.8 A, 3 cells a side): no grid, the sweep runs."""
+    _, st, _, _ = evaluate(params_fs2fs3, lattice_box(512, seed=45), 1)
+    assert st["n_cells"] == 0 and st["n_fs3"] > 0
