@@ -122,9 +122,10 @@ def bench_batched(p, seed, reps=10):
                 torch.tensor(b.site.transpose(2, 0, 1).copy(), device=dev), b.box) for b in boxes]
     out = {"n_boxes": len(boxes), "n_molecules": int(ms[-1]),
            "box_sizes": [int(b.nm) for b in boxes]}
+    res = {}
     for prec in ("mixed", "fp64"):
         tb = ThreeBody(p, prec)
-        tb.eval_batch_device(ms, bl, com, site)
+        res[prec] = [t.cpu().numpy() for t in tb.eval_batch_device(ms, bl, com, site)]
         n3 = tb.stats()["n_fs3"]
         seq = wall_ms(lambda: [tb.eval_device(c, s, L) for c, s, L in per_box], max(2, reps // 4))
         batch = wall_ms(lambda: tb.eval_batch_device(ms, bl, com, site), reps)
@@ -138,6 +139,51 @@ def bench_batched(p, seed, reps=10):
                      "evals_per_s_graph": n3 / (graph * 1e-3), "evals_per_s_sequential": n3 / (seq * 1e-3),
                      "graph_over_sequential": seq / graph}
         log(f"batched {prec}: sequential {seq:.3f} ms, batched {batch:.3f} ms, graph {graph:.3f} ms, tuning {tune}")
+    # mixed against FP64 over the whole batch: site, COM and molecular forces,
+    # and the largest relative error of a box's FS3 and FS2 energies
+    (fm, c2m, c3m, em), (f6, c26, c36, e6) = res["mixed"], res["fp64"]
+    rel = lambda a, b: float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+    mol_m = fm[:, :, :17].sum(2) + c2m + c3m
+    mol_6 = f6[:, :, :17].sum(2) + c26 + c36
+    out["mixed_error_vs_fp64"] = {
+        "site_force_rel_rms": rel(fm[:, :, :17], f6[:, :, :17]),
+        "com_force_rel_rms": rel(np.concatenate([c2m, c3m]), np.concatenate([c26, c36])),
+        "molecular_force_rel_rms": rel(mol_m, mol_6),
+        "fs3_energy_rel_max": float(np.max(np.abs(em[:, 1] - e6[:, 1]) / np.abs(e6[:, 1]))),
+        "fs2_energy_rel_max": float(np.max(np.abs(em[:, 0] - e6[:, 0]) / np.abs(e6[:, 0])))}
+    log(f"batched mixed vs fp64: {out['mixed_error_vs_fp64']}")
+    return out
+
+
+def bench_neighbours(p, seed, sizes=(1024, 2048), reps=7):
+    """The list build (conversion, neighbour search, offsets: ms_lists) at larger
+    boxes, with the cell grid (4 and 5 cells a side) and with the sweep over
+    the whole box; the lists are the same."""
+    import torch
+    from ccpol23_plus_2025_mbmd_amd.geometry import lattice_box
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    dev = torch.device("cuda:0")
+    out = {}
+    for n in sizes:
+        box = lattice_box(n, seed=seed)
+        com = torch.tensor(box.com.T.copy(), device=dev).contiguous()
+        site = torch.tensor(box.site.transpose(2, 0, 1).copy(), device=dev).contiguous()
+        row = {"box_A": box.box}
+        for cells in (1, 0):
+            tb = ThreeBody(p, "fp64")
+            tb.set_tuning(neighbour_cells=cells)
+            lists, total = [], []
+            for r in range(reps + 2):
+                tb.eval_device(com, site, box.box)
+                torch.cuda.synchronize()
+                st = tb.stats()
+                if r >= 2:
+                    lists.append(st["ms_lists"])
+                    total.append(st["ms_total"])
+            row["cells" if cells else "sweep"] = {"ms_lists": float(np.median(lists)),
+                                                  "ms_call": float(np.median(total)), "n_cells": st["n_cells"]}
+        out[str(n)] = row
+        log(f"neighbours {n}: cells {row['cells']}, sweep {row['sweep']}")
     return out
 
 
@@ -391,6 +437,7 @@ def main():
         out["batched_96_boxes"] = bench_batched(p, args.seed)
         out["gradient_4096_trimers"] = bench_gradient(p, args.seed)
         out["md_nve_512"] = bench_md(p, args.n, args.seed)
+        out["neighbour_search"] = bench_neighbours(p, args.seed)
     if not args.no_cpu_baseline:
         host = host_cpu()
         threads = args.cpu_threads or host["affinity"]

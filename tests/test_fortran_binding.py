@@ -136,7 +136,7 @@ def test_fortran_md_loop_matches_python(tmp_path):
 
 # ------------------------------------------------ the INTEGRATION.md snippets at the reference's shapes
 REF_DRIVER = HERE / "fortran" / "_build" / "drive_ref"
-REF_SOURCE = HERE / "fortran" / "drive_ref.f90"
+REF_SOURCE = HERE / "fortran" / "ref_dropin.f90"
 INTEGRATION = HERE.parent / "INTEGRATION.md"
 
 
@@ -155,14 +155,15 @@ def integration_snippets():
 
 def test_integration_snippets_are_the_tested_program():
     """Every line of the two snippets a maintainer copies from INTEGRATION.md
-    appears, in the same order, in tests/fortran/drive_ref.f90 (which the GPU
-    test below runs against the golden boxes); and the cutoff passed is rcut3
-    (A), not the squared rcutsd3."""
+    appears, in the same order, in tests/fortran/ref_dropin.f90 (the drop-in
+    that drive_ref and drive_nve link, which the GPU tests below run against
+    the golden boxes and through 1000 MD steps); and the cutoff passed is
+    rcut3 (A), not the squared rcutsd3."""
     src = [ln.strip() for ln in REF_SOURCE.read_text().splitlines() if ln.strip()]
     for block in integration_snippets():
         at = 0
         for line in block:
-            assert line in src[at:], f"INTEGRATION.md line not in drive_ref.f90 (in order): {line!r}"
+            assert line in src[at:], f"INTEGRATION.md line not in ref_dropin.f90 (in order): {line!r}"
             at = src.index(line, at) + 1
     flat = " ".join(" ".join(b) for b in integration_snippets())
     assert "tb_par%rcut3 = rcut3" in flat and "rcutsd3" not in flat
@@ -206,5 +207,99 @@ def test_reference_shapes_drop_in_matches_golden(tmp_path, name):
     rel = lambda a, b: np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
     assert rel(fsite, g["fsite"]) <= 1e-9
     if s2:
-        assert rel(com[:, :3], g["fcom2"]) <= 1e-7
-    assert rel(com[:, 3:], g["fcom3"]) <= 1e-7
+        assert rel(com[:, :3], g["fcom2"]) <= 1e-9
+    assert rel(com[:, 3:], g["fcom3"]) <= 1e-9
+
+
+# ------------------------------------------------ a Fortran MD loop calling the drop-in every step
+NVE_DRIVER = HERE / "fortran" / "_build" / "drive_nve"
+
+
+def read_nve(path, nm):
+    """Energies per step and the snapshots of drive_nve's output."""
+    lines = Path(path).read_text().splitlines()
+    energies, snaps, k = [], {}, 0
+    arr = lambda rows: np.array([[float(v) for v in ln.split()] for ln in rows])
+    while k < len(lines):
+        tag = lines[k].split()
+        if tag[0] == "E":
+            energies.append([float(v) for v in tag[2:]])
+            k += 1
+        else:
+            step = int(tag[1])
+            k += 1
+            vals = [float(v) for v in lines[k].split()]
+            k += 1
+            snap = {"qfs2": vals[0], "qfs3": vals[1], "vir": vals[2]}
+            snap["com"] = arr(lines[k:k + nm]); k += nm
+            snap["site"] = arr(lines[k:k + 17 * nm]).reshape(nm, 17, 3); k += 17 * nm
+            snap["fsite"] = arr(lines[k:k + 17 * nm]).reshape(nm, 17, 3); k += 17 * nm
+            c = arr(lines[k:k + nm]); k += nm
+            snap["fcom2"], snap["fcom3"] = c[:, :3], c[:, 3:]
+            snaps[step] = snap
+    return np.array(energies), snaps
+
+
+@pytest.mark.gpu
+def test_fortran_nve_loop_1000_steps(tmp_path, oracle):
+    """256 waters (L = 19.71 A) from 298 K, FS2+FS3 in FP64: 1000 NVE steps of
+    0.5 fs by a Fortran host loop (tests/fortran/drive_nve.f90) that calls the
+    INTEGRATION.md drop-in threebody_term at every step, as the reference's MD
+    loop calls the original.  Every 100 steps: the snapshot's triplet and
+    item lists bit-exact with the oracle's, and the energies, virial, site
+    forces and FS2 and FS3 COM forces the Fortran loop received within the
+    FP64 bar of the oracle's.  Over the first 20 steps the total energy stays
+    put (the three-body term alone has no short-range repulsion, so the
+    trajectory is unphysical after ~20 fs, but every snapshot is a valid
+    geometry to compare on)."""
+    from ccpol23_plus_2025_mbmd_amd.md import maxwell_state, water_constants
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    assert NVE_DRIVER.exists(), "tests/fortran/_build/drive_nve is built by __graft_entry__.build()"
+    p = load_variant("fs2fs3")
+    box = lattice_box(256, seed=17)
+    pc, mass, pmi = water_constants()
+    v, j = maxwell_state(box.nm, 298.0, mass, pmi, seed=5)
+    fmt = lambda x: f"{float(x):.17g}"
+    d = DATA_DIR / "fs2fs3"
+    inp, out = tmp_path / "in.txt", tmp_path / "out.txt"
+    with open(inp, "w") as fh:
+        fh.write(f"{box.nm} {fmt(box.box)} 1000 100 {fmt(5e-4)}\n")
+        fh.write(f"{fmt(p.alpha3)} {fmt(p.delta33)} {fmt(p.rcut3)} {int(p.s2_term)} {int(p.s3_term)}\n")
+        fh.write(f"{d / 'data3b_full_SR_5p5'}\n{d / 'data3b_full_LR_4p5'}\n")
+        fh.write(" ".join(fmt(x) for x in [mass, *pmi]) + "\n")
+        for s in range(17):
+            fh.write(" ".join(fmt(x) for x in pc[s]) + "\n")
+        for i in range(box.nm):
+            fh.write(" ".join(fmt(x) for x in [*box.com[i], *box.quats[i], *v[i], *j[i]]) + "\n")
+    print("fortran NVE loop: 256 waters, 1000 steps of 0.5 fs, drop-in called every step", flush=True)
+    r = subprocess.run([str(NVE_DRIVER), str(inp), str(out)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and "ok" in r.stdout, (r.stdout[-2000:], r.stderr[-2000:])
+    energies, snaps = read_nve(out, box.nm)
+    assert energies.shape == (1001, 3) and sorted(snaps) == list(range(0, 1001, 100))
+    etot = energies.sum(1)
+    k = energies[:21, 1] + energies[:21, 2]
+    drift = np.abs(etot[:21] - etot[0]).max()
+    print(f"first 20 steps: E3 {energies[0, 0]:.4f} -> {energies[20, 0]:.4f}, max |dE| {drift:.2e} kcal/mol",
+          flush=True)
+    assert np.abs(energies[20, 0] - energies[0, 0]) > 10 * drift and drift <= 2e-3 * k.mean()
+    rel = lambda a, b: float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+    tb = ThreeBody(p, "fp64")
+    for step, sn in sorted(snaps.items()):
+        nm = box.nm
+        x, y, z = (sn["com"][:, c].copy() for c in range(3))
+        xs, ys, zs = (sn["site"][:, :, c].reshape(-1).copy() for c in range(3))
+        ref = oracle.threebody_term(p, x=x, y=y, z=z, xs=xs, ys=ys, zs=zs, box=box.box, ld=17, nthreads=16)
+        tb.threebody_term(x, y, z, xs, ys, zs, box.box, *(np.zeros(17 * nm) for _ in range(3)),
+                          *(np.zeros(nm) for _ in range(6)), 0.0, ld=17)
+        np.testing.assert_array_equal(tb.fs3_list(), oracle.fs3_list(p, x, y, z, box.box))
+        np.testing.assert_array_equal(tb.fs2_list(), oracle.fs2_list(p, x, y, z, box.box))
+        for key in ("qfs2", "qfs3"):
+            assert abs(sn[key] - ref[key]) <= 1e-10 * abs(ref[key]) + 2e-12, (step, key, sn[key], ref[key])
+        assert abs(sn["vir"] - ref["vir"]) <= 1e-9 * abs(ref["vir"]) + 1e-9, step
+        fr = np.stack([ref[c].reshape(nm, 17) for c in ("fxs", "fys", "fzs")], -1)
+        e_site = rel(sn["fsite"], fr)
+        e_c2 = rel(sn["fcom2"], np.stack([ref[c] for c in ("fxfs2", "fyfs2", "fzfs2")], -1))
+        e_c3 = rel(sn["fcom3"], np.stack([ref[c] for c in ("fxfs3", "fyfs3", "fzfs3")], -1))
+        print(f"step {step}: E3 {sn['qfs2'] + sn['qfs3']:.6f} kcal/mol, site {e_site:.1e}, "
+              f"COM FS2 {e_c2:.1e}, COM FS3 {e_c3:.1e}", flush=True)
+        assert e_site <= 1e-9 and e_c2 <= 1e-9 and e_c3 <= 1e-9, step

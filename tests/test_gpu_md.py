@@ -22,10 +22,12 @@ def oracle_forces(oracle, params):
     return force_fn
 
 
-@pytest.mark.parametrize("variant", ["fs3", "fs2fs3"])
-def test_md_matches_reference_step(variant, oracle, params_fs3, params_fs2fs3):
-    """Ten 0.5 fs steps of 64 waters (L = 12.42 A) from 298 K: device driver vs
-    the restated reference step with oracle forces."""
+@pytest.mark.parametrize("variant,nsteps", [("fs3", 100), ("fs2fs3", 10)])
+def test_md_matches_reference_step(variant, nsteps, oracle, params_fs3, params_fs2fs3):
+    """0.5 fs steps of 64 waters (L = 12.42 A) from 298 K Maxwell velocities:
+    device driver vs the restated reference step with oracle forces; the FS3
+    variant for 100 steps (the 64-water NVE run of the FS3 workload), FS2+FS3
+    for 10."""
     from ccpol23_plus_2025_mbmd_amd.geometry import lattice_box
     from ccpol23_plus_2025_mbmd_amd.md import RigidMD, maxwell_state, water_constants
     from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
@@ -33,21 +35,22 @@ def test_md_matches_reference_step(variant, oracle, params_fs3, params_fs2fs3):
     box = lattice_box(64, seed=1)
     pc, mass, pmi = water_constants()
     v, j = maxwell_state(box.nm, 298.0, mass, pmi, seed=2)
-    print(f"md vs reference step: {variant}, 64 waters, 10 steps", flush=True)
+    print(f"md vs reference step: {variant}, 64 waters, {nsteps} steps", flush=True)
     md = RigidMD(ThreeBody(p, "fp64"), box.com, box.quats, box.box, v, j)
-    rec = md.run(10, 5e-4)
+    rec = md.run(nsteps, 5e-4)
     got = md.state()
     ff = oracle_forces(oracle, p)
     ff.box = box.box
     st = {"com": box.com.copy(), "quat": box.quats.copy(), "vel": v.copy(), "jbody": j.copy()}
     f = ff(st["com"], md_ref.sites(st["quat"], pc))[:2]
     e = []
-    for _ in range(10):
+    for _ in range(nsteps):
         st, f, en = md_ref.nve_step(st, f, ff, 5e-4, pc, mass, pmi, box.box)
         e.append(en)
     for key, tol in (("com", 1e-9), ("quat", 1e-9), ("vel", 1e-7), ("jbody", 1e-7)):
         a, b = got[key], st[key]
         err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        print(f"{key}: max relative difference {err:.2e} after {nsteps} steps", flush=True)
         assert err <= tol, (key, err)
     e = np.array(e)
     assert np.allclose(rec[:, 0], e[:, 0], rtol=1e-9, atol=1e-10)
