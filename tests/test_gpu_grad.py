@@ -255,3 +255,143 @@ def test_autograd_matches_reference_analytic_forces(params_fs2fs3, trimers, orac
                           for b in sel]) * KCAL_PER_FORCE_UNIT
     err = np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum())
     assert err <= (1e-9 if precision == "fp64" else 1e-3), err
+
+
+def rotation_residual(com, g_com, quat, g_quat):
+    """Per box of 3 molecules and lab axis e: sum_m (e x x_m) . dE/dx_m + dE/dq_m . dq_m/dtheta,
+    the derivative of E under a rigid rotation of the whole trimer (zero for an
+    isolated trimer), with dq/dtheta = (0, e) (x) q / 2; and the sum of the
+    magnitudes of its terms."""
+    nb = len(com) // 3
+    res = np.zeros((nb, 3))
+    mag = np.zeros((nb, 3))
+    for a in range(3):
+        e = np.eye(3)[a]
+        t1 = (np.cross(e, com) * g_com).sum(1)
+        q0, qv = quat[:, 0], quat[:, 1:]
+        dq = 0.5 * np.concatenate([(-(qv @ e))[:, None], q0[:, None] * e + np.cross(e, qv)], 1)
+        t2 = (g_quat * dq).sum(1)
+        res[:, a] = (t1 + t2).reshape(nb, 3).sum(1)
+        mag[:, a] = (np.abs(t1) + np.abs(t2)).reshape(nb, 3).sum(1)
+    return res, mag
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_4096_trimers_gradient_path(params_fs2fs3, precision):
+    """The differentiation workload at full size: 4096 isolated trimers, R_avg
+    uniform in 2.5-9.0 A (SR, switch and LR regions), monomer separations
+    >= 2.3 A, random orientations, through torch.autograd in one batched call
+    (energies, dE/d(COM), dE/d(orientation) via the quaternions, dE/dc), with
+    size-independent properties: every region present; E_FS3 = phi . c for
+    each trimer (E is linear in the coefficients) and dE/dc = sum_t w_t phi_t;
+    the COM gradients of each trimer sum to zero (translation invariance) and
+    the derivative under a rigid rotation of each trimer vanishes (rotation
+    invariance); the Gram matrix of the 4096 x 728 coefficient gradients on the
+    matrix cores against FP64 torch.  Mixed precision also against the FP64
+    gradients (measured 6.4e-5 COM, 5.2e-5 orientation, 2.7e-8 coefficients;
+    asserted at 2e-4, 2e-4, 1e-7)."""
+    from ccpol23_plus_2025_mbmd_amd.autograd import threebody_energy
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    dev = torch.device("cuda:0")
+    tri = trimer_set(4096, seed=2025)
+    com0, site0, ms, bl = pack(tri)
+    q0 = np.concatenate([b.quats for b in tri])
+    pc = torch.tensor(principal_coords()[1], device=dev)
+    c0 = coef_vector(params_fs2fs3)
+    w = torch.linspace(0.5, 1.5, len(tri), device=dev, dtype=torch.float64)
+    print(f"4096 trimers through the gradient path, {precision}", flush=True)
+
+    def grads(prec):
+        tb = ThreeBody(params_fs2fs3, prec)
+        com = torch.tensor(com0, device=dev, requires_grad=True)
+        quat = torch.tensor(q0, device=dev, requires_grad=True)
+        coef = torch.tensor(c0, device=dev, requires_grad=True)
+        e = threebody_energy(tb, com, quat, coef, pc, ms, bl)
+        (w * e).sum().backward()
+        st = tb.stats()
+        com_t = torch.tensor(com0.T.copy(), device=dev).contiguous()
+        site_t = torch.tensor(site0.transpose(2, 0, 1).copy(), device=dev).contiguous()
+        _, _, _, en, phi = tb.eval_batch_grad_device(ms, bl, com_t, site_t)
+        return (e.detach().cpu().numpy(), com.grad.cpu().numpy(), quat.grad.cpu().numpy(),
+                coef.grad.cpu().numpy(), en.cpu().numpy(), phi, st, tb)
+
+    e, g_com, g_quat, g_coef, en, phi, st, tb = grads(precision)
+    print(f"FS3 triplets: SR {st['n_fs3_sr']}, switch {st['n_fs3_switch']}, LR {st['n_fs3_lr']} "
+          f"(the other trimers have a pair beyond the 6.3 A FS3 cutoff)", flush=True)
+    assert st["n_fs3_sr"] > 50 and st["n_fs3_switch"] > 50 and st["n_fs3_lr"] > 50, st
+    phi_h = phi.cpu().numpy()
+    lin = np.abs(phi_h @ c0 - en[:, 1])
+    scale_e = np.abs(phi_h) @ np.abs(c0)
+    assert np.all(lin <= (1e-12 if precision == "fp64" else 1e-5) * scale_e + 1e-15), (lin / scale_e).max()
+    wg = w.cpu().numpy() @ phi_h
+    assert np.all(np.abs(g_coef - wg) <= 1e-12 * (w.cpu().numpy() @ np.abs(phi_h)) + 1e-300)
+    tsum = np.abs(g_com.reshape(-1, 3, 3).sum(1))
+    tmag = np.abs(g_com.reshape(-1, 3, 3)).sum(1)
+    print(f"translation residual max {(tsum / (tmag + 1e-300)).max():.1e}", flush=True)
+    assert np.all(tsum <= 1e-9 * tmag + 1e-15)
+    res, mag = rotation_residual(com0, g_com, q0, g_quat)
+    print(f"rotation residual max {(np.abs(res) / (mag + 1e-300)).max():.1e}", flush=True)
+    assert np.all(np.abs(res) <= 1e-9 * mag + 1e-15)
+    # the Gram of this phi on the matrix cores
+    g = tb.gram(phi)
+    ref = phi.T @ phi
+    m = phi.abs().max(0).values
+    bound = 1e-12 * (phi.abs().T @ phi.abs()) + 16 * 2.0 ** -53 * len(tri) * torch.outer(m, m)
+    assert ((g - ref).abs() <= bound).all(), ((g - ref).abs() / bound).max().item()
+    if precision == "mixed":
+        _, g64, q64, c64, *_ = grads("fp64")
+        rel = lambda a, b: float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+        errs = {"com": rel(g_com, g64), "quat": rel(g_quat, q64), "coef": rel(g_coef, c64)}
+        print(f"mixed vs fp64 gradients: {errs}", flush=True)
+        assert errs["com"] <= 2e-4 and errs["quat"] <= 2e-4 and errs["coef"] <= 1e-7, errs
+
+
+def test_mixed_gradient_against_fd_of_fp64_energy(params_fs2fs3, trimers):
+    """The mixed-precision dE/d(COM) and dE/d(orientation) against central
+    differences (h = 1e-5 A, 1e-5 rad) of the FP64 energy: the mixed gradient is
+    the derivative of the reference energy up to the mixed path's force error
+    (measured 8.2e-5 of the largest gradient component; asserted at 3e-4)."""
+    from ccpol23_plus_2025_mbmd_amd.autograd import threebody_energy
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    tb_m = ThreeBody(params_fs2fs3, "mixed")
+    tb_d = ThreeBody(params_fs2fs3, "fp64")
+    dev = torch.device("cuda:0")
+    sel = trimers[:12]
+    com0 = np.concatenate([b.com for b in sel])
+    q0 = np.concatenate([b.quats for b in sel])
+    ms = np.cumsum([0] + [3] * len(sel)).astype(np.int32)
+    bl = np.zeros(len(sel))
+    pc = torch.tensor(principal_coords()[1], device=dev)
+    c0 = torch.tensor(coef_vector(params_fs2fs3), device=dev)
+    print(f"mixed gradient vs FD of the FP64 energy: {len(sel)} trimers", flush=True)
+    com = torch.tensor(com0, device=dev, requires_grad=True)
+    quat = torch.tensor(q0, device=dev, requires_grad=True)
+    threebody_energy(tb_m, com, quat, c0, pc, ms, bl).sum().backward()
+    g_com, g_quat = com.grad.cpu().numpy(), quat.grad.cpu().numpy()
+
+    def L(comn, quatn):
+        with torch.no_grad():
+            return float(threebody_energy(tb_d, torch.tensor(comn, device=dev), torch.tensor(quatn, device=dev),
+                                          c0, pc, ms, bl).sum())
+
+    h = 1e-5
+    scale = np.abs(g_com).max()
+    worst = 0.0
+    for m in (0, 4, 11, 20, 33):
+        for a in range(3):
+            cp, cm = com0.copy(), com0.copy()
+            cp[m, a] += h
+            cm[m, a] -= h
+            fd = (L(cp, q0) - L(cm, q0)) / (2 * h)
+            worst = max(worst, abs(fd - g_com[m, a]) / scale)
+    for m in (1, 7, 30):
+        for a in range(3):
+            e_ax = np.eye(3)[a]
+            qp, qm = q0.copy(), q0.copy()
+            qp[m] = rotate_quat(q0[m], e_ax, h)
+            qm[m] = rotate_quat(q0[m], e_ax, -h)
+            fd = (L(com0, qp) - L(com0, qm)) / (2 * h)
+            dq = (rotate_quat(q0[m], e_ax, 1e-7) - rotate_quat(q0[m], e_ax, -1e-7)) / 2e-7
+            worst = max(worst, abs(fd - float(g_quat[m] @ dq)) / scale)
+    print(f"largest difference / largest component: {worst:.2e}", flush=True)
+    assert worst <= 3e-4, worst
