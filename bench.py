@@ -155,10 +155,11 @@ def bench_batched(p, seed, reps=10):
     return out
 
 
-def bench_neighbours(p, seed, sizes=(1024, 2048), reps=7):
+def bench_neighbours(p, seed, sizes=(1024, 2048, 4096, 8192), reps=7):
     """The list build (conversion, neighbour search, offsets: ms_lists) at larger
-    boxes, with the cell grid (4 and 5 cells a side) and with the sweep over
-    the whole box; the lists are the same."""
+    boxes with the cell grid (tuning neighbour_cells = 2: from 4 cells a side)
+    and with the sweep over the whole box (0); the lists are the same.  The
+    default (1) takes the grid from 6 cells a side."""
     import torch
     from ccpol23_plus_2025_mbmd_amd.geometry import lattice_box
     from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
@@ -169,7 +170,7 @@ def bench_neighbours(p, seed, sizes=(1024, 2048), reps=7):
         com = torch.tensor(box.com.T.copy(), device=dev).contiguous()
         site = torch.tensor(box.site.transpose(2, 0, 1).copy(), device=dev).contiguous()
         row = {"box_A": box.box}
-        for cells in (1, 0):
+        for name, cells in (("cells", 2), ("sweep", 0), ("default", 1)):
             tb = ThreeBody(p, "fp64")
             tb.set_tuning(neighbour_cells=cells)
             lists, total = [], []
@@ -180,10 +181,10 @@ def bench_neighbours(p, seed, sizes=(1024, 2048), reps=7):
                 if r >= 2:
                     lists.append(st["ms_lists"])
                     total.append(st["ms_total"])
-            row["cells" if cells else "sweep"] = {"ms_lists": float(np.median(lists)),
-                                                  "ms_call": float(np.median(total)), "n_cells": st["n_cells"]}
+            row[name] = {"ms_lists": float(np.median(lists)), "ms_call": float(np.median(total)),
+                         "n_cells": st["n_cells"]}
         out[str(n)] = row
-        log(f"neighbours {n}: cells {row['cells']}, sweep {row['sweep']}")
+        log(f"neighbours {n}: cells {row['cells']}, sweep {row['sweep']}, default {row['default']}")
     return out
 
 

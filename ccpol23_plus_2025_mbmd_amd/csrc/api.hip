@@ -21,7 +21,6 @@ namespace ccpol3b {
 __global__ void k_prep(int, int, const double*, const double*, const double*, const double*,
                        const double*, const double*, Dev);
 __global__ void k_neighbours(Dev, Consts, int);
-__global__ void k_cells_assign(Dev);
 template <int NT> __global__ void k_cells_build(Dev);
 __global__ void k_fs3_fill(Dev, Consts);
 __global__ void k_fs2_items(Dev, Consts, int);
@@ -162,6 +161,7 @@ struct ccpol3b_ctx {
   // cell grids of the uploaded box description (cells a side, first cell)
   std::vector<int> h_cellnc, h_cell0;
   long cap_cells = 0;
+  bool cells_zero = false;  // cell_cnt holds zeros (k_cells_build leaves it so)
   hipEvent_t ev[9] = {};
   // the FS3 list build runs on a second stream beside the FS2 passes
   hipStream_t side = nullptr;
@@ -301,9 +301,10 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
 
 // Cell grids of the box description in c->h_*: a periodic box gets nc cells a
 // side, nc = floor(L / (r_max (1 + 1e-9))) with r_max the larger of the two
-// cutoffs, when that is at least CELL_MIN (with fewer the 27-cell stencil
-// covers the whole box and the sweep is cheaper), at most cbrt(8 n_box) (a few
-// molecules per cell or more) and at most 1024.  Allocates the count arrays.
+// cutoffs, at most cbrt(8 n_box) (a few molecules per cell or more) and at
+// most 1024, when nc reaches CELL_AUTO (tuning neighbour_cells = 1, the
+// default) or CELL_MIN (= 2); below that the sweep over the box is cheaper
+// than the grid's build and stencil.  Allocates the count arrays.
 int setup_cells(ccpol3b_ctx* c, int nbox) {
   Dev& D = c->D;
   const double rmax = std::sqrt(std::max(c->K.rc2_3, c->K.rc2_2));
@@ -316,7 +317,7 @@ int setup_cells(ccpol3b_ctx* c, int nbox) {
     int nc = 0;
     if (c->tune.neighbour_cells && L > 0.0 && rmax > 0.0 && nmb > 0) {
       const double f = std::min({std::floor(L / (rmax * (1.0 + 1e-9))), std::floor(std::cbrt(8.0 * nmb)), 1024.0});
-      if (f >= CELL_MIN) nc = (int)f;
+      if (f >= (c->tune.neighbour_cells == 2 ? CELL_MIN : CELL_AUTO)) nc = (int)f;
     }
     c->h_cellnc[b] = nc;
     c->h_cell0[b] = (int)ncells;
@@ -327,6 +328,7 @@ int setup_cells(ccpol3b_ctx* c, int nbox) {
     int rc;
     if ((rc = dev_alloc(c, D.cell_cnt, ncells)) || (rc = dev_alloc(c, D.cell_start, ncells + 1))) return rc;
     c->cap_cells = ncells;
+    c->cells_zero = false;
   }
   D.ncells = (int)ncells;
   return CCPOL3B_OK;
@@ -379,8 +381,6 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     c->up_boxstart = c->h_boxstart;
   }
   EVREC(0);
-  const long nprep = std::max<long>({(long)nm * ACC_W, (long)T_COUNT, (long)D.ncells});
-  hipLaunchKernelGGL(k_prep, dim3((nprep + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
   const size_t lds = fs2_lds_bytes(D.cap_nb), lds_p = fs2_pair_lds_bytes(D.cap_nb);
   const size_t lds3 = fs3_list_lds_bytes(D.cap_nb);
   // the neighbour search: FS3 staging, then the cell path's accepted entries
@@ -389,11 +389,20 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     return fail(c, CCPOL3B_ECAPACITY, "FS2 neighbour count too large for the FS2 passes");
   if (std::max(do3 ? lds3 : 0, ldsn) > 65536)
     return fail(c, CCPOL3B_ECAPACITY, "neighbour count too large for the neighbour search and the FS3 list build");
-  if (D.ncells > 0) {
-    // cell grids: each molecule's cell, then the cells' molecule ranges
-    hipLaunchKernelGGL(k_cells_assign, dim3((nm + 255) / 256), dim3(256), 0, st, D);
+  const bool cells = D.ncells > 0 && !fixed;
+  if (cells && !c->cells_zero) {
+    if (capture) return fail(c, CCPOL3B_EHIP, "cell counts not cleared before graph capture");
+    HIPCHK(c, hipMemsetAsync(D.cell_cnt, 0, sizeof(int) * (size_t)D.ncells, st));
+    c->cells_zero = true;
+  }
+  const long nprep = std::max<long>((long)nm * ACC_W, T_COUNT);
+  if (cells) c->cells_zero = false;  // until k_cells_build has run
+  hipLaunchKernelGGL(k_prep, dim3((nprep + 255) / 256), dim3(256), 0, st, nm, ld, x, y, z, xs, ys, zs, D);
+  if (cells) {
+    // cell grids: k_prep counted each cell's molecules; their ranges, then the fill
     hipLaunchKernelGGL(k_cells_build<1024>, dim3(1), dim3(1024), 0, st, D);
     LAUNCHCHK(c, "cell grids");
+    c->cells_zero = true;
   }
   // neighbour lists with the per-molecule FS3 triplet counts, then every offset
   hipLaunchKernelGGL(k_neighbours, dim3(nm), dim3(64), ldsn, st, D, c->K, do3);
@@ -840,9 +849,9 @@ int ccpol3b_get_tuning(const ccpol3b_ctx* c, ccpol3b_tuning* t) {
 
 int ccpol3b_set_tuning(ccpol3b_ctx* c, const ccpol3b_tuning* t) {
   if (!c || !t || t->fs2_parts < 1 || t->fs2_parts > FS2_PARTS_MAX || t->fs2_pair_parts < 1 ||
-      t->fs2_pair_parts > FS2_PARTS_MAX)
+      t->fs2_pair_parts > FS2_PARTS_MAX || t->neighbour_cells < 0 || t->neighbour_cells > 2)
     return fail(c, CCPOL3B_EINVAL, "invalid tuning");
-  const int cells = t->neighbour_cells ? 1 : 0;
+  const int cells = t->neighbour_cells;
   if (cells != c->tune.neighbour_cells) forget_upload(c);  // the cell grids follow the box upload
   c->tune = *t;
   c->tune.fs3_lds_coef = t->fs3_lds_coef ? 1 : 0;

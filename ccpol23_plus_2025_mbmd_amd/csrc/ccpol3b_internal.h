@@ -76,8 +76,11 @@ struct TriGeo {
 };
 
 // Smallest cell grid used: with 3 cells a side the 27-cell stencil is the
-// whole box, so the sweep over the box does the same with less work.
+// whole box, so the sweep over the box does the same with less work.  By
+// default a grid is used from CELL_AUTO cells a side, where the measured list
+// build gets faster than the sweep.
 constexpr int CELL_MIN = 4;
+constexpr int CELL_AUTO = 6;
 
 // Device arrays of one evaluation (sized by the context; see api.hip).
 struct Dev {
@@ -96,7 +99,7 @@ struct Dev {
   int ncells;             // cells of all boxes
   int *cell_nc;           // [nbox] cells a side, 0: no grid
   int *cell0;             // [nbox] first cell of the box
-  int *cell_cnt;          // [ncells] molecules per cell (zeroed by k_prep, back to 0 after the fill)
+  int *cell_cnt;          // [ncells] molecules per cell (counted by k_prep, back to 0 after the fill)
   int *cell_start;        // [ncells+1] first entry of each cell in cell_mol
   int *cell_mol;          // [nm] molecules grouped by cell
   int *mol_cell;          // [nm] cell of each molecule (-1: no grid)

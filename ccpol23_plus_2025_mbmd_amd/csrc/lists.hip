@@ -39,31 +39,6 @@ __device__ __forceinline__ int upper_start(const int* list, int n, int v) {
   return lo;
 }
 
-// ------------------------------------------------------------------ prep
-// COM (A) -> bohr and site offsets (A, Fortran order xs(is,i)) -> [m][s][c] bohr;
-// zero the force accumulators and the counters of the call.
-__global__ void k_prep(int nm, int ld, const double* __restrict__ x, const double* __restrict__ y,
-                       const double* __restrict__ z, const double* __restrict__ xs,
-                       const double* __restrict__ ys, const double* __restrict__ zs, Dev D) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nm) {
-    D.cx[t] = x[t] / BOHR2A;
-    D.cy[t] = y[t] / BOHR2A;
-    D.cz[t] = z[t] / BOHR2A;
-  }
-  if (t < nm * NS3) {
-    const int m = t / NS3, s = t - m * NS3;
-    const long src = s + (long)ld * m;
-    double* dst = D.site + (long)t * 3;
-    dst[0] = xs[src] / BOHR2A;
-    dst[1] = ys[src] / BOHR2A;
-    dst[2] = zs[src] / BOHR2A;
-  }
-  if (t < nm * ACC_W) D.acc[2 * t] = D.acc[2 * t + 1] = 0ull;
-  if (t < T_COUNT) D.totals[t] = 0;
-  if (t < D.ncells) D.cell_cnt[t] = 0;
-}
-
 // ------------------------------------------------------------------ cell grids
 // Cell of coordinate x (bohr) along one side: the position wrapped into [0, L)
 // over the cell side L/nc.  Rounding can only move a molecule lying on a cell
@@ -76,20 +51,41 @@ __device__ __forceinline__ int cell_coord(double x, double L, int nc) {
   return q < 0 ? 0 : (q >= nc ? nc - 1 : q);
 }
 
-// Thread per molecule: its cell (or -1 when its box has no grid) and the cell
-// occupancy counts.
-__global__ void k_cells_assign(Dev D) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= D.nm) return;
-  const int b = D.mol_box[m], nc = D.cell_nc[b];
-  int cc = -1;
-  if (nc >= CELL_MIN) {
-    const double L = D.boxL[b];
-    cc = D.cell0[b] + (cell_coord(D.cx[m], L, nc) * nc + cell_coord(D.cy[m], L, nc)) * nc +
-         cell_coord(D.cz[m], L, nc);
-    atomicAdd(D.cell_cnt + cc, 1);
+// ------------------------------------------------------------------ prep
+// COM (A) -> bohr and site offsets (A, Fortran order xs(is,i)) -> [m][s][c] bohr;
+// zero the force accumulators and the counters of the call; each molecule's
+// cell (or -1 when its box has no grid) and the cell occupancy counts, which
+// k_cells_build leaves at zero for the next call.
+__global__ void k_prep(int nm, int ld, const double* __restrict__ x, const double* __restrict__ y,
+                       const double* __restrict__ z, const double* __restrict__ xs,
+                       const double* __restrict__ ys, const double* __restrict__ zs, Dev D) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nm) {
+    const double cx = x[t] / BOHR2A, cy = y[t] / BOHR2A, cz = z[t] / BOHR2A;
+    D.cx[t] = cx;
+    D.cy[t] = cy;
+    D.cz[t] = cz;
+    if (D.ncells > 0) {
+      const int b = D.mol_box[t], nc = D.cell_nc[b];
+      int cc = -1;
+      if (nc >= CELL_MIN) {
+        const double L = D.boxL[b];
+        cc = D.cell0[b] + (cell_coord(cx, L, nc) * nc + cell_coord(cy, L, nc)) * nc + cell_coord(cz, L, nc);
+        atomicAdd(D.cell_cnt + cc, 1);
+      }
+      D.mol_cell[t] = cc;
+    }
   }
-  D.mol_cell[m] = cc;
+  if (t < nm * NS3) {
+    const int m = t / NS3, s = t - m * NS3;
+    const long src = s + (long)ld * m;
+    double* dst = D.site + (long)t * 3;
+    dst[0] = xs[src] / BOHR2A;
+    dst[1] = ys[src] / BOHR2A;
+    dst[2] = zs[src] / BOHR2A;
+  }
+  if (t < nm * ACC_W) D.acc[2 * t] = D.acc[2 * t + 1] = 0ull;
+  if (t < T_COUNT) D.totals[t] = 0;
 }
 
 // ------------------------------------------------------------------ FS3 triplet sweep
@@ -493,8 +489,8 @@ __device__ void block_exclusive_scan(int* buf, const int* const (&cnt)[K], int* 
 
 // One block: exclusive scan of the cell counts into cell_start, then every
 // molecule into its cell's range of cell_mol (order within a cell is not
-// used: the neighbour search ranks what it accepts).  Counting the cell down
-// leaves cell_cnt at zero.
+// used: the neighbour search ranks what it accepts).  Counting each cell down
+// leaves cell_cnt at zero, as k_prep of the next call expects.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_cells_build(Dev D) {
   __shared__ int buf[SCAN_BUF];

@@ -1,11 +1,13 @@
 """Cell-grid neighbour search (periodic boxes of at least 4 cells a side, each
-side at least the FS2 cutoff of 7.5 A): it must give the lists of the sweep
-over the whole box, which are the reference's (threebody_potential.f90
-:297-336, 859-892), so every list, energy and force equals the sweep's bit for
-bit.  Cases: 1,024 waters (4 cells a side), 2,048 waters (5), 1,024 waters
-with the molecules moved by whole box lengths in and out of [0, L) (cell
-assignment wraps positions), and 256 waters at 1/4 density (4 cells a side)
-against the oracle's lists."""
+side at least the FS2 cutoff of 7.5 A; tuning neighbour_cells = 2 forces the
+grid from 4 cells a side, the default takes it from 6): it must give the
+lists of the sweep over the whole box, which are the reference's
+(threebody_potential.f90:297-336, 859-892), so every list, energy and force
+equals the sweep's bit for bit.  Cases: 1,024 waters (4 cells a side), 2,048
+waters (5), 1,024 waters with the molecules moved by whole box lengths in and
+out of [0, L) (cell assignment wraps positions), 256 waters at 1/4 density (4
+cells a side) against the oracle's lists, and 4,096 waters (6 cells a side)
+with the default tuning."""
 import numpy as np
 import pytest
 
@@ -30,8 +32,8 @@ def cases():
     b256 = lattice_box(256, seed=43)
     s = 4.0 ** (1 / 3)
     dilute = WaterBox(b256.com * s, b256.quats, b256.box * s)
-    return {"1024": (b1024, 4), "2048": (lattice_box(2048, seed=44), 5), "1024_shifted": (shifted, 4),
-            "256_dilute": (dilute, 4)}
+    return {"1024": (b1024, 4, 2), "2048": (lattice_box(2048, seed=44), 5, 2), "1024_shifted": (shifted, 4, 2),
+            "256_dilute": (dilute, 4, 2), "4096_default": (lattice_box(4096, seed=46), 6, 1)}
 
 
 CASES = cases()
@@ -39,9 +41,9 @@ CASES = cases()
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_cells_match_sweep(params_fs2fs3, oracle, name):
-    box, nc = CASES[name]
+    box, nc, tuning = CASES[name]
     print(f"cell lists: {name} ({box.nm} waters, L = {box.box:.2f} A, {nc} cells a side)", flush=True)
-    got, st, l3, l2 = evaluate(params_fs2fs3, box, 1)
+    got, st, l3, l2 = evaluate(params_fs2fs3, box, tuning)
     ref, st0, r3, r2 = evaluate(params_fs2fs3, box, 0)
     assert st["n_cells"] == nc ** 3 and st0["n_cells"] == 0
     print(f"lists: {st['ms_lists']:.3f} ms with cells, {st0['ms_lists']:.3f} ms sweeping; "
@@ -59,6 +61,9 @@ def test_cells_match_sweep(params_fs2fs3, oracle, name):
 
 
 def test_cells_small_box_sweeps(params_fs2fs3):
-    """512 waters (L = 24.8 A, 3 cells a side): no grid, the sweep runs."""
-    _, st, _, _ = evaluate(params_fs2fs3, lattice_box(512, seed=45), 1)
+    """512 waters (L = 24.8 A, 3 cells a side): no grid even when forced; 1,024
+    waters (4 cells a side): no grid by default."""
+    _, st, _, _ = evaluate(params_fs2fs3, lattice_box(512, seed=45), 2)
+    assert st["n_cells"] == 0 and st["n_fs3"] > 0
+    _, st, _, _ = evaluate(params_fs2fs3, lattice_box(1024, seed=45), 1)
     assert st["n_cells"] == 0 and st["n_fs3"] > 0
