@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import numpy as np
 import torch
+from torch.autograd.function import once_differentiable
 
 from .threebody import NLIN, ThreeBody
 
@@ -41,13 +42,22 @@ class ThreeBodyEnergy(torch.autograd.Function):
     com [nm, 3] and site [nm, ns>=17, 3] (A, float64, cuda); coef [728] the SR
     then LR linear coefficients (cs3_s, cs3_l) used for this call; tb a
     ThreeBody context (its precision decides FP64 or mixed); mol_start [nbox+1]
-    and box_len [nbox] (A, <= 0 open) describe the packed boxes."""
+    and box_len [nbox] (A, <= 0 open) describe the packed boxes.
+
+    The coefficients are loaded into tb, which keeps them for its later calls;
+    they are copied to the host (a stream synchronisation) only when coef is
+    not the tensor of tb's previous load or has been modified since.  The
+    gradients are first derivatives only (the adjoint is the kernels' analytic
+    force and dE/dc, not a differentiable graph): differentiating them again
+    raises."""
 
     @staticmethod
     def forward(ctx, com, site, coef, tb: ThreeBody, mol_start, box_len):
         nm = com.shape[0]
-        coef_np = coef.detach().to("cpu", torch.float64).numpy()
-        tb.set_coefficients(coef_np[:NLIN], coef_np[NLIN:])
+        if not (coef is tb._coef_ref and coef._version == tb._coef_version):
+            coef_np = coef.detach().to("cpu", torch.float64).numpy()
+            tb.set_coefficients(coef_np[:NLIN], coef_np[NLIN:])
+            tb._coef_ref, tb._coef_version = coef, coef._version
         com_t = com.detach().t().contiguous()
         ns = site.shape[1]
         site_t = site.detach().permute(2, 0, 1).contiguous()
@@ -68,6 +78,7 @@ class ThreeBodyEnergy(torch.autograd.Function):
         return en[:, 0] + en[:, 1]
 
     @staticmethod
+    @once_differentiable
     def backward(ctx, gE):
         dcom, dsite, mol_box, phi = ctx.saved_tensors
         g = gE[mol_box]

@@ -287,31 +287,24 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     if constexpr (PHI)
       for (int q = tid; q < NCOEF; q += 256) sDC[q] = T(0);
     if (rr == 0) __syncthreads();  // sP and g
-    // ---------------- pair stage: e, e*r and d ln e / dr for the 867 site pairs.
-    // Thread tid takes p = tid + 256 k.  In the mixed kernels (blk, x, y) of p
-    // advance incrementally (256 = 15 * 17 + 1), and the distances of the first
-    // region are reused from sR in the second; the FP64 kernels divide and
-    // recompute (fewer live registers).
+    // ---------------- pair stage: e, e*r and d ln e / dr for the 867 site pairs
+    // (the 17 x 17 blocks ij, ik, jk).  Thread tid < 256 takes site pair
+    // q = tid of each block: one load of the pair parameters serves the three
+    // blocks (they depend on the two site indices only), and both sites are
+    // fixed offsets from two per-thread bases.  The last 33 pairs of each block
+    // go to the first 99 threads, one each.  The mixed kernels reuse the first
+    // region's distances from sR in the second.
     {
-      int blk = 0, x = tid / NS3, y = tid - (tid / NS3) * NS3;
-      for (int p = tid; p < NPAIR3; p += 256) {
-        if constexpr (sizeof(T) == 8) {  // FP64: fewer live registers with the divisions
-          blk = p / NPAIR;
-          x = (p - blk * NPAIR) / NS3;
-          y = p - blk * NPAIR - x * NS3;
-        }
-        const int rem = x * NS3 + y;
+      using PP = typename PairPar<T>::type;
+      auto site_pair = [&](const double* pa, const double* pb, const PP& par, int p) {
         T rt;
         if (rr == 0 || sizeof(T) == 8) {  // FP64: recomputing costs fewer registers than the branch
-          const double* pa = &sP[0][0][0] + __umul24(blk == 2 ? NS3 + x : x, 3);
-          const double* pb = &sP[0][0][0] + __umul24((blk == 0 ? NS3 : 2 * NS3) + y, 3);
           const double vx = pa[0] - pb[0], vy = pa[1] - pb[1], vz = pa[2] - pb[2];
           rt = t_dist<T>(vx * vx + vy * vy + vz * vz);
           if (rr == 0) sR[p] = rt;
         } else {
           rt = sR[p];
         }
-        const auto par = pairpar[X * NPAIR + rem];
         const T tg = t_exp<T>(-T(par.y) * (rt - T(par.z)));
         const T damp = t_rcp<T>(T(1) + tg);
         const T e = t_exp<T>(-T(par.x) * rt) * damp;
@@ -320,18 +313,28 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
         v.y = e * (rt - T(VSHIFT));
         sV[p] = v;
         sDer[p] = -T(par.x) + T(par.y) * tg * damp;
-        if constexpr (sizeof(T) == 4) {
-          y += 1;
-          x += 15;
-          if (y >= NS3) {
-            y -= NS3;
-            ++x;
-          }
-          if (x >= NS3) {
-            x -= NS3;
-            ++blk;
-          }
-        }
+      };
+      // the indices come from an opaque copy of tid: hoisted out of the region
+      // loop they would hold registers through the triple stage
+      int tp = tid;
+      asm volatile("" : "+v"(tp));
+      const double* s0 = &sP[0][0][0];
+      {
+        const int x = tp / NS3, y = tp - x * NS3;
+        const PP par = pairpar[X * NPAIR + tp];
+        const double* pa = s0 + 3 * x;
+        const double* pb = s0 + 3 * y;
+        // blocks ij (site x of i, site y of j), ik, jk
+#pragma unroll 1
+        for (int blk = 0; blk < 3; ++blk)
+          site_pair(pa + (blk == 2 ? 3 * NS3 : 0), pb + (blk == 0 ? 3 * NS3 : 6 * NS3), par, blk * NPAIR + tp);
+      }
+      constexpr int TAIL = NPAIR - 256;
+      if (tp < 3 * TAIL) {
+        const int blk = tp / TAIL, q = 256 + tp - blk * TAIL;
+        const int x = q / NS3, y = q - x * NS3;
+        site_pair(s0 + 3 * (blk == 2 ? NS3 + x : x), s0 + 3 * ((blk == 0 ? NS3 : 2 * NS3) + y),
+                  pairpar[X * NPAIR + q], blk * NPAIR + q);
       }
     }
     __syncthreads();

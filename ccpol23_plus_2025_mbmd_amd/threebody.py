@@ -55,6 +55,7 @@ class ThreeBody:
         _lib.check(L.ccpol3b_create(C.byref(p), C.byref(ctx)), None, "ccpol3b_create")
         self._ctx = ctx
         self.precision = precision
+        self._coef_ref, self._coef_version = None, None  # the last tensor loaded by ThreeBodyEnergy
 
     def close(self):
         if getattr(self, "_ctx", None):
@@ -202,6 +203,7 @@ class ThreeBody:
         assert a.size == NLIN and b.size == NLIN
         _lib.check(_lib.load().ccpol3b_set_coefficients(self._ctx, _dp(a), _dp(b)), self._ctx,
                    "ccpol3b_set_coefficients")
+        self._coef_ref, self._coef_version = None, None  # the last tensor loaded by ThreeBodyEnergy
 
     def gram(self, phi, stream=None):
         """G = phi^T phi (float64 [ncol, ncol]) of a float64 cuda matrix phi [nrow, ncol]

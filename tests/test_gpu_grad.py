@@ -395,3 +395,34 @@ def test_mixed_gradient_against_fd_of_fp64_energy(params_fs2fs3, trimers):
             worst = max(worst, abs(fd - float(g_quat[m] @ dq)) / scale)
     print(f"largest difference / largest component: {worst:.2e}", flush=True)
     assert worst <= 3e-4, worst
+
+
+def test_autograd_reuses_loaded_coefficients(params_fs2fs3, trimers):
+    """The same coefficient tensor in consecutive forwards is loaded once (no
+    host copy); a modified or different tensor is loaded again and changes the
+    energy; a second derivative raises (the adjoint is first order only)."""
+    from ccpol23_plus_2025_mbmd_amd.autograd import threebody_energy
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    tb = ThreeBody(params_fs2fs3, "fp64")
+    dev = torch.device("cuda:0")
+    sel = trimers[:6]
+    com = torch.tensor(np.concatenate([b.com for b in sel]), device=dev, requires_grad=True)
+    quat = torch.tensor(np.concatenate([b.quats for b in sel]), device=dev)
+    ms = np.cumsum([0] + [3] * len(sel)).astype(np.int32)
+    bl = np.zeros(len(sel))
+    pc = torch.tensor(principal_coords()[1], device=dev)
+    coef = torch.tensor(coef_vector(params_fs2fs3), device=dev)
+    print("autograd coefficient reuse", flush=True)
+    e1 = threebody_energy(tb, com, quat, coef, pc, ms, bl)
+    calls = []
+    orig = tb.set_coefficients
+    tb.set_coefficients = lambda a, b: (calls.append(1), orig(a, b))
+    e2 = threebody_energy(tb, com, quat, coef, pc, ms, bl)
+    assert not calls and torch.equal(e1, e2)
+    with torch.no_grad():
+        coef[:NLIN] *= 2.0
+    e3 = threebody_energy(tb, com, quat, coef, pc, ms, bl)
+    assert len(calls) == 1 and not torch.equal(e3, e1)
+    g, = torch.autograd.grad(e3.sum(), com, create_graph=True)
+    with pytest.raises(RuntimeError):
+        torch.autograd.grad(g.sum(), com)
