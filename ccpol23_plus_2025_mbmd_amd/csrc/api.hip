@@ -35,8 +35,8 @@ __global__ void k_phi_boxes(Dev, const double*, double*);
 int set_orbit_tables(const short* off, const short* mem);
 __global__ void k_gram_f64(int, int, const double*, int, int, double*);
 __global__ void k_gram_sum(int, int, const double*, double*, int);
-__global__ void k_fs2_pair_pass(Dev, Consts);
-__global__ void k_fs2_spectator_pass(Dev, Consts);
+template <bool MIXED> __global__ void k_fs2_pair_pass(Dev, Consts);
+template <bool MIXED> __global__ void k_fs2_spectator_pass(Dev, Consts);
 size_t fs2_lds_bytes(int cap_nb);
 size_t fs2_pair_lds_bytes(int cap_nb);
 size_t fs3_list_lds_bytes(int cap_nb);
@@ -421,8 +421,13 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   HIPCHK(c, hipStreamWaitEvent(c->side, c->fork, 0));
   if (do2) {
     // pair pass (S_ij, pair-owned forces) before the spectator pass that reads S_ij
-    hipLaunchKernelGGL(k_fs2_pair_pass, dim3(nm * D.fs2_pair_parts), dim3(256), lds_p, st, D, c->K);
-    hipLaunchKernelGGL(k_fs2_spectator_pass, dim3(nm * D.fs2_parts), dim3(256), lds, st, D, c->K);
+    if (c->precision == CCPOL3B_MIXED) {
+      hipLaunchKernelGGL(k_fs2_pair_pass<true>, dim3(nm * D.fs2_pair_parts), dim3(256), lds_p, st, D, c->K);
+      hipLaunchKernelGGL(k_fs2_spectator_pass<true>, dim3(nm * D.fs2_parts), dim3(256), lds, st, D, c->K);
+    } else {
+      hipLaunchKernelGGL(k_fs2_pair_pass<false>, dim3(nm * D.fs2_pair_parts), dim3(256), lds_p, st, D, c->K);
+      hipLaunchKernelGGL(k_fs2_spectator_pass<false>, dim3(nm * D.fs2_parts), dim3(256), lds, st, D, c->K);
+    }
     LAUNCHCHK(c, "FS2");
   }
   EVREC(2);

@@ -103,6 +103,78 @@ __device__ __forceinline__ double fs2_T(const Fs2Geo& G, const double* __restric
   return T;
 }
 
+// The same T_k, site and COM derivatives with the charge-site distances in
+// FP32 (the mixed precision mode).  The two inverse distances of a charge site
+// to a molecule centre and to its exchange charge nearly cancel (the exchange
+// charges sit g = 0.4 bohr from the centres), so each pair difference is formed
+// without a subtraction: with c the vector from the centre, w the shift and
+// q = c -/+ w, delta = |q|^2 - |c|^2 = g^2 -/+ 2 c.w and h = 1 / (|c| + |q|),
+//   1/|c|   - 1/|q|   = delta h / (|c| |q|),
+//   1/|c|^3 - 1/|q|^3 = delta h (|c|^2 + |c||q| + |q|^2) / (|c|^3 |q|^3).
+// The displacements (FP64 sums) are rounded to FP32 once per site; the sums
+// over sites, and everything outside T, stay FP64.
+template <bool WANT_SITES, bool WANT_COM>
+__device__ __forceinline__ double fs2_T_mixed(const Fs2Geo& G, const double* __restrict__ sk, double* Fk,
+                                              double mult, double* dTdRi, double* dTdRj, double& Tl) {
+  const double iRij = rcp_nr(G.Rij);
+  const float u[3] = {float(G.xij[0] * iRij), float(G.xij[1] * iRij), float(G.xij[2] * iRij)};
+  const float g = float(G3), g2 = g * g;  // G3 is a float value, so g is exact
+  const float w[3] = {g * u[0], g * u[1], g * u[2]};
+  const float gR = float(G3 * iRij);
+  double T = 0.0;
+  Tl = 0.0;
+  if constexpr (WANT_COM)
+    for (int c = 0; c < 3; ++c) dTdRi[c] = dTdRj[c] = 0.0;
+#pragma unroll WANT_SITES ? NS2 : 1
+  for (int ks = 0; ks < NS2; ++ks) {
+    const float q = float(c_q2[ks]);
+    float ci[3], cj[3];
+    for (int c = 0; c < 3; ++c) {
+      ci[c] = float(G.xki[c] + sk[ks * 3 + c]);
+      cj[c] = float(G.xkj[c] + sk[ks * 3 + c]);
+    }
+    const float ci2 = ci[0] * ci[0] + ci[1] * ci[1] + ci[2] * ci[2];
+    const float cj2 = cj[0] * cj[0] + cj[1] * cj[1] + cj[2] * cj[2];
+    const float cui = ci[0] * u[0] + ci[1] * u[1] + ci[2] * u[2];
+    const float cuj = cj[0] * u[0] + cj[1] * u[1] + cj[2] * u[2];
+    const float di = g2 - 2.0f * g * cui;  // |q_i|^2 - |c_i|^2, q_i = c_i - w
+    const float dj = g2 + 2.0f * g * cuj;  // |q_j|^2 - |c_j|^2, q_j = c_j + w
+    const float qi2 = ci2 + di, qj2 = cj2 + dj;
+    const float ici = __builtin_amdgcn_rsqf(ci2), icj = __builtin_amdgcn_rsqf(cj2);
+    const float iqi = __builtin_amdgcn_rsqf(qi2), iqj = __builtin_amdgcn_rsqf(qj2);
+    const float rci = ci2 * ici, rcj = cj2 * icj, rqi = qi2 * iqi, rqj = qj2 * iqj;
+    const float hi = __builtin_amdgcn_rcpf(rci + rqi), hj = __builtin_amdgcn_rcpf(rcj + rqj);
+    const float term0 = di * hi * ici * iqi + dj * hj * icj * iqj;  // 1/|c_i| - 1/|q_i| + 1/|c_j| - 1/|q_j|
+    const float iqi3 = iqi * iqi * iqi, iqj3 = iqj * iqj * iqj;
+    const float pqi = cui - g, pqj = cuj + g;  // q_i.u, q_j.u
+    T += double(q * term0);
+    Tl += double(-q * term0 + g * q * (pqi * iqi3 - pqj * iqj3));
+    if constexpr (WANT_SITES || WANT_COM) {
+      const float Bi = di * hi * (ci2 + rci * rqi + qi2) * (ici * ici * ici) * iqi3;  // 1/|c_i|^3 - 1/|q_i|^3
+      const float Bj = dj * hj * (cj2 + rcj * rqj + qj2) * (icj * icj * icj) * iqj3;
+      if constexpr (WANT_SITES)
+        for (int c = 0; c < 3; ++c)
+          Fk[ks * 3 + c] += mult * double(q * (-ci[c] * Bi - cj[c] * Bj + w[c] * (iqj3 - iqi3)));
+      if constexpr (WANT_COM)
+        for (int c = 0; c < 3; ++c) {
+          const float Pqi = gR * (ci[c] - w[c] - pqi * u[c]);
+          const float Pqj = gR * (cj[c] + w[c] - pqj * u[c]);
+          const float x = q * (Pqj * iqj3 - Pqi * iqi3);
+          dTdRi[c] += double(q * (ci[c] * Bi + w[c] * iqi3) + x);
+          dTdRj[c] += double(q * (cj[c] * Bj - w[c] * iqj3) - x);
+        }
+    }
+  }
+  return T;
+}
+
+template <bool MIXED, bool WANT_SITES, bool WANT_COM>
+__device__ __forceinline__ double fs2_T_in(const Fs2Geo& G, const double* __restrict__ sk, double* Fk, double mult,
+                                           double* dTdRi, double* dTdRj, double& Tl) {
+  if constexpr (MIXED) return fs2_T_mixed<WANT_SITES, WANT_COM>(G, sk, Fk, mult, dTdRi, dTdRj, Tl);
+  else return fs2_T<WANT_SITES, WANT_COM>(G, sk, Fk, mult, dTdRi, dTdRj, Tl);
+}
+
 // Accepted candidates queued per block before they are evaluated.
 constexpr int FS2_QCAP = 1024;
 // The spectator pass takes FS2_SU candidates per thread at a time (their
@@ -128,6 +200,7 @@ size_t fs2_pair_lds_bytes(int cap_nb) {
 // the block keeps S_ij (:361-372), sum_k f f T (for the exchange-site forces
 // F = -(sum_k f f T) dS/ds, :477-519) and the COM force on j in LDS; i's COM
 // force stays in registers until the end.
+template <bool MIXED>
 __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
   extern __shared__ double dsm[];
   __shared__ int wcnt[4];
@@ -235,7 +308,8 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
       const double iRkj = rsqrt_nr(r2kj);
       G.Rkj = r2kj * iRkj;
       double dTdRi[3], dTdRj[3], Tl;
-      const double T = fs2_T<false, true>(G, D.site + (long)snb[a] * NS3 * 3, nullptr, 0.0, dTdRi, dTdRj, Tl);
+      const double T =
+          fs2_T_in<MIXED, false, true>(G, D.site + (long)snb[a] * NS3 * 3, nullptr, 0.0, dTdRi, dTdRj, Tl);
       const double fki = sF[a], fpki = sFp[a];
       double fkj, fpkj;
       fs2_damp(G.Rkj, K, fkj, fpkj);
@@ -300,6 +374,7 @@ __global__ __launch_bounds__(256) void k_fs2_pair_pass(Dev D, Consts K) {
 // time, 256 apart, and appends the accepted ones in a fixed order (thread by
 // thread, each in its own candidate order); every thread then sums its queue
 // entries in order, so the energy and virial sums are the same run to run.
+template <bool MIXED>
 __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
   extern __shared__ double dsm[];
   __shared__ int wtot[4];
@@ -452,7 +527,7 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
       fs2_damp(G.Rkj, K, fkj, fpkj);
       const double ff = fki * fkj, ffS = ff * S;
       double Tl;
-      const double T = fs2_T<true, false>(G, skp, Fk, -ffS, nullptr, nullptr, Tl);
+      const double T = fs2_T_in<MIXED, true, false>(G, skp, Fk, -ffS, nullptr, nullptr, Tl);
       const double ci = S * T * fpki * fkj * rcp_nr(G.Rki), cj = S * T * fki * fpkj * iRkj;
       for (int c = 0; c < 3; ++c) Fk[NS2 * 3 + c] += -(ci * G.xki[c] + cj * G.xkj[c]);
       E += ff * S * T;
@@ -488,5 +563,10 @@ __global__ __launch_bounds__(256) void k_fs2_spectator_pass(Dev D, Consts K) {
     else if (v > 0.0) atomicAdd(D.totals + T_NI2, (int)v);
   }
 }
+
+template __global__ void k_fs2_pair_pass<false>(Dev, Consts);
+template __global__ void k_fs2_pair_pass<true>(Dev, Consts);
+template __global__ void k_fs2_spectator_pass<false>(Dev, Consts);
+template __global__ void k_fs2_spectator_pass<true>(Dev, Consts);
 
 }  // namespace ccpol3b

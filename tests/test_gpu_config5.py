@@ -12,7 +12,7 @@ launch shapes from the autotuner.
   (translation invariance of E).  In FP64 and in mixed precision.
 * Eight small uneven boxes (64-128 waters) in one batch, box by box against
   the oracle: the FP64 parity bar, and the mixed path's bar (site forces 1e-5)
-  with its measured envelopes on energies and COM forces.
+  with its measured envelopes on energies (FS3 and FS2) and COM forces.
 """
 import numpy as np
 import pytest
@@ -143,5 +143,5 @@ def test_small_uneven_batch_against_oracle(params_fs2fs3, oracle, precision):
             assert err_c3 <= 1e-9
         else:
             assert err_f <= 1e-5 and err_all <= 1e-5
-            assert err_e2 <= 1e-10  # FS2 is FP64 in both modes
+            assert err_e2 <= 2e-4  # FS2 with FP32 charge-site distances: measured <= 3.4e-5
             assert err_e3 <= 3e-3 and err_c3 <= 4e-3  # measured envelopes, not a bar

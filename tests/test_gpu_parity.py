@@ -275,8 +275,11 @@ def test_mixed_precision_error_512(params_fs2fs3, oracle):
     err_m = rel_rms(molecular(got), molecular(ref))
     err_t = rel_rms(torque(got), torque(ref))
     vir_err = abs(got["vir"] - ref["vir"]) / abs(ref["vir"])
+    err_e2 = abs(got["qfs2"] - ref["qfs2"]) / abs(ref["qfs2"])
+    err_c2 = rel_rms(com_forces(got, "fs2"), com_forces(ref, "fs2"))
     print(f"mixed vs fp64: site force rms {err_f:.3e}, all forces {err_all:.3e}, FS3 COM {err_c:.3e}, "
-          f"molecular force {err_m:.3e}, torque {err_t:.3e}, FS3 energy {err_e:.3e}, virial {vir_err:.3e}")
+          f"molecular force {err_m:.3e}, torque {err_t:.3e}, FS3 energy {err_e:.3e}, virial {vir_err:.3e}, "
+          f"FS2 energy {err_e2:.3e}, FS2 COM {err_c2:.3e}")
     # The 1e-5 force bar holds for the site forces (and the full output vector,
     # which they dominate).
     assert err_f <= 1e-5
@@ -295,6 +298,32 @@ def test_mixed_precision_error_512(params_fs2fs3, oracle):
     # the virial sums r dE/dr over site pairs and COM vectors; in a periodic
     # 512-water box its FP32 error stays at the energy's level
     assert vir_err <= 5e-3
+    # FS2 in mixed precision (FP32 charge-site distances, cancellation-free
+    # pair differences): measured energy 7.8e-6, COM 8.7e-7 on this box
+    assert err_e2 <= 1e-4 and err_c2 <= 1e-5
+
+
+def test_fs2_mixed_against_oracle(params_fs2fs3, oracle):
+    """FS2 alone (FS3 off) in mixed precision against the oracle at 256 and 512
+    waters: energy, site forces (sites 1-5), COM forces and virial; the FP32
+    charge-site distances enter T_k through cancellation-free pair differences.
+    Measured: energy 4.5e-6 / 6.0e-5, site forces 3.2e-7 / 3.5e-7, COM forces
+    7.1e-7 / 8.5e-7, virial 6.0e-6 / 1.3e-5 (256 / 512 waters)."""
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    p = params_fs2fs3.replace(s3_term=False)
+    tb = ThreeBody(p, "mixed")
+    for n, seed in ((256, 61), (512, 62)):
+        box = lattice_box(n, seed=seed)
+        got = tb.evaluate(box)
+        ref = oracle.threebody_term(p, box, nthreads=NTHREADS)
+        err_e = abs(got["qfs2"] - ref["qfs2"]) / abs(ref["qfs2"])
+        err_f = rel_rms(site_forces(got, 5), site_forces(ref, 5))
+        err_c = rel_rms(com_forces(got, "fs2"), com_forces(ref, "fs2"))
+        err_v = abs(got["vir"] - ref["vir"]) / abs(ref["vir"])
+        print(f"FS2 mixed {n} waters: energy {err_e:.2e}, site forces {err_f:.2e}, COM {err_c:.2e}, "
+              f"virial {err_v:.2e}", flush=True)
+        assert got["n_fs2_items"] == ref["n_fs2"]
+        assert err_f <= 2e-6 and err_c <= 5e-6 and err_e <= 2e-4 and err_v <= 1e-4
 
 
 def test_batch_matches_single(params_fs2fs3):
