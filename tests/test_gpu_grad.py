@@ -286,7 +286,7 @@ def test_4096_trimers_gradient_path(params_fs2fs3, precision):
     each trimer (E is linear in the coefficients) and dE/dc = sum_t w_t phi_t;
     the COM gradients of each trimer sum to zero (translation invariance) and
     the derivative under a rigid rotation of each trimer vanishes (rotation
-    invariance); the Gram matrix of the 4096 x 728 coefficient gradients on the
+    invariance), to FP64 rounding (mixed: to FP32 rounding); the Gram matrix of the 4096 x 728 coefficient gradients on the
     matrix cores against FP64 torch.  Mixed precision also against the FP64
     gradients (measured 6.4e-5 COM, 5.2e-5 orientation, 2.7e-8 coefficients;
     asserted at 2e-4, 2e-4, 1e-7)."""
@@ -325,13 +325,18 @@ def test_4096_trimers_gradient_path(params_fs2fs3, precision):
     assert np.all(lin <= (1e-12 if precision == "fp64" else 1e-5) * scale_e + 1e-15), (lin / scale_e).max()
     wg = w.cpu().numpy() @ phi_h
     assert np.all(np.abs(g_coef - wg) <= 1e-12 * (w.cpu().numpy() @ np.abs(phi_h)) + 1e-300)
+    # FP64: invariance to rounding.  Mixed: FS2's site forces on the spectator and
+    # COM forces on the pair come from FP32 derivatives formed by different
+    # threads, so their sum vanishes to FP32 rounding of the terms (measured
+    # 1.0e-5 of the summed magnitudes)
+    inv_tol = 1e-9 if precision == "fp64" else 1e-4
     tsum = np.abs(g_com.reshape(-1, 3, 3).sum(1))
     tmag = np.abs(g_com.reshape(-1, 3, 3)).sum(1)
     print(f"translation residual max {(tsum / (tmag + 1e-300)).max():.1e}", flush=True)
-    assert np.all(tsum <= 1e-9 * tmag + 1e-15)
+    assert np.all(tsum <= inv_tol * tmag + 1e-15)
     res, mag = rotation_residual(com0, g_com, q0, g_quat)
     print(f"rotation residual max {(np.abs(res) / (mag + 1e-300)).max():.1e}", flush=True)
-    assert np.all(np.abs(res) <= 1e-9 * mag + 1e-15)
+    assert np.all(np.abs(res) <= inv_tol * mag + 1e-15)
     # the Gram of this phi on the matrix cores
     g = tb.gram(phi)
     ref = phi.T @ phi
