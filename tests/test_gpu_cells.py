@@ -67,3 +67,32 @@ def test_cells_small_box_sweeps(params_fs2fs3):
     assert st["n_cells"] == 0 and st["n_fs3"] > 0
     _, st, _, _ = evaluate(params_fs2fs3, lattice_box(1024, seed=45), 1)
     assert st["n_cells"] == 0 and st["n_fs3"] > 0
+
+
+def test_cells_in_a_batch_of_uneven_boxes(params_fs2fs3):
+    """The 96 uneven boxes (64-1024 waters) in one batched call with the grid
+    forced from 4 cells a side: the boxes of more than ~900 waters get a grid
+    (each with its own range of cells), the others are swept; every box's
+    energies, forces and list counts equal the all-sweep call bit for bit."""
+    import torch
+    from ccpol23_plus_2025_mbmd_amd.geometry import uneven_boxes
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    dev = torch.device("cuda:0")
+    boxes = uneven_boxes(96, seed=2025)
+    ms = np.cumsum([0] + [b.nm for b in boxes]).astype(np.int32)
+    bl = np.array([b.box for b in boxes], float)
+    com = torch.tensor(np.concatenate([b.com for b in boxes]).T.copy(), device=dev).contiguous()
+    site = torch.tensor(np.concatenate([b.site for b in boxes]).transpose(2, 0, 1).copy(), device=dev).contiguous()
+    out = {}
+    for cells in (2, 0):
+        tb = ThreeBody(params_fs2fs3, "fp64")
+        tb.set_tuning(neighbour_cells=cells)
+        res = [t.cpu().numpy() for t in tb.eval_batch_device(ms, bl, com, site)]
+        out[cells] = (res, tb.stats())
+    (r2, st2), (r0, st0) = out[2], out[0]
+    n_grid = sum(1 for b in boxes if np.floor(b.box / (7.5 * (1 + 1e-9))) >= 4)
+    print(f"batch: {n_grid} of 96 boxes with a grid, {st2['n_cells']} cells", flush=True)
+    assert n_grid >= 3 and st2["n_cells"] >= 64 * n_grid and st0["n_cells"] == 0
+    assert (st2["n_fs3"], st2["n_fs2_items"]) == (st0["n_fs3"], st0["n_fs2_items"])
+    for a, b in zip(r2, r0):
+        np.testing.assert_array_equal(a, b)
