@@ -486,7 +486,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
 }
 
 const char* const FXOVER_MSG =
-    "a force contribution of 64 hartree/bohr or more, or not a number: the geometry is outside the range of the "
+    "a force contribution of 2^30 hartree/bohr or more, or not a number: the geometry is outside the range of the "
     "fixed-point force sums (overlapping sites?)";
 
 // run() until no list outgrew its capacity (at most twice), waiting for the

@@ -19,10 +19,10 @@
  *   - box <= 0 selects an open boundary (isolated clusters).
  * Every entry point returns 0 on success and a negative CCPOL3B_E* code on
  * failure; on failure the caller's output arrays are left untouched.
- * Forces are summed in 64-bit fixed point (2^-56 hartree/bohr), so they are
- * the same bits run to run and whatever the batch a box is evaluated in; a
- * force contribution of 64 hartree/bohr or more (overlapping sites) makes the
- * call fail with CCPOL3B_EINVAL instead.
+ * Forces are summed in 128-bit fixed point (units of 2^-96 hartree/bohr), so
+ * they are the same bits run to run and whatever the batch a box is evaluated
+ * in; a force contribution of 2^30 hartree/bohr or more, or not a number
+ * (overlapping sites), makes the call fail with CCPOL3B_EINVAL instead.
  * One context is used by one host thread at a time.
  */
 #ifndef CCPOL3B_H
