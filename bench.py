@@ -411,6 +411,7 @@ def main():
                                f"that meets the 1e-5 force bar; mixed precision reported beside it)",
                    "n_molecules": box.nm, "n_fs3_triplets": n3, "n_fs2_items": st_64["n_fs2_items"],
                    "n_fs2_pairs": st_64["n_fs2_pairs"],
+                   "fs2_items_per_s": st_64["n_fs2_items"] / (gms_64 * 1e-3),
                    "region_hist_sr_switch_lr": [st_64["n_fs3_sr"], st_64["n_fs3_switch"], st_64["n_fs3_lr"]]},
         "roofline": {"bound": "valu", "kernel": "k_fs3_eval_f64", "achieved": achieved_64,
                      "peak": PEAK_FP64_VECTOR_TFLOPS, "unit": "TFLOP/s", "frac": achieved_64 / PEAK_FP64_VECTOR_TFLOPS,
