@@ -13,6 +13,12 @@
  *   - constant exponents with real kind (x**2.d0, bohr2a**(-2.d0)) go through
  *     libm pow, the rest through multiplication.
  * Accumulation order follows the reference loop nests when nthreads == 1.
+ *
+ * With orc_opts.optimised the same routine is the optimised CPU baseline the
+ * benchmark reports beside the faithful one: neighbour-list list builders
+ * (fs3_list_nb, fs2_list_nb) and the factorised FS3 region
+ * (fs3_region_canonical); tests/test_oracle_canonical.py checks it against the
+ * restatement.
  */
 #include "ccpol3b_oracle.h"
 
@@ -259,6 +265,95 @@ static long fs2_list_bohr(const geom_t *G, double rc2, int *out, long cap) {
   return n;
 }
 
+/* The optimised baseline's list builders: the same predicates and loop order
+ * as fs3_list_bohr / fs2_list_bohr, enumerated from neighbour lists.  A pair
+ * fails the full predicate when it fails its own distance test, computed with
+ * the same arithmetic, so only candidates from the lists can pass.  nb[m] holds
+ * the molecules j != m with |minimg(x_m - x_j)|^2 < rc2 in ascending order
+ * (minimg(-d) = -minimg(d) exactly, so the relation is symmetric). */
+static int **neighbour_lists(const geom_t *G, double rc2, int **count_out) {
+  const double L = G->boxi;
+  int m, j;
+  int **nb = (int **)calloc(G->nm ? G->nm : 1, sizeof(int *));
+  int *cnt = (int *)calloc(G->nm ? G->nm : 1, sizeof(int));
+  for (m = 0; m < G->nm; ++m) {
+    int cap = 16;
+    nb[m] = (int *)malloc(sizeof(int) * cap);
+    for (j = 0; j < G->nm; ++j) {
+      double dx = G->xx[m] - G->xx[j], dy = G->yy[m] - G->yy[j], dz = G->zz[m] - G->zz[j];
+      if (j == m) continue;
+      if (L > 0.0) {
+        dx = dx - L * nint_d(dx / L);
+        dy = dy - L * nint_d(dy / L);
+        dz = dz - L * nint_d(dz / L);
+      }
+      if (dx * dx + dy * dy + dz * dz < rc2) {
+        if (cnt[m] == cap) nb[m] = (int *)realloc(nb[m], sizeof(int) * (cap *= 2));
+        nb[m][cnt[m]++] = j;
+      }
+    }
+  }
+  *count_out = cnt;
+  return nb;
+}
+
+static void free_neighbour_lists(int **nb, int *cnt, int nm) {
+  int m;
+  for (m = 0; m < nm; ++m) free(nb[m]);
+  free(nb);
+  free(cnt);
+}
+
+static long fs3_list_nb(const geom_t *G, double rc2, int *out, long cap) {
+  long n = 0;
+  int i, a, b, *cnt;
+  double v[9], d2[3];
+  int **nb = neighbour_lists(G, rc2, &cnt);
+  for (i = 0; i < G->nm; ++i)
+    for (a = 0; a < cnt[i]; ++a) {
+      const int j = nb[i][a];
+      if (j <= i) continue;
+      for (b = a + 1; b < cnt[i]; ++b) {
+        const int k = nb[i][b];
+        if (fs3_accept(G, i, j, k, rc2, v, d2)) {
+          if (out && n < cap) {
+            out[3 * n] = i;
+            out[3 * n + 1] = j;
+            out[3 * n + 2] = k;
+          }
+          ++n;
+        }
+      }
+    }
+  free_neighbour_lists(nb, cnt, G->nm);
+  return n;
+}
+
+static long fs2_list_nb(const geom_t *G, double rc2, int *out, long cap) {
+  long n = 0;
+  int k, a, b, *cnt;
+  double v[9], d2[3];
+  int **nb = neighbour_lists(G, rc2, &cnt);
+  for (k = 0; k < G->nm; ++k)
+    for (a = 0; a < cnt[k]; ++a) {
+      const int i = nb[k][a];
+      for (b = 0; b < cnt[i]; ++b) {
+        const int j = nb[i][b];
+        if (j <= i || j == k) continue;
+        if (fs2_accept(G, k, i, j, rc2, v, d2)) {
+          if (out && n < cap) {
+            out[3 * n] = k;
+            out[3 * n + 1] = i;
+            out[3 * n + 2] = j;
+          }
+          ++n;
+        }
+      }
+    }
+  free_neighbour_lists(nb, cnt, G->nm);
+  return n;
+}
+
 typedef struct {
   double *x, *y, *z, *xs, *ys, *zs;
   double boxi;
@@ -454,10 +549,123 @@ static void fs3_region(const geom_t *G, int i, int j, int k, const double v[9],
       }
 }
 
+/* The same region sweep as fs3_region, reorganised (not a restatement of the
+ * reference's loop nest; used for the optimised CPU baseline and checked
+ * against fs3_region by tests/test_oracle_canonical.py).  With the pair vector
+ * v_ab = (e_ab, e_ab r_ab), the raw polynomial sum is
+ *   E = sum_{a,b,c} sum_{ip,jp,kp} C[a,b,c,ip,jp,kp] v_jk[b,c][ip] v_ik[a,c][jp] v_ij[a,b][kp],
+ * so dE/dr of a site pair follows from the gradient G of E with respect to its
+ * pair vector: dE/dr = e (G0 der + G1 (r der + 1)), der = d ln e / dr.  Each
+ * site pair's functions are evaluated once (867 per region instead of
+ * 3 x 4913), and the weights, forces and virial are applied per site pair:
+ *   site forces  -(fdmp wE / r) dE/dr x_ab on a, + on b;
+ *   virial       fdmp wE r dE/dr per site pair, wC E (R_ij + R_ik + R_jk);
+ *   COM forces   -wC E (x_ij / R_ij + x_ik / R_ik) on i, and so on. */
+static void fs3_region_canonical(const geom_t *G, int i, int j, int k, const double v[9],
+                                 const double R[3], const double (*bS)[17], const double (*gS)[17],
+                                 const double (*r0S)[17], const double *cs, const int *tbl,
+                                 double fdmp, double wE, double wC, acc_t *A) {
+  enum { N = ORC_NS3 };
+  const int ld = G->ld;
+  const double *xxs = G->xxs, *yys = G->yys, *zzs = G->zzs;
+  /* blocks 0: ij [a][b], 1: ik [a][c], 2: jk [b][c] */
+  const int mol[3][2] = {{i, j}, {i, k}, {j, k}};
+  double sv[3][N][N][3], sr[3][N][N], se[3][N][N][2], sd[3][N][N], sg[3][N][N][2];
+  int blk, a, b, c, m;
+  for (blk = 0; blk < 3; ++blk) {
+    const int m0 = mol[blk][0], m1 = mol[blk][1];
+    for (a = 0; a < N; ++a)
+      for (b = 0; b < N; ++b) {
+        const long s0 = a + (long)ld * m0, s1 = b + (long)ld * m1;
+        const double x = v[3 * blk] + (xxs[s0] - xxs[s1]);
+        const double y = v[3 * blk + 1] + (yys[s0] - yys[s1]);
+        const double z = v[3 * blk + 2] + (zzs[s0] - zzs[s1]);
+        const double r = sqrt(x * x + y * y + z * z);
+        const double be = bS[a][b], ga = gS[a][b], r0 = r0S[a][b];
+        const double t = exp(-ga * (r - r0));
+        const double damp = 1.0 / (1.0 + t);
+        const double e = exp(-be * r) * damp;
+        sv[blk][a][b][0] = x;
+        sv[blk][a][b][1] = y;
+        sv[blk][a][b][2] = z;
+        sr[blk][a][b] = r;
+        se[blk][a][b][0] = e;
+        se[blk][a][b][1] = e * r;
+        sd[blk][a][b] = -be + ga * t * damp;
+        sg[blk][a][b][0] = sg[blk][a][b][1] = 0.0;
+      }
+  }
+  double E = 0.0;
+  for (a = 0; a < N; ++a)
+    for (b = 0; b < N; ++b) {
+      const double *zv = se[0][a][b];
+      double gij0 = 0.0, gij1 = 0.0;
+      for (c = 0; c < N; ++c) {
+        const int *tb = tbl + ((a * N + b) * N + c) * 8;
+        double C8[8];
+        for (m = 0; m < 8; ++m) C8[m] = cs[tb[m] - 1];  /* index ip*4 + jp*2 + kp */
+        const double *xv = se[2][b][c], *yv = se[1][a][c];
+        const double u00 = xv[0] * yv[0], u01 = xv[0] * yv[1], u10 = xv[1] * yv[0], u11 = xv[1] * yv[1];
+        gij0 += C8[0] * u00 + C8[2] * u01 + C8[4] * u10 + C8[6] * u11;
+        gij1 += C8[1] * u00 + C8[3] * u01 + C8[5] * u10 + C8[7] * u11;
+        /* d/dv_ik[jp]: sum over ip, kp of C x[ip] z[kp] */
+        const double w00 = xv[0] * zv[0], w01 = xv[0] * zv[1], w10 = xv[1] * zv[0], w11 = xv[1] * zv[1];
+        sg[1][a][c][0] += C8[0] * w00 + C8[1] * w01 + C8[4] * w10 + C8[5] * w11;
+        sg[1][a][c][1] += C8[2] * w00 + C8[3] * w01 + C8[6] * w10 + C8[7] * w11;
+        /* d/dv_jk[ip]: sum over jp, kp of C y[jp] z[kp] */
+        const double q00 = yv[0] * zv[0], q01 = yv[0] * zv[1], q10 = yv[1] * zv[0], q11 = yv[1] * zv[1];
+        sg[2][b][c][0] += C8[0] * q00 + C8[1] * q01 + C8[2] * q10 + C8[3] * q11;
+        sg[2][b][c][1] += C8[4] * q00 + C8[5] * q01 + C8[6] * q10 + C8[7] * q11;
+      }
+      sg[0][a][b][0] = gij0;
+      sg[0][a][b][1] = gij1;
+      E += zv[0] * gij0 + zv[1] * gij1;
+    }
+  const double fw = fdmp * wE;
+  A->e = A->e + fw * E;
+  double W = 0.0;
+  for (blk = 0; blk < 3; ++blk) {
+    const int m0 = mol[blk][0], m1 = mol[blk][1];
+    for (a = 0; a < N; ++a)
+      for (b = 0; b < N; ++b) {
+        const double r = sr[blk][a][b], e = se[blk][a][b][0], der = sd[blk][a][b];
+        const double dEdr = e * (sg[blk][a][b][0] * der + sg[blk][a][b][1] * (r * der + 1.0));
+        const double g = fw * dEdr / r;
+        const long s0 = a + (long)ld * m0, s1 = b + (long)ld * m1;
+        const double *x = sv[blk][a][b];
+        A->fx[s0] -= g * x[0];
+        A->fy[s0] -= g * x[1];
+        A->fz[s0] -= g * x[2];
+        A->fx[s1] += g * x[0];
+        A->fy[s1] += g * x[1];
+        A->fz[s1] += g * x[2];
+        W += fw * r * dEdr;
+      }
+  }
+  W += wC * E * (R[0] + R[1] + R[2]);
+  A->w = A->w + W;
+  const double uij[3] = {v[0] / R[0], v[1] / R[0], v[2] / R[0]};
+  const double uik[3] = {v[3] / R[1], v[4] / R[1], v[5] / R[1]};
+  const double ujk[3] = {v[6] / R[2], v[7] / R[2], v[8] / R[2]};
+  A->cx[i] -= wC * E * (uij[0] + uik[0]);
+  A->cy[i] -= wC * E * (uij[1] + uik[1]);
+  A->cz[i] -= wC * E * (uij[2] + uik[2]);
+  A->cx[j] -= wC * E * (-uij[0] + ujk[0]);
+  A->cy[j] -= wC * E * (-uij[1] + ujk[1]);
+  A->cz[j] -= wC * E * (-uij[2] + ujk[2]);
+  A->cx[k] -= wC * E * (-uik[0] - ujk[0]);
+  A->cy[k] -= wC * E * (-uik[1] - ujk[1]);
+  A->cz[k] -= wC * E * (-uik[2] - ujk[2]);
+}
+
+typedef void (*fs3_region_fn)(const geom_t *, int, int, int, const double *, const double *,
+                              const double (*)[17], const double (*)[17], const double (*)[17],
+                              const double *, const int *, double, double, double, acc_t *);
+
 /* One accepted FS3 triplet: R_avg, damping, region dispatch (:894-1943) */
 static int fs3_triplet(const orc_params *P, const geom_t *G, int i, int j, int k,
                        const double v[9], const double d2[3], int *tbl, int rebuild,
-                       acc_t *A) {
+                       acc_t *A, fs3_region_fn fs3_region) {
   const double alp3 = P->alpha3 * bohr2a;            /* :771 */
   const double r0dmp = 6.20446 / bohr2a;             /* :772 */
   const double rmin = 4.5 / bohr2a, rmax = 5.5 / bohr2a; /* :774-775 */
@@ -714,9 +922,12 @@ void orc_fs3_term(const orc_params *P, int nm, int ld,
   double t0, t1, t2;
   make_geom(nm, ld, ORC_NS3, x, y, z, xs, ys, zs, box, &G, &B);
   t0 = wall_seconds();
-  n = fs3_list_bohr(&G, rc2, NULL, 0);
-  list = (int *)malloc(sizeof(int) * 3 * (n ? n : 1));
-  fs3_list_bohr(&G, rc2, list, n);
+  {
+    long (*build)(const geom_t *, double, int *, long) = opts->optimised ? fs3_list_nb : fs3_list_bohr;
+    n = build(&G, rc2, NULL, 0);
+    list = (int *)malloc(sizeof(int) * 3 * (n ? n : 1));
+    build(&G, rc2, list, n);
+  }
   t1 = wall_seconds();
   first = opts->fs3_first < 0 ? 0 : opts->fs3_first;
   last = (opts->fs3_last < 0 || opts->fs3_last > n) ? n : opts->fs3_last;
@@ -739,7 +950,8 @@ void orc_fs3_term(const orc_params *P, int nm, int ld,
       double v[9], d2[3];
       int i = list[3 * t], j = list[3 * t + 1], k = list[3 * t + 2];
       fs3_accept(&G, i, j, k, rc2, v, d2);
-      lh[fs3_triplet(P, &G, i, j, k, v, d2, tbl, opts->rebuild_table_per_triplet, &A[tid])]++;
+      lh[fs3_triplet(P, &G, i, j, k, v, d2, tbl, opts->rebuild_table_per_triplet, &A[tid],
+                     opts->optimised ? fs3_region_canonical : fs3_region)]++;
     }
 #pragma omp critical
     {
@@ -784,9 +996,12 @@ void orc_fs2_term(const orc_params *P, int nm, int ld,
   double t0, t1, t2;
   make_geom(nm, ld, 5, x, y, z, xs, ys, zs, box, &G, &B);
   t0 = wall_seconds();
-  n = fs2_list_bohr(&G, rc2, NULL, 0);
-  list = (int *)malloc(sizeof(int) * 3 * (n ? n : 1));
-  fs2_list_bohr(&G, rc2, list, n);
+  {
+    long (*build)(const geom_t *, double, int *, long) = opts->optimised ? fs2_list_nb : fs2_list_bohr;
+    n = build(&G, rc2, NULL, 0);
+    list = (int *)malloc(sizeof(int) * 3 * (n ? n : 1));
+    build(&G, rc2, list, n);
+  }
   t1 = wall_seconds();
   first = opts->fs2_first < 0 ? 0 : opts->fs2_first;
   last = (opts->fs2_last < 0 || opts->fs2_last > n) ? n : opts->fs2_last;

@@ -45,6 +45,14 @@ typedef struct {
   long fs3_first, fs3_last;      /* evaluate only accepted FS3 triplets with ordinal in
                                     [first,last) (bounded timing sample); last<0 = all */
   long fs2_first, fs2_last;      /* same for FS2 accepted (k;i<j) items */
+  int optimised;                 /* 1: the optimised CPU baseline: the same FS3 and FS2 lists
+                                    built from neighbour lists (O(N n_nb^2) instead of the
+                                    O(N^3) scans), and each FS3 region evaluated by the
+                                    factorised contraction (fs3_region_canonical: site-pair
+                                    functions once per site pair, the trilinear polynomial
+                                    summed over site triples with its three pair gradients)
+                                    instead of the reference's loop nest; same lists, same
+                                    results to rounding */
 } orc_opts;
 
 typedef struct {

@@ -32,7 +32,7 @@ class _Params(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("rebuild_table_per_triplet", C.c_int), ("nthreads", C.c_int),
                 ("fs3_first", C.c_long), ("fs3_last", C.c_long),
-                ("fs2_first", C.c_long), ("fs2_last", C.c_long)]
+                ("fs2_first", C.c_long), ("fs2_last", C.c_long), ("optimised", C.c_int)]
 
 
 class _Stats(C.Structure):
@@ -102,10 +102,12 @@ def ind_lin_2() -> np.ndarray:
 
 def threebody_term(p, box_obj=None, *, x=None, y=None, z=None, xs=None, ys=None, zs=None,
                    box=None, ld=None, nthreads=1, rebuild=False,
-                   fs3_window=(0, -1), fs2_window=(0, -1)):
+                   fs3_window=(0, -1), fs2_window=(0, -1), optimised=False):
     """Run the oracle on one box.  Returns a dict with qfs2, qfs3 (kcal/mol),
     site forces fxs/fys/fzs [nm, ld] (+= onto zeros), COM forces fxfs2.. [nm],
-    vir (+= onto 0) and the list statistics."""
+    vir (+= onto 0) and the list statistics.  optimised=True is the optimised
+    CPU baseline: the same lists from neighbour lists, and FS3 by the factorised
+    contraction instead of the reference's loop nest."""
     if box_obj is not None:
         x, y, z = box_obj.x, box_obj.y, box_obj.z
         xs, ys, zs = box_obj.xs, box_obj.ys, box_obj.zs
@@ -117,7 +119,8 @@ def threebody_term(p, box_obj=None, *, x=None, y=None, z=None, xs=None, ys=None,
     for k in ("fxfs2", "fyfs2", "fzfs2", "fxfs3", "fyfs3", "fzfs3"):
         out[k] = np.zeros(nm)
     q2, q3, vir = C.c_double(0.0), C.c_double(0.0), C.c_double(0.0)
-    o = _Opts(int(rebuild), int(nthreads), fs3_window[0], fs3_window[1], fs2_window[0], fs2_window[1])
+    o = _Opts(int(rebuild), int(nthreads), fs3_window[0], fs3_window[1], fs2_window[0], fs2_window[1],
+              int(optimised))
     st = _Stats()
     cp = to_cparams(p)
     lib().orc_threebody_term(C.byref(cp), nm, ld, _dp(x), _dp(y), _dp(z), _dp(xs), _dp(ys), _dp(zs),
