@@ -141,6 +141,21 @@ def test_fs2fs3_box(tb_fs2fs3, oracle, params_fs2fs3, n, seed):
     assert_parity(got, ref)
 
 
+def test_benchmark_box_against_oracle(tb_fs2fs3, oracle, params_fs2fs3):
+    """The box bench.py times (512 waters, seed 2025: 31,290 triplets, 450,861
+    FS2 items) in FP64 against the oracle: both lists bit for bit, energies,
+    forces and virial to the parity bars."""
+    box = lattice_box(512, seed=2025)
+    print("benchmark box: 512 waters, seed 2025", flush=True)
+    got = tb_fs2fs3.evaluate(box)
+    l3, l2 = tb_fs2fs3.fs3_list(), tb_fs2fs3.fs2_list()
+    ref = oracle.threebody_term(params_fs2fs3, box, nthreads=NTHREADS)
+    assert (ref["n_fs3"], ref["n_fs2"]) == (31290, 450861)
+    np.testing.assert_array_equal(l3, oracle.fs3_list(params_fs2fs3, box.x, box.y, box.z, box.box))
+    np.testing.assert_array_equal(l2, oracle.fs2_list(params_fs2fs3, box.x, box.y, box.z, box.box))
+    assert_parity(got, ref)
+
+
 def region_trimers(seed=23):
     """Equilateral trimers at R_avg = 4.0 (SR), 5.0 (switch) and 5.9 A (LR)."""
     from ccpol23_plus_2025_mbmd_amd.geometry import WaterBox, random_quaternions
