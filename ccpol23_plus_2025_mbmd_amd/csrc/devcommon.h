@@ -142,6 +142,28 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Sum over the 64 lanes of a wavefront, valid in lane 63 only: DPP row shifts
+// build each row's sum in its lane 15, then row broadcasts add the rows.  A
+// fixed order of additions (the same bits on every run), with no LDS traffic.
+__device__ __forceinline__ double dpp_step(double v, int ctrl_id) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  int a, b;
+  switch (ctrl_id) {  // row_shr:1, 2, 4, 8 within rows of 16; row_bcast15 into rows 1, 3; row_bcast31 into rows 2, 3
+    case 0: a = __builtin_amdgcn_update_dpp(0, lo, 0x111, 0xf, 0xf, true); b = __builtin_amdgcn_update_dpp(0, hi, 0x111, 0xf, 0xf, true); break;
+    case 1: a = __builtin_amdgcn_update_dpp(0, lo, 0x112, 0xf, 0xf, true); b = __builtin_amdgcn_update_dpp(0, hi, 0x112, 0xf, 0xf, true); break;
+    case 2: a = __builtin_amdgcn_update_dpp(0, lo, 0x114, 0xf, 0xf, true); b = __builtin_amdgcn_update_dpp(0, hi, 0x114, 0xf, 0xf, true); break;
+    case 3: a = __builtin_amdgcn_update_dpp(0, lo, 0x118, 0xf, 0xf, true); b = __builtin_amdgcn_update_dpp(0, hi, 0x118, 0xf, 0xf, true); break;
+    case 4: a = __builtin_amdgcn_update_dpp(0, lo, 0x142, 0xa, 0xf, false); b = __builtin_amdgcn_update_dpp(0, hi, 0x142, 0xa, 0xf, false); break;
+    default: a = __builtin_amdgcn_update_dpp(0, lo, 0x143, 0xc, 0xf, false); b = __builtin_amdgcn_update_dpp(0, hi, 0x143, 0xc, 0xf, false); break;
+  }
+  return v + __hiloint2double(b, a);
+}
+__device__ __forceinline__ double wave_sum_lane63(double v) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v = dpp_step(v, k);
+  return v;
+}
+
 // Deterministic block sum over 256 threads (4 wavefronts).
 __device__ __forceinline__ double block_sum256(double v, double* red) {
   v = wave_sum(v);

@@ -18,6 +18,8 @@
 // dv0/dr = e' and dv1/dr = e + r e', e' = e (-beta + gamma (1 - damp)).
 // The triplet energy is f_d(R_avg) * sum_X w_X(R_avg) E_X with the damping and
 // switch of :894-904 and :1420-1423.
+#include <type_traits>
+
 #include "devcommon.h"
 
 namespace ccpol3b {
@@ -259,22 +261,31 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   __shared__ T sDC[PHI ? NCOEF : 1];
 
   const long t = blockIdx.x;  // one block per triplet
-  if (t >= D.totals[T_N3] || t >= D.cap3) return;
+  if (t >= D.cap3) return;
   const int tid = threadIdx.x;
+  // The triplet's record, geometry and site positions are loaded before the
+  // list length is known (t < cap3 keeps every load inside its array), so the
+  // site loads wait on one level of memory latency instead of two.  A block
+  // past the list may read records of an earlier call: its molecule indices
+  // are clamped into range, and it returns once the length has arrived.
   static_assert(sizeof(TriGeo) % 8 == 0 && sizeof(TriGeo) / 8 <= 256, "TriGeo copy");
-  if (tid < (int)(sizeof(TriGeo) / 8))
-    reinterpret_cast<double*>(&g)[tid] = reinterpret_cast<const double*>(D.geo3 + t)[tid];
-  const int nreg = D.geo3[t].nreg, reg0 = D.geo3[t].reg0;
+  const int4 tr = D.tri[t];
+  double gword = 0.0, sval = 0.0, shift = 0.0;
+  if (tid < (int)(sizeof(TriGeo) / 8)) gword = reinterpret_cast<const double*>(D.geo3 + t)[tid];
   if (tid < 3 * NS3 * 3) {
     // site positions in a frame centred on molecule i: s_i, s_j - x_ij and
     // s_k - x_ik, so that every site-pair vector is a plain difference
     // (x_jk = x_ik - x_ij for the derived vector of :882-884)
     const int role = tid / (NS3 * 3), rem = tid - role * NS3 * 3, c = rem - (rem / 3) * 3;
-    const int4 tr = D.tri[t];
-    const int m = role == 0 ? tr.x : role == 1 ? tr.y : tr.z;
-    const double shift = role == 0 ? 0.0 : D.geo3[t].c[role - 1][c];
-    (&sP[0][0][0])[tid] = D.site[(long)m * NS3 * 3 + rem] - shift;
+    int m = role == 0 ? tr.x : role == 1 ? tr.y : tr.z;
+    m = (unsigned)m < (unsigned)D.nm ? m : 0;
+    if (role > 0) shift = D.geo3[t].c[role - 1][c];
+    sval = D.site[(long)m * NS3 * 3 + rem];
   }
+  if (t >= D.totals[T_N3]) return;
+  if (tid < (int)(sizeof(TriGeo) / 8)) reinterpret_cast<double*>(&g)[tid] = gword;
+  if (tid < 3 * NS3 * 3) (&sP[0][0][0])[tid] = sval - shift;
+  const int nreg = D.geo3[t].nreg, reg0 = D.geo3[t].reg0;
 
   T gacc[2][2] = {{T(0), T(0)}, {T(0), T(0)}};  // fd w dE/dr of the tile's site pairs
   double eacc0 = 0.0, eacc1 = 0.0;
@@ -338,73 +349,91 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
       }
     }
     __syncthreads();
-    // ---------------- site-triple stage (the tile is rebuilt from an opaque
-    // copy of tid: cheaper than keeping it live across the region loop)
-    int tq = tid;
-    asm volatile("" : "+v"(tq));
-    const TileSpec ts = tile_spec(tq);
-    if (ts.used) {
-      const T* cp = (CSM ? sC : coef + X * NCOEF) + (ts.tu * NTYPE + ts.tw) * NTYPE * 8;
-      T* dc = nullptr;
-      if constexpr (PHI)
-        if (ts.P == 0) dc = sDC + (ts.tu * NTYPE + ts.tw) * NTYPE * 8;
+    // ---------------- site-triple stage.  Waves 0-2 take pass P = wave on the
+    // 16 x 16 site pairs of sites 1-16, lane = one 2 x 2 tile, with P, its
+    // strides and its owner indices compile-time constants; wave 3 takes the
+    // O row, the O column and (O, O) of the three passes (tile_spec).  The
+    // lane's tile is rebuilt each region from an opaque copy of tid: cheaper
+    // than keeping it live across the region loop.
+    {
+      const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
       const T fdw = T(g.fd * g.w[rr]);
-      // M, G and the gacc update of the tile; the energy is sum_xy v_xy G_xy
-      // over pass 0's tiles
-      auto finish = [&](auto& G, int nu) {
+      const T* cbase = CSM ? sC : coef + X * NCOEF;
+      if (wave < 3) {
+        int lq = tid & 63;
+        asm volatile("" : "+v"(lq));
+        const int kx = lq >> 3, ky = lq & 7, x0 = 1 + 2 * kx, y0 = 1 + 2 * ky;
+        // site_type(1 + 2k) = k + 1 for k < 2 (H, D1), else 2 + k / 2 (N1-N3)
+        const int tx = kx < 2 ? kx + 1 : 2 + (kx >> 1), ty = ky < 2 ? ky + 1 : 2 + (ky >> 1);
+        const int tb = (tx * NTYPE + ty) * NTYPE * 8;
+        auto pass = [&](auto pc) {
+          constexpr int P = decltype(pc)::value;
+          constexpr PassLayout L = pass_layout(P);
+          T G[2][2][2], vown[2][2][2] = {};
+          T* dc = nullptr;
+          if constexpr (PHI && P == 0) {
+            dc = sDC + tb;
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          if (a >= nu) break;
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int b2 = 0; b2 < 2; ++b2) {
+                const T2 v = sV[(x0 + a) * NS3 + y0 + b2];
+                vown[a][b2][0] = v.x;
+                vown[a][b2][1] = v.y;
+              }
+          }
+          contract_tile<T, 2, 2, PHI>(sV + L.ablk * NPAIR + x0 * L.arow, L.arow, L.az,
+                                      sV + L.bblk * NPAIR + y0 * L.brow, L.brow, L.bz, cbase + tb, G, dc, vown);
+          // the tile's site pairs (x0 + a, y0 + b2) of block P: the energy is
+          // sum_xy v_xy G_xy over pass 0
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b2 = 0; b2 < 2; ++b2) {
+              const int po = P * NPAIR + (x0 + a) * NS3 + y0 + b2;
+              const T2 v = sV[po];
+              const T ev = v.x * G[a][b2][0] + v.y * G[a][b2][1];
+              gacc[a][b2] += fdw * (sDer[po] * ev + v.x * G[a][b2][1]);
+              if constexpr (P == 0) {
+                if (rr == 0) eacc0 += double(ev); else eacc1 += double(ev);
+              }
+            }
+        };
+        if (wave == 0) pass(std::integral_constant<int, 0>{});
+        else if (wave == 1) pass(std::integral_constant<int, 1>{});
+        else pass(std::integral_constant<int, 2>{});
+      } else {
+        int tq = tid;
+        asm volatile("" : "+v"(tq));
+        const TileSpec ts = tile_spec(tq);
+        if (ts.used) {
+          const T* cp = cbase + (ts.tu * NTYPE + ts.tw) * NTYPE * 8;
+          T* dc = nullptr;
+          if constexpr (PHI)
+            if (ts.P == 0) dc = sDC + (ts.tu * NTYPE + ts.tw) * NTYPE * 8;
+          T G[1][2][2], vown[1][2][2] = {};
+          if constexpr (PHI)
+            if (ts.P == 0)
+#pragma unroll
+              for (int b2 = 0; b2 < 2; ++b2) {
+                const int po = ts.own[0][b2];
+                const T2 v = sV[po < 0 ? 0 : po];
+                vown[0][b2][0] = po < 0 ? T(0) : v.x;
+                vown[0][b2][1] = po < 0 ? T(0) : v.y;
+              }
+          contract_tile<T, 1, 2, PHI>(sV + ts.u_off, 0, ts.u_zs, sV + ts.w_off, ts.w_rs, ts.w_zs, cp, G, dc, vown);
 #pragma unroll
           for (int b2 = 0; b2 < 2; ++b2) {
-            const int po = ts.own[a][b2];
+            const int po = ts.own[0][b2];
             if (po < 0) continue;
             const T2 v = sV[po];
-            const T ev = v.x * G[a][b2][0] + v.y * G[a][b2][1];
-            gacc[a][b2] += fdw * (sDer[po] * ev + v.x * G[a][b2][1]);
+            const T ev = v.x * G[0][b2][0] + v.y * G[0][b2][1];
+            gacc[0][b2] += fdw * (sDer[po] * ev + v.x * G[0][b2][1]);
             if (ts.P == 0) {
               if (rr == 0) eacc0 += double(ev); else eacc1 += double(ev);
             }
           }
         }
-      };
-      if (tid < 192) {
-        T G[2][2][2], vown[2][2][2] = {};
-        if constexpr (PHI)
-          if (ts.P == 0)
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-              for (int b2 = 0; b2 < 2; ++b2) {
-                const T2 v = sV[ts.own[a][b2]];
-                vown[a][b2][0] = v.x;
-                vown[a][b2][1] = v.y;
-              }
-        // one pass per wave, its strides as constants
-        if (ts.P == 0) {
-          constexpr PassLayout L = pass_layout(0);
-          contract_tile<T, 2, 2, PHI>(sV + ts.u_off, L.arow, L.az, sV + ts.w_off, L.brow, L.bz, cp, G, dc, vown);
-        } else if (ts.P == 1) {
-          constexpr PassLayout L = pass_layout(1);
-          contract_tile<T, 2, 2, PHI>(sV + ts.u_off, L.arow, L.az, sV + ts.w_off, L.brow, L.bz, cp, G, dc, vown);
-        } else {
-          constexpr PassLayout L = pass_layout(2);
-          contract_tile<T, 2, 2, PHI>(sV + ts.u_off, L.arow, L.az, sV + ts.w_off, L.brow, L.bz, cp, G, dc, vown);
-        }
-        finish(G, 2);
-      } else {
-        T G[1][2][2], vown[1][2][2] = {};
-        if constexpr (PHI)
-          if (ts.P == 0)
-#pragma unroll
-            for (int b2 = 0; b2 < 2; ++b2) {
-              const int po = ts.own[0][b2];
-              const T2 v = sV[po < 0 ? 0 : po];
-              vown[0][b2][0] = po < 0 ? T(0) : v.x;
-              vown[0][b2][1] = po < 0 ? T(0) : v.y;
-            }
-        contract_tile<T, 1, 2, PHI>(sV + ts.u_off, 0, ts.u_zs, sV + ts.w_off, ts.w_rs, ts.w_zs, cp, G, dc, vown);
-        finish(G, 1);
       }
     }
     __syncthreads();
@@ -434,27 +463,37 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   }
   // the tile owner turns dE/dr into h = (dE/dr)/r and the site virial sum r dE/dr
   double wsum = 0.0;
-  int tq = tid;
-  asm volatile("" : "+v"(tq));
-  const TileSpec ts = tile_spec(tq);
-  if (ts.used) {
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  auto own_h = [&](int po, T ga) {
+    const T r = sR[po];
+    wsum += double(ga) * double(r);
+    sR[po] = ga * t_rcp<T>(r);
+  };
+  if (wave < 3) {
+    const int lane = tid & 63, x0 = 1 + 2 * (lane >> 3), y0 = 1 + 2 * (lane & 7);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b2 = 0; b2 < 2; ++b2) {
-        const int po = ts.own[a][b2];
-        if (po < 0) continue;
-        const T r = sR[po];
-        wsum += double(gacc[a][b2]) * double(r);
-        sR[po] = gacc[a][b2] * t_rcp<T>(r);
-      }
+      for (int b2 = 0; b2 < 2; ++b2) own_h(wave * NPAIR + (x0 + a) * NS3 + y0 + b2, gacc[a][b2]);
+  } else {
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
+    const TileSpec ts = tile_spec(tq);
+    if (ts.used)
+#pragma unroll
+      for (int b2 = 0; b2 < 2; ++b2)
+        if (ts.own[0][b2] >= 0) own_h(ts.own[0][b2], gacc[0][b2]);
   }
   // region energies and the site virial (FP64, fixed reduction trees: each
   // wavefront's butterfly, then the four wavefront sums); the barrier also
-  // publishes h
+  // publishes h.  Only waves 0 and 3 hold pass-0 tiles, so only they have
+  // energy terms, and only switch-band triplets have a second region.
   {
-    const double s0 = wave_sum(eacc0), s1 = wave_sum(eacc1), s2 = wave_sum(wsum);
-    if ((tid & 63) == 0) {
+    const bool has_e = wave == 0 || wave == 3;
+    const double s0 = has_e ? wave_sum_lane63(eacc0) : 0.0;
+    const double s1 = has_e && nreg == 2 ? wave_sum_lane63(eacc1) : 0.0;
+    const double s2 = wave_sum_lane63(wsum);
+    if ((tid & 63) == 63) {
       sRed[tid >> 6][0] = s0;
       sRed[tid >> 6][1] = s1;
       sRed[tid >> 6][2] = s2;
