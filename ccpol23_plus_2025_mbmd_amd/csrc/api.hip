@@ -405,7 +405,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
     c->cells_zero = true;
   }
   // neighbour lists with the per-molecule FS3 triplet counts, then every offset
-  hipLaunchKernelGGL(k_neighbours, dim3(nm), dim3(64), ldsn, st, D, c->K, do3);
+  hipLaunchKernelGGL(k_neighbours, dim3(nm), dim3(256), ldsn, st, D, c->K, do3);
   hipLaunchKernelGGL(k_scan<1024>, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
   LAUNCHCHK(c, "neighbour lists");
   EVREC(1);
@@ -432,7 +432,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   }
   EVREC(2);
   hipStream_t s3 = c->side;
-  if (do3) hipLaunchKernelGGL(k_fs3_fill, dim3(nm), dim3(64), lds3, s3, D, c->K);
+  if (do3) hipLaunchKernelGGL(k_fs3_fill, dim3(nm), dim3(256), lds3, s3, D, c->K);
   LAUNCHCHK(c, "FS3 list");
   const long cap3 = D.cap3;
   EVREC_SIDE(8);

@@ -6,8 +6,9 @@
 // zero, squares are summed left to right, and this file is compiled without
 // FMA contraction (the reference's x86-64 build has no FMA).
 //
-// Work decomposition: one wavefront per molecule for the neighbour search (which
-// also counts the molecule's FS3 triplets) and the FS3 list fill, one per
+// Work decomposition: one block of four wavefronts per molecule for the
+// neighbour search (which also counts the molecule's FS3 triplets) and the FS3
+// list fill, one wavefront per
 // directed neighbour pair for the exported FS2 items.  The neighbour search
 // tests the molecules of the 27 surrounding cells of a cell grid when the box
 // has one (periodic, at least CELL_MIN cells a side of at least the larger
@@ -118,19 +119,50 @@ __device__ __forceinline__ void pair_of(long t, int m, int& r, int& s) {
   s = q + 1 + (int)(t - ((long)q * m - (long)q * (q + 1) / 2));
 }
 
-// One wavefront sweeps the m staged neighbours of i, 64 pairs at a time.  mode 0
-// returns the number of triplets; mode 1 also writes them from `out` with their
-// geometry (TriGeo) and counts their regions in shist.
+// Appends in block order: the LB_WAVES wavefronts of a block each pass K
+// wave-uniform counts (ballot popcounts); `before` gets the sum over lower
+// wavefronts and `total` the block's sum, the same in every thread.  Entries
+// then go to running offset + before + rank in the wavefront, so a round of
+// LB_WAVES * 64 candidates keeps candidate order.  Two barriers; buf is LDS.
+constexpr int LB_WAVES = 4;
+constexpr int LB_THREADS = 64 * LB_WAVES;
+template <int K>
+__device__ __forceinline__ void wave_prefix(const int (&cnt)[K], int (&before)[K], int (&total)[K],
+                                            int (*buf)[K]) {
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) buf[w][k] = cnt[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    before[k] = 0;
+    total[k] = 0;
+  }
+#pragma unroll
+  for (int u = 0; u < LB_WAVES; ++u)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      before[k] += u < w ? buf[u][k] : 0;
+      total[k] += buf[u][k];
+    }
+  __syncthreads();
+}
+
+// The block sweeps the m staged neighbours of i, LB_THREADS pairs at a time.
+// mode 0 returns the number of triplets; mode 1 also writes them from `out`
+// with their geometry (TriGeo) and counts their regions in shist.
 template <int mode>
 __device__ int fs3_sweep(const Dev& D, const Consts& K, int i, const int* nb, int m, const double* st, int cap,
                          double L, double xi, double yi, double zi, long out, int* shist) {
+  __shared__ int pbuf[LB_WAVES][1];
   const int lane = threadIdx.x & 63;
   const double *px = st, *py = st + cap, *pz = st + 2 * cap;
   const double *qx = st + 3 * cap, *qy = st + 4 * cap, *qz = st + 5 * cap;
   const long npair = (long)m * (m - 1) / 2;
   int count = 0;
-  for (long t0 = 0; t0 < npair; t0 += 64) {
-    const long t = t0 + lane;
+  for (long t0 = 0; t0 < npair; t0 += LB_THREADS) {
+    const long t = t0 + threadIdx.x;
     bool ok = false;
     int r = 0, s = 0;
     double xjk = 0.0, yjk = 0.0, zjk = 0.0;
@@ -147,8 +179,11 @@ __device__ int fs3_sweep(const Dev& D, const Consts& K, int i, const int* nb, in
       ok = sq3(xjk, yjk, zjk) < K.rc2_3;
     }
     const uint64_t msk = __ballot(ok);
+    const int c[1] = {(int)__popcll(msk)};
+    int before[1], total[1];
+    wave_prefix<1>(c, before, total, pbuf);
     if (mode == 1 && ok) {
-      const long p = out + count + __popcll(msk & lanemask_lt());
+      const long p = out + count + before[0] + __popcll(msk & lanemask_lt());
       if (p < D.cap3) {
         D.tri[p] = make_int4(i, nb[r], nb[s], 0);
         const double cij[3] = {(xi - px[r]) - L * qx[r], (yi - py[r]) - L * qy[r], (zi - pz[r]) - L * qz[r]};
@@ -160,26 +195,27 @@ __device__ int fs3_sweep(const Dev& D, const Consts& K, int i, const int* nb, in
         atomicAdd(shist + (g.nreg == 2 ? 1 : 2 * g.reg0), 1);
       }
     }
-    count += __popcll(msk);
+    (void)lane;
+    count += total[0];
   }
   return count;
 }
 
 // ------------------------------------------------------------------ neighbour lists
-// One wavefront (one 64-thread block) per molecule i; the candidates j are the
+// One block (four wavefronts) per molecule i; the candidates j are the
 // molecules of the 27 cells around i's cell when the box has a cell grid
 // (:297-336 test every molecule; the cells leave out only molecules beyond
 // the cutoffs), every other molecule of the box otherwise.  Each is tested
 // with the reference's pair predicate (min image of x_i - x_j), both cutoffs
 // at once.  The sweep meets j in ascending order; the cell path ranks the
 // accepted j and writes each at its rank, so both give the same lists.  With
-// do3 the
-// upper FS3 neighbours are staged in LDS on the way and the wavefront then
+// do3 the upper FS3 neighbours are staged in LDS on the way and the block then
 // counts i's FS3 triplets into cnt3s[i], so that the triplet offsets come out of
 // the scan that follows and the list can be filled in one more launch.
-__global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
+__global__ __launch_bounds__(LB_THREADS) void k_neighbours(Dev D, Consts K, int do3) {
   extern __shared__ double lsm[];
-  const int lane = threadIdx.x;
+  __shared__ int pbuf[LB_WAVES][4];
+  const int tid = threadIdx.x, lane = tid & 63;
   const int i = blockIdx.x;
   if (i >= D.nm) return;  // the whole block
   const int b = D.mol_box[i];
@@ -187,7 +223,7 @@ __global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
   const int j0 = D.box_start[b], j1 = D.box_start[b + 1];
   const double xi = D.cx[i], yi = D.cy[i], zi = D.cz[i];
   const int cap = D.cap_nb;
-  int n3 = 0, n2 = 0, u3 = 0, u2 = 0;
+  int n3 = 0, n2 = 0, u3 = 0, u2 = 0;  // running counts, the same in every thread
   int* nb3 = D.nb3 + (long)i * cap;
   int* nb2 = D.nb2 + (long)i * cap;
   double* v2 = D.nbv2 + (long)i * 3 * cap;
@@ -198,30 +234,32 @@ __global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
     __shared__ int spre[28], sbeg[27];
     int* A3 = reinterpret_cast<int*>(lsm + (do3 ? 6 * cap : 0));  // accepted, in candidate order
     int* A2 = A3 + cap;
-    const int c0 = D.cell0[b], ci = D.mol_cell[i] - c0;
-    const int ix = ci / (nc * nc), iy = (ci / nc) % nc, iz = ci % nc;
-    int len = 0;
-    if (lane < 27) {
-      int jx = ix + lane / 9 - 1, jy = iy + (lane / 3) % 3 - 1, jz = iz + lane % 3 - 1;
-      jx += jx < 0 ? nc : (jx >= nc ? -nc : 0);
-      jy += jy < 0 ? nc : (jy >= nc ? -nc : 0);
-      jz += jz < 0 ? nc : (jz >= nc ? -nc : 0);
-      const int cc = c0 + (jx * nc + jy) * nc + jz;
-      sbeg[lane] = D.cell_start[cc];
-      len = D.cell_start[cc + 1] - sbeg[lane];
-    }
-    int pre = len;  // inclusive prefix over lanes 0..26
+    if (tid < 64) {
+      const int c0 = D.cell0[b], ci = D.mol_cell[i] - c0;
+      const int ix = ci / (nc * nc), iy = (ci / nc) % nc, iz = ci % nc;
+      int len = 0;
+      if (lane < 27) {
+        int jx = ix + lane / 9 - 1, jy = iy + (lane / 3) % 3 - 1, jz = iz + lane % 3 - 1;
+        jx += jx < 0 ? nc : (jx >= nc ? -nc : 0);
+        jy += jy < 0 ? nc : (jy >= nc ? -nc : 0);
+        jz += jz < 0 ? nc : (jz >= nc ? -nc : 0);
+        const int cc = c0 + (jx * nc + jy) * nc + jz;
+        sbeg[lane] = D.cell_start[cc];
+        len = D.cell_start[cc + 1] - sbeg[lane];
+      }
+      int pre = len;  // inclusive prefix over lanes 0..26
 #pragma unroll
-    for (int d = 1; d < 32; d <<= 1) {
-      const int u = __shfl_up(pre, d, 64);
-      if (lane >= d) pre += u;
+      for (int d = 1; d < 32; d <<= 1) {
+        const int u = __shfl_up(pre, d, 64);
+        if (lane >= d) pre += u;
+      }
+      if (lane < 27) spre[lane + 1] = pre;
+      if (lane == 0) spre[0] = 0;
     }
-    if (lane < 27) spre[lane + 1] = pre;
-    if (lane == 0) spre[0] = 0;
-    const int total = __shfl(pre, 26, 64);
     __syncthreads();
-    for (int f0 = 0; f0 < total; f0 += 64) {
-      const int f = f0 + lane;
+    const int total = spre[27];
+    for (int f0 = 0; f0 < total; f0 += LB_THREADS) {
+      const int f = f0 + tid;
       bool in3 = false, in2 = false;
       int j = -1;
       if (f < total) {
@@ -238,23 +276,27 @@ __global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
         }
       }
       const uint64_t m3 = __ballot(in3), m2 = __ballot(in2);
+      const int c[4] = {(int)__popcll(m3), (int)__popcll(m2), (int)__popcll(__ballot(in3 && j < i)),
+                        (int)__popcll(__ballot(in2 && j < i))};
+      int before[4], tot[4];
+      wave_prefix<4>(c, before, tot, pbuf);
       if (in3) {
-        const int p = n3 + __popcll(m3 & lanemask_lt());
+        const int p = n3 + before[0] + __popcll(m3 & lanemask_lt());
         if (p < cap) A3[p] = j;
       }
       if (in2) {
-        const int p = n2 + __popcll(m2 & lanemask_lt());
+        const int p = n2 + before[1] + __popcll(m2 & lanemask_lt());
         if (p < cap) A2[p] = j;
       }
-      n3 += __popcll(m3);
-      n2 += __popcll(m2);
-      u3 += __popcll(__ballot(in3 && j < i));
-      u2 += __popcll(__ballot(in2 && j < i));
+      n3 += tot[0];
+      n2 += tot[1];
+      u3 += tot[2];
+      u2 += tot[3];
     }
     __syncthreads();
     // ascending order: each stored entry goes to its rank among the stored entries
     const int s3 = n3 < cap ? n3 : cap, s2 = n2 < cap ? n2 : cap;
-    for (int e = lane; e < s3; e += 64) {
+    for (int e = tid; e < s3; e += LB_THREADS) {
       const int j = A3[e];
       int r = 0;
       for (int f = 0; f < s3; ++f) r += A3[f] < j ? 1 : 0;
@@ -262,7 +304,7 @@ __global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
       const int pu = r - u3;  // position among the upper entries
       if (do3 && j > i && pu >= 0 && pu < cap) stage_upper(lsm, cap, pu, xi, yi, zi, D.cx[j], D.cy[j], D.cz[j], L);
     }
-    for (int e = lane; e < s2; e += 64) {
+    for (int e = tid; e < s2; e += LB_THREADS) {
       const int j = A2[e];
       int r = 0;
       for (int f = 0; f < s2; ++f) r += A2[f] < j ? 1 : 0;
@@ -272,8 +314,8 @@ __global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
       v2[2 * cap + r] = minimg(zi - D.cz[j], L);
     }
   } else {
-    for (int base = j0; base < j1; base += 64) {
-      const int j = base + lane;
+    for (int base = j0; base < j1; base += LB_THREADS) {
+      const int j = base + tid;
       bool in3 = false, in2 = false;
       double xj = 0.0, yj = 0.0, zj = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
       if (j < j1 && j != i) {
@@ -288,16 +330,20 @@ __global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
         in2 = d2 < K.rc2_2;
       }
       const uint64_t m3 = __ballot(in3), m2 = __ballot(in2);
-      const uint64_t lo3 = __ballot(in3 && j < i), lo2 = __ballot(in2 && j < i);
+      const int c[4] = {(int)__popcll(m3), (int)__popcll(m2), (int)__popcll(__ballot(in3 && j < i)),
+                        (int)__popcll(__ballot(in2 && j < i))};
+      int before[4], tot[4];
+      wave_prefix<4>(c, before, tot, pbuf);
       if (in3) {
-        const int p = n3 + __popcll(m3 & lanemask_lt());
+        const int p = n3 + before[0] + __popcll(m3 & lanemask_lt());
         if (p < cap) nb3[p] = j;
-        // upper entries (j > i) are staged at their position among the upper ones
-        const int pu = p - u3 - __popcll(lo3);
+        // upper entries (j > i) are staged at their position among the upper
+        // ones; the candidates ascend, so every lower entry of the round precedes them
+        const int pu = p - u3 - tot[2];
         if (do3 && j > i && pu < cap) stage_upper(lsm, cap, pu, xi, yi, zi, xj, yj, zj, L);
       }
       if (in2) {
-        const int p = n2 + __popcll(m2 & lanemask_lt());
+        const int p = n2 + before[1] + __popcll(m2 & lanemask_lt());
         if (p < cap) {
           nb2[p] = j;
           v2[p] = dx;
@@ -305,15 +351,15 @@ __global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
           v2[2 * cap + p] = dz;
         }
       }
-      n3 += __popcll(m3);
-      n2 += __popcll(m2);
-      u3 += __popcll(lo3);
-      u2 += __popcll(lo2);
+      n3 += tot[0];
+      n2 += tot[1];
+      u3 += tot[2];
+      u2 += tot[3];
     }
   }
   const int nn3 = n3 < cap ? n3 : cap, nn2 = n2 < cap ? n2 : cap;
   const int up3 = u3 < nn3 ? u3 : nn3, up2 = u2 < nn2 ? u2 : nn2;
-  if (lane == 0) {
+  if (tid == 0) {
     if (n3 > cap || n2 > cap) atomicMax(D.totals + T_NBOVER, n3 > n2 ? n3 : n2);
     D.nb3_cnt[i] = nn3;
     D.nb2_cnt[i] = nn2;
@@ -325,15 +371,16 @@ __global__ __launch_bounds__(64) void k_neighbours(Dev D, Consts K, int do3) {
   if (do3) {
     __syncthreads();  // the staged neighbours
     const int count = fs3_sweep<0>(D, K, i, nullptr, nn3 - up3, lsm, cap, L, xi, yi, zi, 0, nullptr);
-    if (lane == 0) D.cnt3s[i] = count;
+    if (tid == 0) D.cnt3s[i] = count;
   }
 }
 
+
 // ------------------------------------------------------------------ FS3 triplets
-// One wavefront (one 64-thread block) per molecule i: stages its upper FS3
+// One block (four wavefronts) per molecule i: stages its upper FS3
 // neighbours again and writes its triplets from off3s[i] (fs3_sweep), with the
 // geometry of each triplet and the SR / switch / LR histogram.
-__global__ __launch_bounds__(64) void k_fs3_fill(Dev D, Consts K) {
+__global__ __launch_bounds__(LB_THREADS) void k_fs3_fill(Dev D, Consts K) {
   extern __shared__ double lsm[];
   __shared__ int shist[3];
   const int lane = threadIdx.x;
@@ -345,7 +392,7 @@ __global__ __launch_bounds__(64) void k_fs3_fill(Dev D, Consts K) {
   const int* nb = D.nb3 + (long)i * cap + up;
   const double L = D.boxL[D.mol_box[i]];
   const double xi = D.cx[i], yi = D.cy[i], zi = D.cz[i];
-  for (int a = lane; a < m; a += 64) {
+  for (int a = lane; a < m; a += LB_THREADS) {
     const int j = nb[a];
     stage_upper(lsm, cap, a, xi, yi, zi, D.cx[j], D.cy[j], D.cz[j], L);
   }
