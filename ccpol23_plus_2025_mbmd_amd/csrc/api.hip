@@ -136,7 +136,7 @@ struct ccpol3b_ctx {
   float* d_coef32s = nullptr;    // [2][NCOEF] FP32, shifted pair-vector basis of the mixed kernels
   double* d_phitrip = nullptr;   // [cap_phi][728] per-triplet coefficient-gradient rows
   long cap_phi = 0;
-  ccpol3b_tuning tune{0, 1, 2, 1};
+  ccpol3b_tuning tune{0, 1, 1, 1};
   // creation parameters (Fortran-ordered 17x17 tables, current coefficients),
   // kept so a graph can own a private copy of the context
   std::vector<double> h_tab[6], h_cs[2];

@@ -248,14 +248,16 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   using T2 = typename Pair2<T>::type;
   // pair-vector basis (e, e (r - VSHIFT)); the coefficient table passed in is in the same basis
   constexpr double VSHIFT = sizeof(T) == 4 ? MIXED_VSHIFT : 0.0;
-  // sR holds r and then h = (dE/dr)/r.  The mixed variant's r is FP32-exact
-  // (FP32 square root), so T storage loses nothing.
+  // sR holds r (mixed: FP32-exact, from the FP32 square root, and reused by the
+  // second region) or 1/r (FP64, the by-product of the distance's reciprocal
+  // square root), and then h = (dE/dr)/r.
   __shared__ T sC[CSM ? NCOEF : 1];
   __shared__ double sP[3][NS3][3];
   __shared__ T2 sV[NPAIR3];
   __shared__ T sDer[NPAIR3];
   __shared__ T sR[NPAIR3];
-  __shared__ double sRed[4][3];
+  __shared__ double sRed[4][2];
+  __shared__ double sW[2];
   __shared__ TriGeo g;  // block-uniform geometry, read with lane-varying indices
   // PHI: dE_X/dC of the dense table [tx][ty][tz][p][q][s] of the current region
   __shared__ T sDC[PHI ? NCOEF : 1];
@@ -309,10 +311,15 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
       using PP = typename PairPar<T>::type;
       auto site_pair = [&](const double* pa, const double* pb, const PP& par, int p) {
         T rt;
-        if (rr == 0 || sizeof(T) == 8) {  // FP64: recomputing costs fewer registers than the branch
+        if constexpr (sizeof(T) == 8) {  // FP64: recomputing costs fewer registers than keeping r
+          const double vx = pa[0] - pb[0], vy = pa[1] - pb[1], vz = pa[2] - pb[2];
+          const double r2 = vx * vx + vy * vy + vz * vz, ir = rsqrt_nr(r2);
+          rt = r2 * ir;
+          if (rr == 0) sR[p] = ir;
+        } else if (rr == 0) {
           const double vx = pa[0] - pb[0], vy = pa[1] - pb[1], vz = pa[2] - pb[2];
           rt = t_dist<T>(vx * vx + vy * vy + vz * vz);
-          if (rr == 0) sR[p] = rt;
+          sR[p] = rt;
         } else {
           rt = sR[p];
         }
@@ -461,13 +468,11 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
       __syncthreads();
     }
   }
-  // the tile owner turns dE/dr into h = (dE/dr)/r and the site virial sum r dE/dr
-  double wsum = 0.0;
+  // the tile owner turns dE/dr into h = (dE/dr)/r
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   auto own_h = [&](int po, T ga) {
-    const T r = sR[po];
-    wsum += double(ga) * double(r);
-    sR[po] = ga * t_rcp<T>(r);
+    if constexpr (sizeof(T) == 8) sR[po] = ga * sR[po];
+    else sR[po] = ga * t_rcp<T>(sR[po]);
   };
   if (wave < 3) {
     const int lane = tid & 63, x0 = 1 + 2 * (lane >> 3), y0 = 1 + 2 * (lane & 7);
@@ -484,25 +489,22 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
       for (int b2 = 0; b2 < 2; ++b2)
         if (ts.own[0][b2] >= 0) own_h(ts.own[0][b2], gacc[0][b2]);
   }
-  // region energies and the site virial (FP64, fixed reduction trees: each
-  // wavefront's butterfly, then the four wavefront sums); the barrier also
-  // publishes h.  Only waves 0 and 3 hold pass-0 tiles, so only they have
-  // energy terms, and only switch-band triplets have a second region.
+  // region energies (FP64, fixed reduction trees: each wavefront's DPP sum,
+  // then the four wavefront sums); the barrier also publishes h.  Only waves 0
+  // and 3 hold pass-0 tiles, so only they have energy terms, and only
+  // switch-band triplets have a second region.
   {
     const bool has_e = wave == 0 || wave == 3;
     const double s0 = has_e ? wave_sum_lane63(eacc0) : 0.0;
     const double s1 = has_e && nreg == 2 ? wave_sum_lane63(eacc1) : 0.0;
-    const double s2 = wave_sum_lane63(wsum);
     if ((tid & 63) == 63) {
       sRed[tid >> 6][0] = s0;
       sRed[tid >> 6][1] = s1;
-      sRed[tid >> 6][2] = s2;
     }
   }
   __syncthreads();
   const double EX0 = (sRed[0][0] + sRed[1][0]) + (sRed[2][0] + sRed[3][0]);
   const double EX1 = (sRed[0][1] + sRed[1][1]) + (sRed[2][1] + sRed[3][1]);
-  const double Wsites = (sRed[0][2] + sRed[1][2]) + (sRed[2][2] + sRed[3][2]);
   const double Eb = g.w[0] * EX0 + g.w[1] * EX1;
   const double Et = g.fd * Eb;
   const double dEdR = g.dfd * Eb + g.fd * (g.wp[0] * EX0 + g.wp[1] * EX1);  // dE/dR_avg
@@ -513,8 +515,12 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   // T = sum_u h_u Q_u, whichever molecule comes first in the block.  One code
   // path for every lane; the two partners of a (role, site) sit in adjacent
   // lanes and are added there, and the sums are staged in LDS for the atomics.
+  // The site virial sum_pairs r dE/dr equals -sum_sites Q . F (the energy
+  // depends on site differences only); each thread adds -Q_own . F of its
+  // partner's share, summed over waves 0-1 with the DPP tree.
   double* sF = reinterpret_cast<double*>(sV);  // site forces [role][site][c]; the pair vectors are dead here
   static_assert(sizeof(sV) >= 3 * NS3 * 3 * sizeof(double), "site-force staging");
+  double wpart = 0.0;
   if (tid < 102) {
     const int role = tid / 34, r = tid - role * 34, s = r >> 1, half = r & 1;
     const int blk = half ? (role == 0 ? 1 : 2) : (role == 2 ? 1 : 0);  // ij, ik, jk
@@ -535,6 +541,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     double F0 = fma(-sP[role][s][0], H, T0);
     double F1 = fma(-sP[role][s][1], H, T1);
     double F2 = fma(-sP[role][s][2], H, T2);
+    wpart = -(sP[role][s][0] * F0 + sP[role][s][1] * F1 + sP[role][s][2] * F2);
     F0 += __shfl_xor(F0, 1, 64);
     F1 += __shfl_xor(F1, 1, 64);
     F2 += __shfl_xor(F2, 1, 64);
@@ -555,6 +562,10 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
     else F = g3 * (-uik - ujk);
     fx_add(D, (long)g.m[role] * ACC_W + ACC_C3 + c, F);
   }
+  if (wave < 2) {
+    const double ws = wave_sum_lane63(wpart);
+    if ((tid & 63) == 63) sW[wave] = ws;
+  }
   __syncthreads();
   // one fixed-point atomic per site-force component, consecutive lanes on
   // consecutive accumulator entries (51 per molecule)
@@ -564,7 +575,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
   }
   if (tid == 0) {
     D.ew3[2 * t] = Et;
-    D.ew3[2 * t + 1] = Wsites + g.Ravg * dEdR;
+    D.ew3[2 * t + 1] = (sW[0] + sW[1]) + g.Ravg * dEdR;
   }
 }
 

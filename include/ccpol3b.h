@@ -185,7 +185,7 @@ typedef struct {
   int fs3_lds_coef;           /* 1: FS3 blocks stage the region's coefficient table in LDS;
                                  0: they read it through the vector cache (default) */
   int fs2_parts;              /* FS2 spectator-pass blocks per molecule, 1..8 (default 1) */
-  int fs2_pair_parts;         /* FS2 pair-pass blocks per molecule, 1..8 (default 2) */
+  int fs2_pair_parts;         /* FS2 pair-pass blocks per molecule, 1..8 (default 1) */
   int neighbour_cells;        /* neighbour search in periodic boxes: 1 (default) a cell grid where
                                  the box holds at least 6 cells a side of the larger cutoff, 2 a
                                  grid from 4 cells a side, 0 every molecule of the box tested.
