@@ -170,6 +170,7 @@ struct ccpol3b_ctx {
   // when that was the last call
   Dev nD{};
   bool last_noalloc = false;
+  long last_n3 = 0;  // FS3 triplets of the last completed call (sizes the next evaluation grid)
   long version = 0;  // bumped by set_precision, set_coefficients, set_tuning
 };
 
@@ -293,6 +294,19 @@ int ensure_lists(ccpol3b_ctx* c, long n3, long ni2) {
 
 int waves_grid(long nwaves) { return (int)((nwaves * 64 + 255) / 256); }
 
+// Blocks of a per-call FS3 evaluation (one per triplet): 96 per molecule of
+// this call (bulk water has ~61 triplets per molecule, counted by its first
+// member), or the last call's count with 1/8 headroom if that is more, at most
+// the capacity; so a context whose capacity a large batch has grown does not
+// launch millions of empty blocks for a small box.  A list longer than the grid
+// is flagged like a capacity overflow (nothing is written) and evaluated again
+// with a grid that covers it.  Recorded graphs and the allocation-free entry
+// point launch the full capacity.
+long fs3_grid(const ccpol3b_ctx* c, int nm, long cap3) {
+  const long want = std::max<long>({1024, (long)nm * 96, c->last_n3 + c->last_n3 / 8});
+  return std::max<long>(1, std::min(cap3, want));
+}
+
 float event_ms(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
@@ -366,6 +380,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   }
   D.fs2_parts = c->tune.fs2_parts;
   D.fs2_pair_parts = c->tune.fs2_pair_parts;
+  D.grid3 = capture || fixed ? D.cap3 : fs3_grid(c, nm, D.cap3);
   if (!fixed && !capture &&
       (c->h_molbox != c->up_molbox || c->h_boxL != c->up_boxL || c->h_boxstart != c->up_boxstart)) {
     forget_upload(c);
@@ -437,7 +452,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   const long cap3 = D.cap3;
   EVREC_SIDE(8);
   if (do3) {
-    const int grid = (int)cap3;  // one block per triplet; blocks past the list exit at once
+    const int grid = (int)D.grid3;  // one block per triplet; blocks past the list exit at once
     if (phi) {
       if (c->precision == CCPOL3B_MIXED)
         hipLaunchKernelGGL(k_fs3_phi_f32, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar32,
@@ -515,6 +530,11 @@ int run_checked(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const
     if (tot[T_N3] > D.cap3) {
       int rc2 = ensure_lists(c, tot[T_N3], 0);
       if (rc2) return rc2;
+      c->last_n3 = tot[T_N3];
+      continue;
+    }
+    if (tot[T_N3] > D.grid3) {  // more triplets than evaluation blocks: again with enough
+      c->last_n3 = tot[T_N3];
       continue;
     }
     return CCPOL3B_OK;
@@ -538,6 +558,7 @@ void finish_stats(ccpol3b_ctx* c) {
   c->stats.ms_total = event_ms(c->ev[0], c->ev[4]);
   c->stats.ms_fs3_eval = c->stats.n_fs3 > 0 ? event_ms(c->ev[8], c->ev[3]) : 0.0;
   c->stats.n_cells = c->last_noalloc ? 0 : c->D.ncells;
+  c->last_n3 = c->s3 ? tot[T_N3] : 0;
 }
 
 void set_single_box(ccpol3b_ctx* c, int nm, double box) {

@@ -88,6 +88,7 @@ struct Dev {
   int fs2_parts;              // spectator-pass blocks per molecule (1..FS2_PARTS_MAX)
   int fs2_pair_parts;         // pair-pass blocks per molecule (1..FS2_PARTS_MAX)
   long cap3, cap_pairs2, cap_items2, cap_sl2;
+  long grid3;                 // blocks of the FS3 evaluation (<= cap3): a longer list overflows like cap3
   // geometry in bohr
   double *cx, *cy, *cz;   // [nm]
   double *site;           // [nm][17][3] site offsets (bohr)

@@ -30,7 +30,7 @@ __device__ __forceinline__ double sq3_nc(double x, double y, double z) {
 // capacities), or a force fell outside the fixed-point range (the call fails):
 // either way the caller's outputs are not written.
 __device__ __forceinline__ bool lists_overflowed(const Dev& D) {
-  return D.totals[T_NBOVER] > 0 || D.totals[T_N3] > D.cap3 || D.totals[T_FXOVER] != 0;
+  return D.totals[T_NBOVER] > 0 || D.totals[T_N3] > D.cap3 || D.totals[T_N3] > D.grid3 || D.totals[T_FXOVER] != 0;
 }
 
 // Forces are summed in 128-bit fixed point, in units of 2^-96 hartree/bohr,
