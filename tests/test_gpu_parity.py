@@ -89,6 +89,28 @@ def assert_parity(got, ref, s2=True, s3=True, e_tol=1e-10, f_tol=1e-9, com_tol=C
     assert abs(got["vir"] - ref["vir"]) <= 10 * e_tol * abs(ref["vir"]) + 418.4 * E_FLOOR, (got["vir"], ref["vir"])
 
 
+def test_fs3_list_longer_than_first_grid(oracle, params_fs3):
+    """A per-call FS3 evaluation launches one block per triplet for 96 triplets
+    per molecule, or the last call's count if that is more.  125 waters at 1.4x
+    water density hold 156 per molecule: the first call on a fresh context is
+    flagged, writes nothing, and is evaluated again with a grid that covers the
+    list.  Its outputs match the oracle, and equal bit for bit those of the next
+    call, whose grid comes from the first call's count."""
+    from ccpol23_plus_2025_mbmd_amd.threebody import ThreeBody
+    n = 125
+    box = lattice_box(n, seed=9, box=box_length(n) * (1 / 1.4) ** (1 / 3))
+    print("fs3 list longer than the first grid: 125 waters at 1.4x density", flush=True)
+    p = params_fs3.replace(s2_term=False)
+    tb = ThreeBody(p, "fp64")
+    first = tb.evaluate(box)
+    assert first["n_fs3"] > 96 * n
+    second = tb.evaluate(box)
+    for k in ("qfs3", "vir", "fxs", "fys", "fzs", "fxfs3", "fyfs3", "fzfs3"):
+        assert np.array_equal(first[k], second[k]), k
+    np.testing.assert_array_equal(tb.fs3_list(), oracle.fs3_list(p, box.x, box.y, box.z, box.box))
+    assert_parity(first, oracle.threebody_term(p, box), s2=False)
+
+
 @pytest.mark.parametrize("n,seed", [(64, 1), (64, 2), (256, 3)])
 def test_lists_bit_exact(tb_fs2fs3, oracle, params_fs2fs3, n, seed):
     box = lattice_box(n, seed=seed)
