@@ -40,8 +40,8 @@ template <bool MIXED> __global__ void k_fs2_spectator_pass(Dev, Consts);
 size_t fs2_lds_bytes(int cap_nb);
 size_t fs2_pair_lds_bytes(int cap_nb);
 size_t fs3_list_lds_bytes(int cap_nb);
-__global__ void k_epilogue(Dev, int, int, int, int, double*, double*, double*, double*, double*, long, long, long);
-__global__ void k_energy_boxes(Dev, int, int, double*);
+__global__ void k_epilogue(Dev, int, int, int, int, double*, double*, double*, double*, double*, long, long, long,
+                           double*, int*);
 }  // namespace ccpol3b
 
 using namespace ccpol3b;
@@ -148,6 +148,7 @@ struct ccpol3b_ctx {
   Dev D{};
   int cap_nm = 0, cap_box = 0;
   int* h_totals = nullptr;  // pinned: [0, T_COUNT) mid-call copy, [T_COUNT, 2 T_COUNT) end of call
+  int* d_htotals = nullptr; // h_totals + T_COUNT as the device addresses it (null: copied instead)
   // staging for the host entry point
   double* d_in = nullptr;   // x,y,z,xs,ys,zs
   double* d_out = nullptr;  // fxs,fys,fzs (ld=17), fcom2, fcom3, energies
@@ -491,11 +492,11 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   const long nch3 = do3 ? cap3 / RED_CHUNK + nbox + 1 : 0;
   const long nch2 = do2 ? (long)nm * D.fs2_parts / RED_CHUNK + nbox + 1 : 0;
   hipLaunchKernelGGL(k_epilogue, dim3(nfin + nch3 + nch2), dim3(256), 0, st, D, ld, do3 ? NS3 : (do2 ? NS2 : 0),
-                     do2, do3, fxs, fys, fzs, fcom2, fcom3, nfin, nch3, nch2);
-  hipLaunchKernelGGL(k_energy_boxes, dim3(nbox), dim3(64), 0, st, D, do2, do3, energies);
+                     do2, do3, fxs, fys, fzs, fcom2, fcom3, nfin, nch3, nch2, energies, c->d_htotals);
   LAUNCHCHK(c, "finish");
   EVREC(4);
-  HIPCHK(c, hipMemcpyAsync(c->h_totals + T_COUNT, D.totals, T_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
+  if (!c->d_htotals)
+    HIPCHK(c, hipMemcpyAsync(c->h_totals + T_COUNT, D.totals, T_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
   return CCPOL3B_OK;
 #undef EVREC
 }
@@ -766,9 +767,18 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
   }
   if (hipMemcpy(c->d_pairpar, pairpar.data(), sizeof(double4) * 2 * NPAIR, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_pairpar32, pairpar32.data(), sizeof(float4) * 2 * NPAIR, hipMemcpyHostToDevice) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_totals, 2 * T_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_totals, 2 * T_COUNT * sizeof(int), hipHostMallocMapped) != hipSuccess) {
     ccpol3b_destroy(c);
     return CCPOL3B_EHIP;
+  }
+  {
+    // the epilogue writes the end-of-call counters straight into the pinned
+    // buffer when the device can address it (else they are copied)
+    void* dp = nullptr;
+    if (hipHostGetDevicePointer(&dp, c->h_totals, 0) == hipSuccess && dp)
+      c->d_htotals = static_cast<int*>(dp) + T_COUNT;
+    else
+      (void)hipGetLastError();
   }
   for (auto& e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) {

@@ -31,6 +31,7 @@ constexpr int T_NI2L = 9;    // FS2 items of an exported list
 constexpr int T_NCH3 = 10;   // FS3 / FS2 energy reduction chunks
 constexpr int T_NCH2 = 11;
 constexpr int T_FXOVER = 12; // a force contribution fell outside the fixed-point range
+constexpr int T_DONE = 13;   // epilogue blocks finished (the last one forms the box sums)
 constexpr int T_COUNT = 16;
 constexpr int FS2_PARTS_MAX = 8; // spectator-pass blocks per molecule, at most
 constexpr int RED_CHUNK = 2048;  // entries per block in the first level of the energy reduction

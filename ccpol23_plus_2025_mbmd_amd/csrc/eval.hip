@@ -637,29 +637,12 @@ __device__ __forceinline__ void energy_chunk(const Dev& D, int which, long q) {
   }
 }
 
-// The call's epilogue in one launch: blocks [0, nfin) write the accumulators
-// back, the next nch3 blocks reduce FS3 energy chunks and the last nch2 blocks
-// FS2 energy chunks.  Nothing is written when a list overflowed.
-__global__ __launch_bounds__(256) void k_epilogue(Dev D, int ld, int nsite, int do2, int do3, double* fxs,
-                                                  double* fys, double* fzs, double* fcom2, double* fcom3,
-                                                  long nfin, long nch3, long nch2) {
-  if (lists_overflowed(D)) return;
-  const long b = blockIdx.x;
-  if (b < nfin)
-    finish_one(D, b * blockDim.x + threadIdx.x, ld, nsite, do2, do3, fxs, fys, fzs, fcom2, fcom3);
-  else if (b < nfin + nch3)
-    energy_chunk(D, 3, b - nfin);
-  else if (b < nfin + nch3 + nch2)
-    energy_chunk(D, 2, b - nfin - nch3);
-}
-
-// Level 2: one wavefront per box; lane l sums chunks l, l+64, ... of its box in
-// order, then a fixed butterfly -> {qfs2, qfs3 (kcal/mol), vir increment
-// (10 J/mol)}.  The order depends only on the chunk layout, not on timing.
-__global__ __launch_bounds__(64) void k_energy_boxes(Dev D, int do2, int do3, double* out) {
-  const int b = blockIdx.x;
-  if (b >= D.nbox || lists_overflowed(D)) return;
-  const int lane = threadIdx.x;
+// Level 2 of the energy sums, one wavefront per box: lane l sums chunks l,
+// l+64, ... of its box in order, then a fixed butterfly -> {qfs2, qfs3
+// (kcal/mol), vir increment (10 J/mol)}.  The order depends only on the chunk
+// layout, not on timing.
+__device__ __forceinline__ void box_energies(const Dev& D, int b, int do2, int do3, double* out) {
+  const int lane = threadIdx.x & 63;
   double e2 = 0.0, w2 = 0.0, e3 = 0.0, w3 = 0.0;
   if (do2)
     for (int q = D.box_c2[b] + lane; q < D.box_c2[b + 1]; q += 64) {
@@ -679,6 +662,43 @@ __global__ __launch_bounds__(64) void k_energy_boxes(Dev D, int do2, int do3, do
     out[3 * b + 0] = e2 * H2KCAL;
     out[3 * b + 1] = H2KCAL * e3;
     out[3 * b + 2] = w2 * H2KCAL * 418.4 + w3 * H2KCAL * 418.4;
+  }
+}
+
+// The call's epilogue in one launch: blocks [0, nfin) write the accumulators
+// back, the next nch3 blocks reduce FS3 energy chunks and the last nch2 blocks
+// FS2 energy chunks.  The last block to finish (a counter in totals) then forms
+// the per-box sums and, with host_totals, copies the call's counters to the
+// host (pinned memory mapped into the device's address space).  Nothing but
+// the counters is written when a list overflowed.
+__global__ __launch_bounds__(256) void k_epilogue(Dev D, int ld, int nsite, int do2, int do3, double* fxs,
+                                                  double* fys, double* fzs, double* fcom2, double* fcom3,
+                                                  long nfin, long nch3, long nch2, double* energies,
+                                                  int* host_totals) {
+  __shared__ bool last;
+  const bool over = lists_overflowed(D);
+  const long b = blockIdx.x;
+  if (!over) {
+    if (b < nfin)
+      finish_one(D, b * blockDim.x + threadIdx.x, ld, nsite, do2, do3, fxs, fys, fzs, fcom2, fcom3);
+    else if (b < nfin + nch3)
+      energy_chunk(D, 3, b - nfin);
+    else if (b < nfin + nch3 + nch2)
+      energy_chunk(D, 2, b - nfin - nch3);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();  // this block's results before its count
+    last = atomicAdd(D.totals + T_DONE, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // every other block's results after their counts
+  if (!over)
+    for (int bx = threadIdx.x >> 6; bx < D.nbox; bx += 4) box_energies(D, bx, do2, do3, energies);
+  if (host_totals && threadIdx.x < T_COUNT) {
+    host_totals[threadIdx.x] = __atomic_load_n(D.totals + threadIdx.x, __ATOMIC_RELAXED);
+    __threadfence_system();
   }
 }
 
