@@ -422,7 +422,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
   }
   // neighbour lists with the per-molecule FS3 triplet counts, then every offset
   hipLaunchKernelGGL(k_neighbours, dim3(nm), dim3(256), ldsn, st, D, c->K, do3);
-  hipLaunchKernelGGL(k_scan<1024>, dim3(1), dim3(1024), 0, st, D, 0, do3, do2);
+  hipLaunchKernelGGL(k_scan<256>, dim3(1), dim3(256), 0, st, D, 0, do3, do2);
   LAUNCHCHK(c, "neighbour lists");
   EVREC(1);
   // Two branches from here: the FS2 passes on `st`, and the FS3 list fill

@@ -703,10 +703,9 @@ __global__ __launch_bounds__(256) void k_epilogue(Dev D, int ld, int nsite, int 
 }
 
 // Coefficients come through the vector cache; LDS per block is ~29 KB (FP64,
-// 5 blocks per CU, at the cost of a few spilled registers) and ~15 KB (mixed,
-// 8 blocks per CU by registers, also with a few spills: measured 2-3 % faster
-// than 7 without).  The mixed kernels take the table in the shifted basis
-// (MIXED_VSHIFT), the FP64 kernels the plain one.
+// 96 VGPRs: 5 blocks per CU) and ~15 KB (mixed, 58 VGPRs: 8 blocks per CU),
+// neither with spilled registers.  The mixed kernels take the table in the
+// shifted basis (MIXED_VSHIFT), the FP64 kernels the plain one.
 __global__ __launch_bounds__(256, 5) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
                                                          const double* __restrict__ coef) {
   fs3_eval_body<double, false, false>(D, K, pairpar, coef, nullptr);

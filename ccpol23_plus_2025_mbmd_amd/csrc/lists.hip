@@ -619,5 +619,6 @@ __global__ __launch_bounds__(NT) void k_scan(Dev D, int phase, int do3, int do2)
   }
 }
 template __global__ void k_scan<1024>(Dev, int, int, int);
+template __global__ void k_scan<256>(Dev, int, int, int);
 
 }  // namespace ccpol3b
