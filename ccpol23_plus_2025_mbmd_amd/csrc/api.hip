@@ -133,6 +133,7 @@ struct ccpol3b_ctx {
   double4* d_pairpar = nullptr;  // [2][289]
   float4* d_pairpar32 = nullptr; // [2][289] FP32 copy for the mixed kernels
   double* d_coef64 = nullptr;    // [2][NCOEF]
+  double* d_coef64i = nullptr;   // [2][NCOEF] the same in the interleaved layout of k_fs3_eval_f64
   float* d_coef32s = nullptr;    // [2][NCOEF] FP32, shifted pair-vector basis of the mixed kernels
   double* d_phitrip = nullptr;   // [cap_phi][728] per-triplet coefficient-gradient rows
   long cap_phi = 0;
@@ -474,7 +475,7 @@ int run(ccpol3b_ctx* c, int nm, int nbox, int ld, const double* x, const double*
                            (const double*)c->d_coef64);
       else
         hipLaunchKernelGGL(k_fs3_eval_f64, dim3(grid), dim3(256), 0, s3, D, c->K, c->d_pairpar,
-                           (const double*)c->d_coef64);
+                           (const double*)c->d_coef64i);
     }
     LAUNCHCHK(c, "k_fs3_eval");
   }
@@ -577,18 +578,21 @@ void* noalloc_scratch(int dev);
 void set_batch(ccpol3b_ctx* c, int nbox, const int* mol_start, const double* box_len);
 int clone_ctx(const ccpol3b_ctx* c, ccpol3b_ctx** out);
 
-// Dense coefficient tables C[X][tx][ty][tz][p][q][s] = cs_X[orbit] (FP64), and
+// Dense coefficient tables C[X][tx][ty][tz][p][q][s] = cs_X[orbit] (FP64), the
+// same entries in the interleaved layout of the FP64 evaluation (cil_index), and
 // the FP32 table in the shifted pair-vector basis of the mixed kernels: with v = (e, e r) = T v', v' = (e, e (r - c)), T = [[1, 0], [c, 1]],
 // the energy sum_pqs C v_p v_q v_s equals sum C' v'_p v'_q v'_s with
 // C'[p'q's'] = sum_pqs C[pqs] T[p][p'] T[q][q'] T[s][s'] (each type block of 8).
 int upload_coefficients(ccpol3b_ctx* c, const double* cs_sr, const double* cs_lr) {
-  std::vector<double> c64(2 * NCOEF);
+  std::vector<double> c64(2 * NCOEF), c64i(2 * NCOEF);
   std::vector<float> c32s(2 * NCOEF);
   const std::vector<int>& dob = dense_orbit();
   for (int X = 0; X < 2; ++X) {
     const double* cs = X ? cs_lr : cs_sr;
     for (int d = 0; d < NCOEF; ++d) {
       c64[(size_t)X * NCOEF + d] = cs[dob[d]];
+      const int k = d % 8, tz = d / 8 % NTYPE, tuw = d / (8 * NTYPE);  // d = (tuw NTYPE + tz) 8 + k
+      c64i[(size_t)X * NCOEF + cil_index(tz, k) + 2 * tuw] = cs[dob[d]];
     }
     for (int blk = 0; blk < NCOEF / 8; ++blk) {
       double b[8];
@@ -601,6 +605,7 @@ int upload_coefficients(ccpol3b_ctx* c, const double* cs_sr, const double* cs_lr
     }
   }
   if (hipMemcpy(c->d_coef64, c64.data(), sizeof(double) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_coef64i, c64i.data(), sizeof(double) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_coef32s, c32s.data(), sizeof(float) * 2 * NCOEF, hipMemcpyHostToDevice) != hipSuccess)
     return fail(c, CCPOL3B_EHIP, "coefficient upload");
   return CCPOL3B_OK;
@@ -760,6 +765,7 @@ int ccpol3b_create(const ccpol3b_params* p, ccpol3b_ctx** out) {
   int rc = CCPOL3B_OK;
   if ((rc = dev_alloc(c, c->d_pairpar, 2 * NPAIR)) || (rc = dev_alloc(c, c->d_pairpar32, 2 * NPAIR)) ||
       (rc = dev_alloc(c, c->d_coef64, 2 * NCOEF)) ||
+      (rc = dev_alloc(c, c->d_coef64i, 2 * NCOEF)) ||
       (rc = dev_alloc(c, c->d_coef32s, 2 * NCOEF)) ||
       (rc = upload_coefficients(c, p->cs_sr, p->cs_lr))) {
     ccpol3b_destroy(c);
@@ -803,6 +809,7 @@ void ccpol3b_destroy(ccpol3b_ctx* c) {
   dev_free(c->d_pairpar);
   dev_free(c->d_pairpar32);
   dev_free(c->d_coef64);
+  dev_free(c->d_coef64i);
   dev_free(c->d_coef32s);
   dev_free(c->d_phitrip);
   dev_free(c->d_gpart);

@@ -44,6 +44,17 @@ __host__ __device__ constexpr int site_type(int s) {
   return s == 0 ? 0 : s <= 2 ? 1 : s <= 4 ? 2 : s <= 8 ? 3 : s <= 12 ? 4 : 5;
 }
 
+// Interleaved layout of a region's coefficient table, read by the FP64
+// evaluation kernel: entry (tu, tw, tz, k) of the dense table
+// C[tu][tw][tz][k] (k = 4 p + 2 q + s) sits at cil_index(tz, k) + 2 (tu NTYPE + tw).
+// For one type block tz and one pair of coefficients (2 k4, 2 k4 + 1) the 36
+// type pairs are adjacent, so a 16-byte load of the lanes of a wavefront, which
+// hold different type pairs, touches 576 contiguous bytes instead of 36 lines.
+constexpr int CIL_K4 = NTYPE * NTYPE * 2;  // step of k / 2
+constexpr int CIL_TZ = 4 * CIL_K4;         // step of tz
+__host__ __device__ constexpr int cil_index(int tz, int k) { return tz * CIL_TZ + (k >> 1) * CIL_K4 + (k & 1); }
+static_assert(cil_index(NTYPE - 1, 7) + 2 * (NTYPE * NTYPE - 1) == NCOEF - 1, "interleaved layout spans the table");
+
 // Mixed-precision FS3 evaluation: the second component of each site-pair
 // vector is e (r - MIXED_VSHIFT) instead of e r, with the coefficient table
 // transformed to that basis exactly (in FP64, once per coefficient upload).

@@ -51,7 +51,9 @@ __host__ __device__ constexpr PassLayout pass_layout(int P) {
 // Both loops are unrolled, so with compile-time strides every LDS read is a
 // base register plus an immediate offset.  With dc (PHI), also
 //   dE/dC[tu][tw][tz][p][q][s] += sum_ab vown[a][b][s] M[a][b][p][q].
-template <typename T, int NU, int NW, bool PHI>
+// cp points at the tile's coefficients: C + (tu NTYPE + tw) NTYPE 8 in the
+// dense layout, C + 2 (tu NTYPE + tw) in the interleaved one (CIL, cil_index).
+template <typename T, int NU, int NW, bool PHI, bool CIL = false>
 __device__ __forceinline__ void contract_tile(const typename Pair2<T>::type* U, int URS, int UZS,
                                               const typename Pair2<T>::type* W, int WRS, int WZS,
                                               const T* __restrict__ cp, T (&G)[NU][NW][2], T* dc,
@@ -124,7 +126,7 @@ __device__ __forceinline__ void contract_tile(const typename Pair2<T>::type* U, 
     }
     T cc[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) cc[k] = cp[tz * 8 + k];
+    for (int k = 0; k < 8; ++k) cc[k] = CIL ? cp[cil_index(tz, k)] : cp[tz * 8 + k];
 #pragma unroll
     for (int a = 0; a < NU; ++a)
 #pragma unroll
@@ -240,11 +242,12 @@ int set_orbit_tables(const short* off, const short* mem) {
 // CSM: stage the region's coefficient table in LDS; otherwise the tile threads
 // read it through the vector cache (less LDS per block, more blocks per CU).
 // pairpar: [2][289] (beta, gamma, r0, -) for SR and LR (FP32 copy for the mixed
-// kernels); coef: [2][NCOEF] dense
-// [tx][ty][tz][p][q][s].
-template <typename T, bool PHI, bool CSM>
+// kernels); coef: [2][NCOEF] dense [tx][ty][tz][p][q][s], or with CIL in the
+// interleaved layout of cil_index (read through the vector cache only).
+template <typename T, bool PHI, bool CSM, bool CIL = false>
 __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename PairPar<T>::type* __restrict__ pairpar,
                                               const T* __restrict__ coef, double* __restrict__ phi_trip) {
+  static_assert(!(CIL && (PHI || CSM)), "the interleaved table is read by the plain evaluation only");
   using T2 = typename Pair2<T>::type;
   // pair-vector basis (e, e (r - VSHIFT)); the coefficient table passed in is in the same basis
   constexpr double VSHIFT = sizeof(T) == 4 ? MIXED_VSHIFT : 0.0;
@@ -389,8 +392,9 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
                 vown[a][b2][1] = v.y;
               }
           }
-          contract_tile<T, 2, 2, PHI>(sV + L.ablk * NPAIR + x0 * L.arow, L.arow, L.az,
-                                      sV + L.bblk * NPAIR + y0 * L.brow, L.brow, L.bz, cbase + tb, G, dc, vown);
+          contract_tile<T, 2, 2, PHI, CIL>(sV + L.ablk * NPAIR + x0 * L.arow, L.arow, L.az,
+                                           sV + L.bblk * NPAIR + y0 * L.brow, L.brow, L.bz,
+                                           cbase + (CIL ? 2 * (tx * NTYPE + ty) : tb), G, dc, vown);
           // the tile's site pairs (x0 + a, y0 + b2) of block P: the energy is
           // sum_xy v_xy G_xy over pass 0
 #pragma unroll
@@ -414,7 +418,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
         asm volatile("" : "+v"(tq));
         const TileSpec ts = tile_spec(tq);
         if (ts.used) {
-          const T* cp = cbase + (ts.tu * NTYPE + ts.tw) * NTYPE * 8;
+          const T* cp = cbase + (CIL ? 2 * (ts.tu * NTYPE + ts.tw) : (ts.tu * NTYPE + ts.tw) * NTYPE * 8);
           T* dc = nullptr;
           if constexpr (PHI)
             if (ts.P == 0) dc = sDC + (ts.tu * NTYPE + ts.tw) * NTYPE * 8;
@@ -428,7 +432,7 @@ __device__ __forceinline__ void fs3_eval_body(Dev D, Consts K, const typename Pa
                 vown[0][b2][0] = po < 0 ? T(0) : v.x;
                 vown[0][b2][1] = po < 0 ? T(0) : v.y;
               }
-          contract_tile<T, 1, 2, PHI>(sV + ts.u_off, 0, ts.u_zs, sV + ts.w_off, ts.w_rs, ts.w_zs, cp, G, dc, vown);
+          contract_tile<T, 1, 2, PHI, CIL>(sV + ts.u_off, 0, ts.u_zs, sV + ts.w_off, ts.w_rs, ts.w_zs, cp, G, dc, vown);
 #pragma unroll
           for (int b2 = 0; b2 < 2; ++b2) {
             const int po = ts.own[0][b2];
@@ -705,10 +709,11 @@ __global__ __launch_bounds__(256) void k_epilogue(Dev D, int ld, int nsite, int 
 // Coefficients come through the vector cache; LDS per block is ~29 KB (FP64,
 // 96 VGPRs: 5 blocks per CU) and ~15 KB (mixed, 58 VGPRs: 8 blocks per CU),
 // neither with spilled registers.  The mixed kernels take the table in the
-// shifted basis (MIXED_VSHIFT), the FP64 kernels the plain one.
+// shifted basis (MIXED_VSHIFT), the FP64 kernels the plain one; k_fs3_eval_f64
+// takes it in the interleaved layout (cil_index).
 __global__ __launch_bounds__(256, 5) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
-                                                         const double* __restrict__ coef) {
-  fs3_eval_body<double, false, false>(D, K, pairpar, coef, nullptr);
+                                                         const double* __restrict__ coef_il) {
+  fs3_eval_body<double, false, false, true>(D, K, pairpar, coef_il, nullptr);
 }
 
 __global__ __launch_bounds__(256, 8) void k_fs3_eval_f32(Dev D, Consts K, const float4* __restrict__ pairpar,
