@@ -69,6 +69,15 @@ __device__ __forceinline__ void contract_tile(const typename Pair2<T>::type* U, 
     // the first one initialises M
     T M[NU][NW][2][2];
     T uv[NU][2], wv[NW][2];
+    // the type block's coefficients: the FP32 kernels issue the loads before
+    // the site loop, which hides their latency behind it (the FP64 kernels have
+    // no registers to spare for that and load them after it)
+    T cc[8];
+    auto load_c = [&]() {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) cc[k] = CIL ? cp[cil_index(tz, k)] : cp[tz * 8 + k];
+    };
+    if constexpr (sizeof(T) == 4) load_c();
     auto load = [&]() {
 #pragma unroll
       for (int a = 0; a < NU; ++a) {
@@ -124,9 +133,7 @@ __device__ __forceinline__ void contract_tile(const typename Pair2<T>::type* U, 
             }
       }
     }
-    T cc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) cc[k] = CIL ? cp[cil_index(tz, k)] : cp[tz * 8 + k];
+    if constexpr (sizeof(T) == 8) load_c();
 #pragma unroll
     for (int a = 0; a < NU; ++a)
 #pragma unroll
