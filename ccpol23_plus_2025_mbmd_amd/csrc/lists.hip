@@ -454,12 +454,13 @@ __global__ __launch_bounds__(256) void k_fs2_items(Dev D, Consts K, int mode) {
 // the sum.
 // Exclusive scans of K int arrays of length n together, by one block of NT
 // threads: off[k][q] = sum of cnt[k][0..q-1], and off[k][n] and *total[k]
-// receive the sums.  off may alias cnt (each tile is read before it is
-// written).  Tiles are staged in buf (SCAN_BUF ints shared by the K arrays).
+// receive the sums; the last array's sum is also returned to every thread.
+// off may alias cnt (each tile is read before it is written).  Tiles are
+// staged in buf (SCAN_BUF ints shared by the K arrays).
 constexpr int SCAN_BUF = 8192;
 
 template <int NT, int K>
-__device__ void block_exclusive_scan(int* buf, const int* const (&cnt)[K], int* const (&off)[K], long n,
+__device__ long block_exclusive_scan(int* buf, const int* const (&cnt)[K], int* const (&off)[K], long n,
                                      int* const (&total)[K]) {
   constexpr int PER = SCAN_BUF / (K * NT);  // entries per thread and array
   constexpr int TILE = NT * PER;
@@ -531,7 +532,9 @@ __device__ void block_exclusive_scan(int* buf, const int* const (&cnt)[K], int* 
       off[k][n] = (int)carry_s[k];
       *total[k] = (int)carry_s[k];
     }
+  const long last = carry_s[K - 1];
   __syncthreads();
+  return last;
 }
 
 // One block: exclusive scan of the cell counts into cell_start, then every
@@ -588,11 +591,12 @@ __global__ __launch_bounds__(NT) void k_scan(Dev D, int phase, int do3, int do2)
   int* const T = D.totals;
   if (phase == 0) {
     // the per-molecule scans of both terms together
+    long n3 = 0;
     if (do2 && do3) {
       const int* const c[3] = {D.sl2_cnt, D.pcnt2, D.cnt3s};
       int* const o[3] = {D.sl2_off, D.poff2, D.off3s};
       int* const tt[3] = {T + T_SL2, T + T_NP2, T + T_N3};
-      block_exclusive_scan<NT, 3>(buf, c, o, D.nm, tt);
+      n3 = block_exclusive_scan<NT, 3>(buf, c, o, D.nm, tt);
     } else if (do2) {
       const int* const c[2] = {D.sl2_cnt, D.pcnt2};
       int* const o[2] = {D.sl2_off, D.poff2};
@@ -602,7 +606,26 @@ __global__ __launch_bounds__(NT) void k_scan(Dev D, int phase, int do3, int do2)
       const int* const c[1] = {D.cnt3s};
       int* const o[1] = {D.off3s};
       int* const tt[1] = {T + T_N3};
-      block_exclusive_scan<NT, 1>(buf, c, o, D.nm, tt);
+      n3 = block_exclusive_scan<NT, 1>(buf, c, o, D.nm, tt);
+    }
+    if (D.nbox == 1) {
+      // one box (molecules [0, nm)): its ranges and reduction chunks follow
+      // from the totals, without reading the offsets back
+      if (threadIdx.x == 0) {
+        const int n2 = do2 ? D.nm * D.fs2_parts : 0;
+        const int c3 = (int)((n3 + RED_CHUNK - 1) / RED_CHUNK), c2 = (n2 + RED_CHUNK - 1) / RED_CHUNK;
+        D.box_t3[0] = 0;
+        D.box_t3[1] = (int)n3;
+        D.box_t2[0] = 0;
+        D.box_t2[1] = n2;
+        D.box_c3[0] = 0;
+        D.box_c3[1] = c3;
+        D.box_c2[0] = 0;
+        D.box_c2[1] = c2;
+        T[T_NCH3] = c3;
+        T[T_NCH2] = c2;
+      }
+      return;
     }
     box_ranges(D, do3, do2);
     const int* const c[2] = {D.box_c3, D.box_c2};
