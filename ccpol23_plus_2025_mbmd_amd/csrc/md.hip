@@ -15,12 +15,14 @@
 // is NSUB such sub-steps.  Torques act on j directly: the half kick
 // p += dt/2 * 2 sum_k tau_k P_k q at fixed q is j += dt/2 * tau (body frame).
 //
-// Per step (one launch sequence, replayed from a HIP graph for the forces):
-//   k_md_stage(FIRST):  half kick of v and j, drift of x, NO_SQUISH rotation;
-//   k_md_sites:         lab-frame site offsets R(q) c_s, site forces zeroed;
+// Per step, two launches:
 //   three-body forces:  the recorded evaluation (ccpol3b_graph_launch);
-//   k_md_stage(SECOND): half kick, COM wrapped into the box, kinetic energies;
-//   k_md_record:        E3, Kt, Kr and their sum for the step.
+//   k_md_edge:          second half of the step (half kick, COM wrapped into
+//                       the box, kinetic energies) and its record (E3, Kt, Kr
+//                       and their sum), then the first half of the next step
+//                       (half kick of v and j, drift of x, NO_SQUISH rotation)
+//                       with the lab-frame site offsets R(q) c_s and zeroed
+//                       site forces for the next evaluation.
 // State per molecule: COM (A), quaternion, COM velocity (A/ps) and body-frame
 // angular momentum (amu A^2/ps); forces in 10 J mol^-1 A^-1.
 #include <hip/hip_runtime.h>
@@ -132,7 +134,8 @@ struct MdDev {
   double* en;     // [3] qfs2, qfs3, vir increment
   double* ke;     // [2][nm] translational, rotational kinetic energy per molecule
   double* rec;    // [nsteps][4] E3, Kt, Kr, E3 + Kt + Kr (kcal/mol)
-  int* failed;    // set when a force evaluation wrote nothing (FAIL_LISTS, FAIL_FORCE)
+  int* failed;    // [0] set when a force evaluation wrote nothing (FAIL_LISTS, FAIL_FORCE);
+                  // [1] blocks of k_md_edge done (the last one writes the step record)
   const int* ftot;  // the force evaluation's counters (see graph_device_totals)
   long fcap3;
 };
@@ -199,91 +202,128 @@ __device__ __forceinline__ int evaluation_failure(const MdDev& M) {
   return (M.ftot[T_NBOVER] > 0 || M.ftot[T_N3] > M.fcap3) ? FAIL_LISTS : 0;
 }
 
-enum { STAGE_FIRST = 1, STAGE_SECOND = 2 };
-
-// One half of the velocity-Verlet step for molecule m, with the forces of the
-// last evaluation.  Once an evaluation outgrew its capacities the state stays
-// put for the rest of the run (the host repeats the run with larger lists).
-__global__ void k_md_stage(MdDev M, int stage) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= M.nm || *M.failed) return;
-  if (const int why = evaluation_failure(M)) {
-    if (m == 0) *M.failed = why;
-    return;
-  }
-  Body b = load_body(M, m);
-  double F[3], tb[3];
-  wrench(M, m, b.q, F, tb);
+// Half kick of v and j with the force F and body-frame torque tb.
+__device__ __forceinline__ void half_kick(const MdDev& M, Body& b, const double* F, const double* tb) {
   const double h = 0.5 * M.dt;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     b.v[c] += h * F[c] / M.mass;
     b.j[c] += h * tb[c];
   }
-  if (stage == STAGE_FIRST) {
+}
+
+// Lab-frame site offsets R(q) c_s of molecule m (zero in the padding rows of
+// the ld-row layout) and zeroed site forces.
+__device__ __forceinline__ void write_sites(const MdDev& M, int m, const double* q) {
+  const long plane = (long)M.nm * M.ld, base = (long)m * M.ld;
+  double R[9];
+  rotation_matrix(q, R);
+  for (int st = 0; st < M.ld; ++st) {
+    double lab[3] = {0.0, 0.0, 0.0};
+    if (st < M.ns) {
+      const double* c = M.pc + 3 * st;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) lab[a] = R[3 * a] * c[0] + R[3 * a + 1] * c[1] + R[3 * a + 2] * c[2];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      M.site[a * plane + base + st] = lab[a];
+      M.fsite[a * plane + base + st] = 0.0;
+    }
+  }
+}
+
+// Sites of every molecule (before the first evaluation of a run).
+__global__ void k_md_sites(MdDev M) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M.nm || *M.failed) return;
+  double q[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) q[c] = M.q[c * M.nm + m];
+  write_sites(M, m, q);
+}
+
+// Everything between two force evaluations in one launch, a thread per
+// molecule: with `second`, the second half of step `step` (half kick, COM
+// wrapped into the box, kinetic energies) and, from the last block to finish
+// (a counter in failed[1]), that step's record as fixed-order sums over the
+// molecules; with `first`, the first half of the next step (half kick, drift,
+// NO_SQUISH rotation) and the molecule's sites for the next evaluation.  Both
+// half kicks use the last evaluation's forces at the same orientation.
+__global__ __launch_bounds__(256) void k_md_edge(MdDev M, int second, int step, int first) {
+  __shared__ bool last;
+  __shared__ double part[2][256];
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  // once an evaluation outgrew its capacities the state stays put for the
+  // rest of the run (the host repeats the run with larger lists)
+  bool live = m < M.nm && !*M.failed;
+  if (live) {
+    if (const int why = evaluation_failure(M)) {
+      if (m == 0) *M.failed = why;
+      live = false;
+    }
+  }
+  Body b;
+  double F[3], tb[3];
+  if (live) {
+    b = load_body(M, m);
+    wrench(M, m, b.q, F, tb);
+  }
+  if (second) {
+    if (live) {
+      half_kick(M, b, F, tb);
+      if (M.box > 0.0)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) b.x[c] -= M.box * round(b.x[c] / M.box);  // NINT: half away from zero
+      M.ke[m] = 0.5 * M.mass * (b.v[0] * b.v[0] + b.v[1] * b.v[1] + b.v[2] * b.v[2]);
+      M.ke[M.nm + m] =
+          0.5 * (b.j[0] * b.j[0] / M.pmi[0] + b.j[1] * b.j[1] / M.pmi[1] + b.j[2] * b.j[2] / M.pmi[2]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();  // this block's energies before its count
+      last = atomicAdd(M.failed + 1, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+      // the step's record: fixed-order sums over the molecules
+      __threadfence();  // every other block's energies after their counts
+      double kt = 0.0, kr = 0.0;
+      for (int q = threadIdx.x; q < M.nm; q += 256) {
+        kt += M.ke[q];
+        kr += M.ke[M.nm + q];
+      }
+      part[0][threadIdx.x] = kt;
+      part[1][threadIdx.x] = kr;
+      __syncthreads();
+      for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+          part[0][threadIdx.x] += part[0][threadIdx.x + w];
+          part[1][threadIdx.x] += part[1][threadIdx.x + w];
+        }
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        const double e3 = M.en[0] + M.en[1];
+        const double t = part[0][0] / 418.4, r = part[1][0] / 418.4;
+        double* o = M.rec + 4 * (long)step;
+        o[0] = e3;
+        o[1] = t;
+        o[2] = r;
+        o[3] = e3 + t + r;
+        M.failed[1] = 0;  // for the next launch
+      }
+    }
+  }
+  if (!live) return;
+  if (first) {
+    half_kick(M, b, F, tb);
 #pragma unroll
     for (int c = 0; c < 3; ++c) b.x[c] += M.dt * b.v[c];
     rotate_rigid_body(M.dt, M.pmi, b.q, b.j);
-  } else {
-    if (M.box > 0.0)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) b.x[c] -= M.box * round(b.x[c] / M.box);  // NINT: half away from zero
-    M.ke[m] = 0.5 * M.mass * (b.v[0] * b.v[0] + b.v[1] * b.v[1] + b.v[2] * b.v[2]);
-    M.ke[M.nm + m] = 0.5 * (b.j[0] * b.j[0] / M.pmi[0] + b.j[1] * b.j[1] / M.pmi[1] + b.j[2] * b.j[2] / M.pmi[2]);
+    write_sites(M, m, b.q);  // for the next evaluation
   }
   store_body(M, m, b);
-}
-
-// Lab-frame site offsets R(q) c_s, and zeroed site forces (the evaluation adds to them).
-__global__ void k_md_sites(MdDev M) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)M.nm * M.ld || *M.failed) return;
-  const int m = (int)(t / M.ld), s = (int)(t - (long)m * M.ld);
-  const long plane = (long)M.nm * M.ld;
-  double lab[3] = {0.0, 0.0, 0.0};
-  if (s < M.ns) {
-    const double q[4] = {M.q[m], M.q[M.nm + m], M.q[2 * M.nm + m], M.q[3 * M.nm + m]};
-    double R[9];
-    rotation_matrix(q, R);
-    const double* c = M.pc + 3 * s;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) lab[a] = R[3 * a] * c[0] + R[3 * a + 1] * c[1] + R[3 * a + 2] * c[2];
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    M.site[a * plane + t] = lab[a];
-    M.fsite[a * plane + t] = 0.0;
-  }
-}
-
-// Step record: E3 = qfs2 + qfs3, Kt, Kr (kcal/mol) and their sum; one block,
-// fixed-order sums.
-__global__ __launch_bounds__(256) void k_md_record(MdDev M, int step) {
-  __shared__ double part[2][256];
-  double kt = 0.0, kr = 0.0;
-  for (int m = threadIdx.x; m < M.nm; m += 256) {
-    kt += M.ke[m];
-    kr += M.ke[M.nm + m];
-  }
-  part[0][threadIdx.x] = kt;
-  part[1][threadIdx.x] = kr;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      part[0][threadIdx.x] += part[0][threadIdx.x + w];
-      part[1][threadIdx.x] += part[1][threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double e3 = M.en[0] + M.en[1];
-    const double t = part[0][0] / 418.4, r = part[1][0] / 418.4;
-    double* o = M.rec + 4 * (long)step;
-    o[0] = e3;
-    o[1] = t;
-    o[2] = r;
-    o[3] = e3 + t + r;
-  }
 }
 
 }  // namespace ccpol3b
@@ -342,8 +382,7 @@ int record_forces(ccpol3b_md* md, long version) {
 }
 
 int launch_sites(const MdDev& M) {
-  const long n = (long)M.nm * M.ld;
-  hipLaunchKernelGGL(k_md_sites, dim3((n + 255) / 256), dim3(256), 0, nullptr, M);
+  hipLaunchKernelGGL(k_md_sites, dim3((M.nm + 255) / 256), dim3(256), 0, nullptr, M);
   return hipGetLastError() == hipSuccess ? CCPOL3B_OK : CCPOL3B_EHIP;
 }
 
@@ -402,7 +441,7 @@ int ccpol3b_md_create(ccpol3b_ctx* ctx, int nm, double box, int ns, const double
   if (md_alloc(M.com, n3) || md_alloc(M.q, (size_t)4 * nm) || md_alloc(M.vel, n3) || md_alloc(M.jb, n3) ||
       md_alloc(M.pc, (size_t)3 * ns) || md_alloc(M.site, nsite) || md_alloc(M.fsite, nsite) ||
       md_alloc(M.fc2, n3) || md_alloc(M.fc3, n3) || md_alloc(M.en, 3) || md_alloc(M.ke, (size_t)2 * nm) ||
-      md_alloc(M.failed, 1) || md_alloc(md->saved, (size_t)13 * nm) ||
+      md_alloc(M.failed, 2) || md_alloc(md->saved, (size_t)13 * nm) ||
       hipMemcpy(M.pc, pcoord, sizeof(double) * 3 * ns, hipMemcpyHostToDevice) != hipSuccess) {
     ccpol3b_md_destroy(md);
     return CCPOL3B_ENOMEM;
@@ -483,18 +522,19 @@ int ccpol3b_md_run(ccpol3b_md* md, int nsteps, double dt, double* records) {
   if ((rc = copy_state(md, true))) return rc;
   const dim3 grid((M.nm + 255) / 256), block(256);
   for (int attempt = 0; attempt < 3; ++attempt) {
-    if (hipMemsetAsync(M.failed, 0, sizeof(int), nullptr) != hipSuccess) return CCPOL3B_EHIP;
+    if (hipMemsetAsync(M.failed, 0, 2 * sizeof(int), nullptr) != hipSuccess) return CCPOL3B_EHIP;
     if (!md->graph || md->graph_version != version) {
       // recording evaluates the forces at the current state
       if ((rc = launch_sites(M)) || (rc = record_forces(md, version))) return rc;
     } else if (!md->forces_current) {
       if ((rc = launch_sites(M)) || (rc = ccpol3b_graph_launch(md->graph, nullptr))) return rc;
     }
+    // per step: the force evaluation, then one launch with the second half of
+    // the step, its record and the first half of the next
+    if (nsteps > 0) hipLaunchKernelGGL(k_md_edge, grid, block, 0, nullptr, M, 0, 0, 1);
     for (int s = 0; s < nsteps; ++s) {
-      hipLaunchKernelGGL(k_md_stage, grid, block, 0, nullptr, M, (int)STAGE_FIRST);
-      if ((rc = launch_sites(M)) || (rc = ccpol3b_graph_launch(md->graph, nullptr))) return rc;
-      hipLaunchKernelGGL(k_md_stage, grid, block, 0, nullptr, M, (int)STAGE_SECOND);
-      hipLaunchKernelGGL(k_md_record, dim3(1), dim3(256), 0, nullptr, M, s);
+      if ((rc = ccpol3b_graph_launch(md->graph, nullptr))) return rc;
+      hipLaunchKernelGGL(k_md_edge, grid, block, 0, nullptr, M, 1, s, s + 1 < nsteps ? 1 : 0);
       if (hipGetLastError() != hipSuccess) return CCPOL3B_EHIP;
     }
     int failed = 0;
