@@ -171,29 +171,6 @@ __device__ __forceinline__ void store_body(const MdDev& M, int m, const Body& b)
   for (int c = 0; c < 4; ++c) M.q[c * nm + m] = b.q[c];
 }
 
-// Molecular force (sites + both COM terms) and the body-frame torque about the COM.
-__device__ __forceinline__ void wrench(const MdDev& M, int m, const double* q, double* F, double* tb) {
-  const long plane = (long)M.nm * M.ld, base = (long)m * M.ld;
-  double tl[3] = {0.0, 0.0, 0.0};
-  for (int c = 0; c < 3; ++c) F[c] = M.fc2[c * M.nm + m] + M.fc3[c * M.nm + m];
-  for (int s = 0; s < M.ns; ++s) {
-    double f[3], r[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      f[c] = M.fsite[c * plane + base + s];
-      r[c] = M.site[c * plane + base + s];
-      F[c] += f[c];
-    }
-    tl[0] += r[1] * f[2] - r[2] * f[1];
-    tl[1] += r[2] * f[0] - r[0] * f[2];
-    tl[2] += r[0] * f[1] - r[1] * f[0];
-  }
-  double R[9];
-  rotation_matrix(q, R);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) tb[a] = R[a] * tl[0] + R[3 + a] * tl[1] + R[6 + a] * tl[2];  // R^T tau
-}
-
 // Why the last force evaluation wrote nothing: 0 it did, FAIL_LISTS a list
 // outgrew its capacity, FAIL_FORCE a force left the fixed-point range.
 enum { FAIL_LISTS = 1, FAIL_FORCE = 2 };
@@ -243,31 +220,66 @@ __global__ void k_md_sites(MdDev M) {
   write_sites(M, m, q);
 }
 
-// Everything between two force evaluations in one launch, a thread per
-// molecule: with `second`, the second half of step `step` (half kick, COM
-// wrapped into the box, kinetic energies) and, from the last block to finish
-// (a counter in failed[1]), that step's record as fixed-order sums over the
-// molecules; with `first`, the first half of the next step (half kick, drift,
-// NO_SQUISH rotation) and the molecule's sites for the next evaluation.  Both
-// half kicks use the last evaluation's forces at the same orientation.
+// Everything between two force evaluations in one launch, MD_LANES lanes per
+// molecule (a molecule's sites are consecutive, so its loads and stores of
+// site positions and forces are coalesced): with `second`, the second half of
+// step `step` (half kick, COM wrapped into the box, kinetic energies) and,
+// from the last block to finish (a counter in failed[1]), that step's record
+// as fixed-order sums over the molecules; with `first`, the first half of the
+// next step (half kick, drift, NO_SQUISH rotation) and the molecule's sites
+// for the next evaluation.  Both half kicks use the last evaluation's forces
+// at the same orientation.  Every lane of a molecule advances the same body
+// (the same arithmetic on the same values); its first lane stores it.
+constexpr int MD_LANES = 32;
+constexpr int MD_MOLS = 256 / MD_LANES;  // molecules per block
 __global__ __launch_bounds__(256) void k_md_edge(MdDev M, int second, int step, int first) {
   __shared__ bool last;
   __shared__ double part[2][256];
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = threadIdx.x % MD_LANES;
+  const int m = blockIdx.x * MD_MOLS + threadIdx.x / MD_LANES;
   // once an evaluation outgrew its capacities the state stays put for the
   // rest of the run (the host repeats the run with larger lists)
   bool live = m < M.nm && !*M.failed;
   if (live) {
     if (const int why = evaluation_failure(M)) {
-      if (m == 0) *M.failed = why;
+      if (m == 0 && l == 0) *M.failed = why;
       live = false;
     }
   }
+  const long plane = (long)M.nm * M.ld, base = (long)m * M.ld;
   Body b;
   double F[3], tb[3];
   if (live) {
     b = load_body(M, m);
-    wrench(M, m, b.q, F, tb);
+    // the molecular force and the lab-frame torque: each lane its sites, then
+    // a fixed butterfly over the molecule's lanes
+    double tl[3] = {0.0, 0.0, 0.0};
+    F[0] = F[1] = F[2] = 0.0;
+    for (int st = l; st < M.ns; st += MD_LANES) {
+      double f[3], r[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        f[c] = M.fsite[c * plane + base + st];
+        r[c] = M.site[c * plane + base + st];
+        F[c] += f[c];
+      }
+      tl[0] += r[1] * f[2] - r[2] * f[1];
+      tl[1] += r[2] * f[0] - r[0] * f[2];
+      tl[2] += r[0] * f[1] - r[1] * f[0];
+    }
+#pragma unroll
+    for (int d = MD_LANES / 2; d >= 1; d >>= 1)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        F[c] += __shfl_xor(F[c], d, MD_LANES);
+        tl[c] += __shfl_xor(tl[c], d, MD_LANES);
+      }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) F[c] += M.fc2[c * M.nm + m] + M.fc3[c * M.nm + m];
+    double R[9];
+    rotation_matrix(b.q, R);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) tb[a] = R[a] * tl[0] + R[3 + a] * tl[1] + R[6 + a] * tl[2];  // R^T tau
   }
   if (second) {
     if (live) {
@@ -275,9 +287,11 @@ __global__ __launch_bounds__(256) void k_md_edge(MdDev M, int second, int step, 
       if (M.box > 0.0)
 #pragma unroll
         for (int c = 0; c < 3; ++c) b.x[c] -= M.box * round(b.x[c] / M.box);  // NINT: half away from zero
-      M.ke[m] = 0.5 * M.mass * (b.v[0] * b.v[0] + b.v[1] * b.v[1] + b.v[2] * b.v[2]);
-      M.ke[M.nm + m] =
-          0.5 * (b.j[0] * b.j[0] / M.pmi[0] + b.j[1] * b.j[1] / M.pmi[1] + b.j[2] * b.j[2] / M.pmi[2]);
+      if (l == 0) {
+        M.ke[m] = 0.5 * M.mass * (b.v[0] * b.v[0] + b.v[1] * b.v[1] + b.v[2] * b.v[2]);
+        M.ke[M.nm + m] =
+            0.5 * (b.j[0] * b.j[0] / M.pmi[0] + b.j[1] * b.j[1] / M.pmi[1] + b.j[2] * b.j[2] / M.pmi[2]);
+      }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -321,9 +335,24 @@ __global__ __launch_bounds__(256) void k_md_edge(MdDev M, int second, int step, 
 #pragma unroll
     for (int c = 0; c < 3; ++c) b.x[c] += M.dt * b.v[c];
     rotate_rigid_body(M.dt, M.pmi, b.q, b.j);
-    write_sites(M, m, b.q);  // for the next evaluation
+    // the next evaluation's sites and zeroed site forces, a site per lane
+    double R[9];
+    rotation_matrix(b.q, R);
+    for (int st = l; st < M.ld; st += MD_LANES) {
+      double lab[3] = {0.0, 0.0, 0.0};
+      if (st < M.ns) {
+        const double* c = M.pc + 3 * st;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) lab[a] = R[3 * a] * c[0] + R[3 * a + 1] * c[1] + R[3 * a + 2] * c[2];
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        M.site[a * plane + base + st] = lab[a];
+        M.fsite[a * plane + base + st] = 0.0;
+      }
+    }
   }
-  store_body(M, m, b);
+  if (l == 0) store_body(M, m, b);
 }
 
 }  // namespace ccpol3b
@@ -520,7 +549,7 @@ int ccpol3b_md_run(ccpol3b_md* md, int nsteps, double dt, double* records) {
   }
   const long version = context_version(md->ctx);
   if ((rc = copy_state(md, true))) return rc;
-  const dim3 grid((M.nm + 255) / 256), block(256);
+  const dim3 grid((M.nm + MD_MOLS - 1) / MD_MOLS), block(256);
   for (int attempt = 0; attempt < 3; ++attempt) {
     if (hipMemsetAsync(M.failed, 0, 2 * sizeof(int), nullptr) != hipSuccess) return CCPOL3B_EHIP;
     if (!md->graph || md->graph_version != version) {
