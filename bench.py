@@ -360,7 +360,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the three-body path is "
-              "measured on a single MI355X and shards by triplet-list chunks (see DESIGN.md).", file=sys.stderr)
+              "measured on a single MI355X; across GPUs it would shard by cost-balanced chunks of the triplet "
+              "and FS2 lists, with one all-reduce of the per-molecule sums.", file=sys.stderr)
         sys.exit(2)
     import torch
     from ccpol23_plus_2025_mbmd_amd.geometry import lattice_box

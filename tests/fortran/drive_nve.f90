@@ -1,5 +1,5 @@
 ! A host MD loop in Fortran that calls the drop-in threebody_term (ref_dropin.f90,
-! the INTEGRATION.md snippets) for its forces at every step, as the reference's
+! the documented integration snippets) for its forces at every step, as the reference's
 ! NVE driver calls the original (evolve.f90:877): rigid waters, velocity Verlet
 ! for the centres of mass and, for the rotation, the symplectic splitting of
 ! the free rigid rotor of Dullweber, Leimkuhler and McLachlan (J. Chem. Phys.

@@ -4,7 +4,7 @@
 ! rcut3/rcutsd3, :46 alpha3/delta33, :100 fxs.., :110 vir, :201-218 the 3-body
 ! tables and COM-force/torque arrays), and the two snippets a maintainer adds
 ! (context creation; the new body of threebody_term), copied from
-! INTEGRATION.md line for line (tests/test_fortran_binding.py checks that).
+! the integration guide line for line (tests/test_fortran_binding.py checks that).
 module ref_globals
   use, intrinsic :: iso_c_binding
   implicit none
@@ -31,7 +31,7 @@ contains
 
   subroutine create_context()
     integer(c_int) :: rc
-    ! ---- INTEGRATION.md snippet 1 (context creation), verbatim ----
+    ! ---- integration snippet 1 (context creation), verbatim ----
     tb_par%beta_sr  = c_loc(bS3_s);  tb_par%gamma_sr = c_loc(gS3_s);  tb_par%r0_sr = c_loc(r0S3_s)
     tb_par%beta_lr  = c_loc(bS3_l);  tb_par%gamma_lr = c_loc(gS3_l);  tb_par%r0_lr = c_loc(r0S3_l)
     tb_par%ld_pair  = nsite
@@ -45,7 +45,7 @@ contains
     ! ---- end of snippet 1 ----
   end subroutine create_context
 
-  ! ---- INTEGRATION.md snippet 2 (the new body of threebody_term), verbatim ----
+  ! ---- integration snippet 2 (the new body of threebody_term), verbatim ----
 subroutine threebody_term(xxx,yyy,zzz,xxxs,yyys,zzzs,boxii,qfs2,qfs3)
   real(8) :: xxx(nom),yyy(nom),zzz(nom), xxxs(nsite,nom),yyys(nsite,nom),zzzs(nsite,nom)
   real(8) :: boxii, qfs2, qfs3

@@ -134,15 +134,15 @@ def test_fortran_md_loop_matches_python(tmp_path):
         np.testing.assert_allclose(got, st[key], rtol=1e-9, atol=1e-12 * np.abs(st[key]).max(), err_msg=key)
 
 
-# ------------------------------------------------ the INTEGRATION.md snippets at the reference's shapes
+# ------------------------------------------------ the integration snippets at the reference's shapes
 REF_DRIVER = HERE / "fortran" / "_build" / "drive_ref"
 REF_SOURCE = HERE / "fortran" / "ref_dropin.f90"
 INTEGRATION = HERE.parent / "INTEGRATION.md"
 
 
 def integration_snippets():
-    """The first two ```fortran blocks of INTEGRATION.md (context creation and
-    the new body of threebody_term)."""
+    """The first two ```fortran blocks of the integration guide at the
+    repository root (context creation and the new body of threebody_term)."""
     text = INTEGRATION.read_text()
     blocks, pos = [], 0
     while len(blocks) < 2:
@@ -154,8 +154,8 @@ def integration_snippets():
 
 
 def test_integration_snippets_are_the_tested_program():
-    """Every line of the two snippets a maintainer copies from INTEGRATION.md
-    appears, in the same order, in tests/fortran/ref_dropin.f90 (the drop-in
+    """Every line of the two snippets a maintainer copies from the integration
+    guide appears, in the same order, in tests/fortran/ref_dropin.f90 (the drop-in
     that drive_ref and drive_nve link, which the GPU tests below run against
     the golden boxes and through 1000 MD steps); and the cutoff passed is
     rcut3 (A), not the squared rcutsd3."""
@@ -163,7 +163,7 @@ def test_integration_snippets_are_the_tested_program():
     for block in integration_snippets():
         at = 0
         for line in block:
-            assert line in src[at:], f"INTEGRATION.md line not in ref_dropin.f90 (in order): {line!r}"
+            assert line in src[at:], f"snippet line not in ref_dropin.f90 (in order): {line!r}"
             at = src.index(line, at) + 1
     flat = " ".join(" ".join(b) for b in integration_snippets())
     assert "tb_par%rcut3 = rcut3" in flat and "rcutsd3" not in flat
@@ -173,7 +173,7 @@ def test_integration_snippets_are_the_tested_program():
 @pytest.mark.parametrize("name", ["box64_fs2fs3", "box64_fs3"])
 def test_reference_shapes_drop_in_matches_golden(tmp_path, name):
     """The reference's globals at their declared shapes (nsite = 50, nom = 5000,
-    bS3_s(50,50) read with ld_pair = 50), the INTEGRATION.md snippets, one call
+    bS3_s(50,50) read with ld_pair = 50), the integration snippets, one call
     of threebody_term: the golden 64-water box at the FP64 parity bar."""
     assert REF_DRIVER.exists(), "tests/fortran/_build/drive_ref is built by __graft_entry__.build()"
     with np.load(HERE / "golden" / f"{name}.npz", allow_pickle=False) as f:
@@ -244,7 +244,7 @@ def read_nve(path, nm):
 def test_fortran_nve_loop_1000_steps(tmp_path, oracle):
     """256 waters (L = 19.71 A) from 298 K, FS2+FS3 in FP64: 1000 NVE steps of
     0.5 fs by a Fortran host loop (tests/fortran/drive_nve.f90) that calls the
-    INTEGRATION.md drop-in threebody_term at every step, as the reference's MD
+    documented drop-in threebody_term at every step, as the reference's MD
     loop calls the original.  Every 100 steps: the snapshot's triplet and
     item lists bit-exact with the oracle's, and the energies, virial, site
     forces and FS2 and FS3 COM forces the Fortran loop received within the
