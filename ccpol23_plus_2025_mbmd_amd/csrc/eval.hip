@@ -714,8 +714,8 @@ __global__ __launch_bounds__(256) void k_epilogue(Dev D, int ld, int nsite, int 
 }
 
 // Coefficients come through the vector cache; LDS per block is ~29 KB (FP64,
-// 96 VGPRs: 5 blocks per CU) and ~15 KB (mixed, 58 VGPRs: 8 blocks per CU),
-// neither with spilled registers.  The mixed kernels take the table in the
+// 94 VGPRs, 96 allocated: 5 blocks per CU) and ~15 KB (mixed, 64 VGPRs: 8
+// blocks per CU), neither with spilled registers.  The mixed kernels take the table in the
 // shifted basis (MIXED_VSHIFT), the FP64 kernels the plain one; k_fs3_eval_f64
 // takes it in the interleaved layout (cil_index).
 __global__ __launch_bounds__(256, 5) void k_fs3_eval_f64(Dev D, Consts K, const double4* __restrict__ pairpar,
